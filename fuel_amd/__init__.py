@@ -1,0 +1,12 @@
+"""fuel_amd — MI355X-native exact linear-scan kNN / range / pairwise engine.
+
+A drop-in for fuel's linear-scan search path (SearchIndex knn / radius_search,
+pairwise_distances, LOF and the kNN outlier score) running hand-written HIP
+kernels for gfx950 behind the C ABI in include/fuel_amd.h.
+"""
+from . import _lib
+from .search import SearchIndex, knn_search, pairwise_distances, range_search
+from .outlier import OutlierResult, k_nearest_neighbors_outlier, local_outlier_factor
+
+__all__ = ["SearchIndex", "knn_search", "range_search", "pairwise_distances", "local_outlier_factor",
+           "k_nearest_neighbors_outlier", "OutlierResult"]

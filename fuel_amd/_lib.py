@@ -1,0 +1,281 @@
+"""ctypes binding of libfuel_amd.so (the C ABI declared in include/fuel_amd.h).
+
+The library is loaded from this package directory only; there is no CPU
+fallback: if the library is missing or no GPU is visible, calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfuel_amd.so")
+
+OK = 0
+ERR_INVALID = 1
+ERR_UNSUPPORTED = 2
+ERR_DEVICE = 3
+ERR_INTERRUPTED = 4
+ERR_NOMEM = 5
+
+F32 = 0
+F64 = 1
+EUCLIDEAN = 0
+SQEUCLIDEAN = 1
+EXCLUDE_NONE = 0
+EXCLUDE_FOR_EACH_KNN = 1
+EXCLUDE_PY_LINEAR = 2
+MODE_AUTO = 0
+MODE_EXACT = 1
+MODE_FP32 = 2
+MODE_BF16 = 3
+FORM_SQUARE = 0
+FORM_CONDENSED = 1
+
+MODES = {"auto": MODE_AUTO, "exact": MODE_EXACT, "fp32": MODE_FP32, "bf16": MODE_BF16}
+
+# every symbol include/fuel_amd.h declares
+EXPORTED = [
+    "fuel_amd_version", "fuel_amd_device_count", "fuel_amd_last_error",
+    "fuel_amd_request_cancel", "fuel_amd_clear_cancel",
+    "fuel_amd_index_create", "fuel_amd_index_create_device", "fuel_amd_index_destroy",
+    "fuel_amd_knn", "fuel_amd_knn_device", "fuel_amd_range", "fuel_amd_csr_free",
+    "fuel_amd_pairwise", "fuel_amd_lof", "fuel_amd_knn_outlier", "fuel_amd_last_stats",
+    "fuel_amd_timing_enable", "fuel_amd_timing_read",
+]
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int32), ("widen", ctypes.c_int32), ("device", ctypes.c_int32),
+                ("flags", ctypes.c_int32), ("reserved", ctypes.c_int64 * 6)]
+
+
+class Csr(ctypes.Structure):
+    _fields_ = [("offsets", ctypes.POINTER(ctypes.c_int64)), ("index", ctypes.POINTER(ctypes.c_int64)),
+                ("dist", ctypes.c_void_p), ("nnz", ctypes.c_int64), ("m", ctypes.c_int64)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("queries", ctypes.c_int64), ("fallback_queries", ctypes.c_int64),
+                ("mode_used", ctypes.c_int32), ("splits", ctypes.c_int32), ("kp", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("ms_total", ctypes.c_double)]
+
+
+class FuelAmdError(RuntimeError):
+    """Raised when the C ABI returns an error status."""
+
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load the in-tree library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m fuel_amd._build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    I32 = ctypes.c_int32
+    I64 = ctypes.c_int64
+    lib.fuel_amd_version.restype = ctypes.c_char_p
+    lib.fuel_amd_device_count.restype = I32
+    lib.fuel_amd_last_error.restype = ctypes.c_char_p
+    lib.fuel_amd_index_create.argtypes = [P, I64, I32, I32, I32, ctypes.POINTER(Opts), ctypes.POINTER(P)]
+    lib.fuel_amd_index_create.restype = I32
+    lib.fuel_amd_index_create_device.argtypes = [P, I64, I32, I64, I32, I32, ctypes.POINTER(Opts),
+                                                 ctypes.POINTER(P)]
+    lib.fuel_amd_index_create_device.restype = I32
+    lib.fuel_amd_index_destroy.argtypes = [P]
+    lib.fuel_amd_index_destroy.restype = I32
+    lib.fuel_amd_knn.argtypes = [P, P, I64, I32, I32, P, P]
+    lib.fuel_amd_knn.restype = I32
+    lib.fuel_amd_knn_device.argtypes = [P, P, I64, I64, I32, I32, P, P, P]
+    lib.fuel_amd_knn_device.restype = I32
+    lib.fuel_amd_range.argtypes = [P, P, I64, ctypes.c_double, I32, ctypes.POINTER(Csr)]
+    lib.fuel_amd_range.restype = I32
+    lib.fuel_amd_csr_free.argtypes = [ctypes.POINTER(Csr)]
+    lib.fuel_amd_pairwise.argtypes = [P, I64, I32, I32, I32, I32, P]
+    lib.fuel_amd_pairwise.restype = I32
+    lib.fuel_amd_lof.argtypes = [P, I32, P]
+    lib.fuel_amd_lof.restype = I32
+    lib.fuel_amd_knn_outlier.argtypes = [P, I32, P]
+    lib.fuel_amd_knn_outlier.restype = I32
+    lib.fuel_amd_last_stats.argtypes = [P, ctypes.POINTER(Stats)]
+    lib.fuel_amd_last_stats.restype = I32
+    lib.fuel_amd_timing_enable.argtypes = [P, I32]
+    lib.fuel_amd_timing_enable.restype = I32
+    lib.fuel_amd_timing_read.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(I64),
+                                         ctypes.c_char_p, I32]
+    lib.fuel_amd_timing_read.restype = I32
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != OK:
+        msg = load().fuel_amd_last_error().decode(errors="replace")
+        if rc == ERR_INVALID:
+            raise ValueError(msg)
+        if rc == ERR_INTERRUPTED:
+            raise KeyboardInterrupt(msg)
+        if rc == ERR_UNSUPPORTED:
+            raise NotImplementedError(msg)
+        raise FuelAmdError(rc, msg)
+
+
+def dtype_code(dt):
+    dt = np.dtype(dt)
+    if dt == np.float32:
+        return F32
+    if dt == np.float64:
+        return F64
+    raise ValueError(f"unsupported dtype {dt}")
+
+
+def metric_code(name):
+    """fuel's parse_distance_fn names for the two metrics this engine serves
+    (python/mod.rs:42-71); other names are outside the linear-scan engine."""
+    key = name.lower()
+    if key in ("euclidean", "l2"):
+        return EUCLIDEAN
+    if key in ("sqeuclidean", "squared_euclidean"):
+        return SQEUCLIDEAN
+    known = {"manhattan", "l1", "cityblock", "chebyshev", "linf", "chessboard", "cosine", "arccosine",
+             "angular", "canberra", "braycurtis", "bray_curtis", "hellinger", "clark", "chi", "chi_squared",
+             "chisquared", "chi2", "jensen_shannon", "jensenshannon", "js", "jeffrey", "jeffreys",
+             "histogram_intersection", "intersection"}
+    if key in known:
+        raise NotImplementedError(f"distance '{name}' is not served by the GPU linear-scan engine")
+    raise ValueError(
+        f"unknown distance '{name}', valid options are: euclidean, sqeuclidean, manhattan, "
+        "chebyshev, cosine, arccosine, canberra, braycurtis, hellinger, clark, chi, "
+        "chi_squared, jensen_shannon, jeffrey, histogram_intersection")
+
+
+class Index:
+    """Owner of a fuel_amd_index handle."""
+
+    def __init__(self, data, metric="euclidean", mode="auto", widen=4, device=0):
+        lib = load()
+        data = np.ascontiguousarray(data)
+        if data.ndim != 2:
+            raise ValueError("data must be a 2D array")
+        self.dtype = data.dtype
+        self.n, self.d = data.shape
+        opts = Opts(MODES[mode] if isinstance(mode, str) else int(mode), int(widen), int(device), 0)
+        h = ctypes.c_void_p()
+        check(lib.fuel_amd_index_create(data.ctypes.data, self.n, self.d, dtype_code(data.dtype),
+                                        metric_code(metric), ctypes.byref(opts), ctypes.byref(h)))
+        self.handle = h
+
+    @classmethod
+    def from_device(cls, ptr, n, d, ld, dtype, metric="euclidean", mode="auto", widen=4, device=0):
+        lib = load()
+        self = cls.__new__(cls)
+        self.dtype = np.dtype(dtype)
+        self.n, self.d = n, d
+        opts = Opts(MODES[mode] if isinstance(mode, str) else int(mode), int(widen), int(device), 0)
+        h = ctypes.c_void_p()
+        check(lib.fuel_amd_index_create_device(ctypes.c_void_p(ptr), n, d, ld, dtype_code(dtype),
+                                               metric_code(metric), ctypes.byref(opts), ctypes.byref(h)))
+        self.handle = h
+        return self
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().fuel_amd_index_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def knn(self, query, k, exclude=EXCLUDE_NONE):
+        query = np.ascontiguousarray(query, dtype=self.dtype)
+        if query.ndim != 2 or query.shape[1] != self.d:
+            raise ValueError("query shape mismatch")
+        m = query.shape[0]
+        idx = np.empty((m, k), np.int64)
+        dist = np.empty((m, k), self.dtype)
+        check(load().fuel_amd_knn(self.handle, query.ctypes.data, m, int(k), int(exclude), idx.ctypes.data,
+                                  dist.ctypes.data))
+        return idx, dist
+
+    def knn_device(self, qptr, m, ldq, k, exclude, idx_ptr, dist_ptr, stream=0):
+        check(load().fuel_amd_knn_device(self.handle, ctypes.c_void_p(qptr), m, ldq, int(k), int(exclude),
+                                         ctypes.c_void_p(idx_ptr), ctypes.c_void_p(dist_ptr),
+                                         ctypes.c_void_p(stream)))
+
+    def range(self, query, radius, exclude_self=False):
+        lib = load()
+        query = np.ascontiguousarray(query, dtype=self.dtype)
+        if query.ndim != 2 or query.shape[1] != self.d:
+            raise ValueError("query shape mismatch")
+        m = query.shape[0]
+        c = Csr()
+        check(lib.fuel_amd_range(self.handle, query.ctypes.data, m, float(radius), int(bool(exclude_self)),
+                                 ctypes.byref(c)))
+        try:
+            offs = np.ctypeslib.as_array(c.offsets, shape=(m + 1,)).copy()
+            nnz = int(c.nnz)
+            idx = np.ctypeslib.as_array(c.index, shape=(max(1, nnz),))[:nnz].copy()
+            cty = ctypes.c_float if self.dtype == np.float32 else ctypes.c_double
+            dist = np.ctypeslib.as_array(ctypes.cast(c.dist, ctypes.POINTER(cty)), shape=(max(1, nnz),))[:nnz].copy()
+        finally:
+            lib.fuel_amd_csr_free(ctypes.byref(c))
+        return offs, idx, dist
+
+    def lof(self, k):
+        out = np.empty(self.n, self.dtype)
+        check(load().fuel_amd_lof(self.handle, int(k), out.ctypes.data))
+        return out
+
+    def knn_outlier(self, k):
+        out = np.empty(self.n, self.dtype)
+        check(load().fuel_amd_knn_outlier(self.handle, int(k), out.ctypes.data))
+        return out
+
+    def timing_enable(self, on=True):
+        check(load().fuel_amd_timing_enable(self.handle, int(bool(on))))
+
+    def timing_read(self):
+        """(summed kernel ms, launches, kernel name) since the last read."""
+        ms = ctypes.c_double()
+        cnt = ctypes.c_int64()
+        name = ctypes.create_string_buffer(128)
+        check(load().fuel_amd_timing_read(self.handle, ctypes.byref(ms), ctypes.byref(cnt), name, 128))
+        return ms.value, cnt.value, name.value.decode()
+
+    def stats(self):
+        s = Stats()
+        check(load().fuel_amd_last_stats(self.handle, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in Stats._fields_ if f != "reserved"}
+
+
+def pairwise(data, metric="euclidean", form="square"):
+    data = np.ascontiguousarray(data)
+    n, d = data.shape
+    out = np.empty((n, n) if form == "square" else (n * (n - 1) // 2,), data.dtype)
+    check(load().fuel_amd_pairwise(data.ctypes.data, n, d, dtype_code(data.dtype), metric_code(metric),
+                                   FORM_SQUARE if form == "square" else FORM_CONDENSED, out.ctypes.data))
+    return out
+
+
+def device_count():
+    return int(load().fuel_amd_device_count())
+
+
+def request_cancel():
+    load().fuel_amd_request_cancel()
+
+
+def clear_cancel():
+    load().fuel_amd_clear_cancel()

@@ -1,0 +1,367 @@
+// C-ABI host layer: index lifetime, argument checking, error/cancel conventions,
+// query batching, workspace reuse and kernel orchestration.  Declarations and
+// the reference interface each entry point replaces are in include/fuel_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/fuel_amd.h"
+#include "api_internal.h"
+#include "kernels.h"
+
+using namespace fa;
+
+namespace fa {
+
+thread_local std::string g_last_error;
+std::atomic<bool> g_cancel{false};
+
+int32_t fail(int32_t code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int32_t hip_fail(hipError_t e, const char* what) {
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  if (e == hipErrorOutOfMemory) return fail(FUEL_AMD_ERR_NOMEM, m);
+  return fail(FUEL_AMD_ERR_DEVICE, m);
+}
+
+DevBuf::~DevBuf() { release(); }
+void DevBuf::release() {
+  if (p) hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+hipError_t DevBuf::ensure(size_t want) {
+  if (want <= bytes && p) return hipSuccess;
+  release();
+  size_t b = std::max<size_t>(want, 256);
+  hipError_t e = hipMalloc(&p, b);
+  if (e != hipSuccess) {
+    p = nullptr;
+    (void)hipGetLastError();
+    return e;
+  }
+  bytes = b;
+  return hipSuccess;
+}
+
+int32_t check_device(int dev) {
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) {
+    (void)hipGetLastError();
+    return fail(FUEL_AMD_ERR_DEVICE, "no HIP device available (fuel_amd runs on the GPU only)");
+  }
+  if (dev < 0 || dev >= count) return fail(FUEL_AMD_ERR_INVALID, "device ordinal out of range");
+  e = hipSetDevice(dev);
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  return FUEL_AMD_OK;
+}
+
+int device_cu_count(int dev) {
+  static int cached[64] = {0};
+  if (dev >= 0 && dev < 64 && cached[dev]) return cached[dev];
+  int cu = 256;
+  if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+    (void)hipGetLastError();
+    cu = 256;
+  }
+  if (dev >= 0 && dev < 64) cached[dev] = cu;
+  return cu;
+}
+
+bool cancelled() { return g_cancel.load(std::memory_order_relaxed); }
+
+// Choose the DB split count so that (query blocks x splits) fills the GPU.
+int choose_splits(int64_t qblocks, int64_t n, int cus, int64_t min_rows) {
+  int64_t target = int64_t(cus) * 8;
+  int64_t s = (target + qblocks - 1) / std::max<int64_t>(1, qblocks);
+  int64_t maxs = std::max<int64_t>(1, n / std::max<int64_t>(1, min_rows));
+  s = std::max<int64_t>(1, std::min<int64_t>({s, maxs, 1024}));
+  return (int)s;
+}
+
+void timing_mark(fuel_amd_index* ix, hipStream_t st, bool begin, const char* name) {
+  if (!ix->timing) return;
+  if (ix->ev_used >= ix->ev_pool.size()) return;  // pool exhausted: stop recording
+  if (begin && (ix->ev_used & 1)) return;
+  if (!begin && !(ix->ev_used & 1)) return;
+  if (hipEventRecord(ix->ev_pool[ix->ev_used], st) == hipSuccess) {
+    ++ix->ev_used;
+    if (begin) ix->timed_kernel = name;
+  } else {
+    (void)hipGetLastError();
+  }
+}
+
+}  // namespace fa
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* fuel_amd_version(void) { return "fuel_amd 0.1.0 (gfx950)"; }
+
+int32_t fuel_amd_device_count(void) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return count;
+}
+
+const char* fuel_amd_last_error(void) { return g_last_error.c_str(); }
+void fuel_amd_request_cancel(void) { g_cancel.store(true); }
+void fuel_amd_clear_cancel(void) { g_cancel.store(false); }
+
+static int32_t index_create_common(const void* data, bool on_device, int64_t n, int32_t d, int64_t ld,
+                                   int32_t dtype, int32_t metric, const fuel_amd_opts* opts,
+                                   fuel_amd_index** out) {
+  g_last_error.clear();
+  if (!out) return fail(FUEL_AMD_ERR_INVALID, "out must not be NULL");
+  *out = nullptr;
+  if (n < 0 || d < 0) return fail(FUEL_AMD_ERR_INVALID, "n and d must be non-negative");
+  if (n > 0xFFFFFFFFll - 1) return fail(FUEL_AMD_ERR_UNSUPPORTED, "more than 2^32-2 rows");
+  if (dtype != FUEL_AMD_F32 && dtype != FUEL_AMD_F64) return fail(FUEL_AMD_ERR_INVALID, "dtype must be F32 or F64");
+  if (metric != FUEL_AMD_EUCLIDEAN && metric != FUEL_AMD_SQEUCLIDEAN)
+    return fail(FUEL_AMD_ERR_UNSUPPORTED,
+                "only the 'euclidean' and 'sqeuclidean' distances run on the linear-scan engine");
+  if (n > 0 && !data) return fail(FUEL_AMD_ERR_INVALID, "data must not be NULL");
+  fuel_amd_opts o{};
+  o.mode = FUEL_AMD_MODE_AUTO;
+  o.widen = 4;
+  if (opts) o = *opts;
+  if (o.widen <= 0) o.widen = 4;
+  if (o.mode < FUEL_AMD_MODE_AUTO || o.mode > FUEL_AMD_MODE_BF16) return fail(FUEL_AMD_ERR_INVALID, "bad mode");
+  int32_t rc = check_device(o.device);
+  if (rc) return rc;
+  auto* ix = new (std::nothrow) fuel_amd_index();
+  if (!ix) return fail(FUEL_AMD_ERR_NOMEM, "host allocation failed");
+  ix->device = o.device;
+  ix->dtype = dtype;
+  ix->metric = metric;
+  ix->n = n;
+  ix->d = d;
+  ix->opts = o;
+  const size_t es = dtype == FUEL_AMD_F32 ? 4 : 8;
+  // rows padded to a 32-byte multiple so the exact kernels can use 16-byte loads
+  ix->ld = dtype == FUEL_AMD_F32 ? ((int64_t(d) + 7) & ~int64_t(7)) : ((int64_t(d) + 3) & ~int64_t(3));
+  if (ix->ld == 0) ix->ld = dtype == FUEL_AMD_F32 ? 8 : 4;
+  hipError_t e = hipStreamCreateWithFlags(&ix->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete ix;
+    return hip_fail(e, "hipStreamCreate");
+  }
+  e = ix->X.ensure(size_t(std::max<int64_t>(n, 1)) * ix->ld * es);
+  if (e == hipSuccess) e = hipMemsetAsync(ix->X.p, 0, size_t(std::max<int64_t>(n, 1)) * ix->ld * es, ix->stream);
+  if (e == hipSuccess && n > 0 && d > 0)
+    e = hipMemcpy2DAsync(ix->X.p, ix->ld * es, data, (on_device ? ld : d) * es, size_t(d) * es, (size_t)n,
+                         on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ix->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ix->stream);
+  if (e != hipSuccess) {
+    int32_t r = hip_fail(e, "index upload");
+    fuel_amd_index_destroy(ix);
+    return r;
+  }
+  *out = ix;
+  return FUEL_AMD_OK;
+}
+
+int32_t fuel_amd_index_create(const void* data, int64_t n, int32_t d, int32_t dtype, int32_t metric,
+                              const fuel_amd_opts* opts, fuel_amd_index** out) {
+  return index_create_common(data, false, n, d, d, dtype, metric, opts, out);
+}
+
+int32_t fuel_amd_index_create_device(const void* d_data, int64_t n, int32_t d, int64_t ld, int32_t dtype,
+                                     int32_t metric, const fuel_amd_opts* opts, fuel_amd_index** out) {
+  if (ld < d) return fail(FUEL_AMD_ERR_INVALID, "ld must be >= d");
+  return index_create_common(d_data, true, n, d, ld, dtype, metric, opts, out);
+}
+
+int32_t fuel_amd_index_destroy(fuel_amd_index* ix) {
+  if (!ix) return FUEL_AMD_OK;
+  (void)hipSetDevice(ix->device);
+  if (ix->stream) hipStreamSynchronize(ix->stream);
+  ix->release_all();
+  for (auto& ev : ix->ev_pool) hipEventDestroy(ev);
+  ix->ev_pool.clear();
+  if (ix->stream) hipStreamDestroy(ix->stream);
+  delete ix;
+  return FUEL_AMD_OK;
+}
+
+int32_t fuel_amd_timing_enable(fuel_amd_index* ix, int32_t enable) {
+  if (!ix) return fail(FUEL_AMD_ERR_INVALID, "index must not be NULL");
+  std::lock_guard<std::mutex> lock(ix->mu);
+  int32_t rc = check_device(ix->device);
+  if (rc) return rc;
+  if (enable && ix->ev_pool.empty()) {
+    ix->ev_pool.resize(4096);
+    for (auto& ev : ix->ev_pool) {
+      hipError_t e = hipEventCreate(&ev);
+      if (e != hipSuccess) return hip_fail(e, "hipEventCreate");
+    }
+  }
+  ix->timing = enable != 0;
+  ix->ev_used = 0;
+  return FUEL_AMD_OK;
+}
+
+int32_t fuel_amd_timing_read(fuel_amd_index* ix, double* kernel_ms, int64_t* launches, char* name,
+                             int32_t name_len) {
+  if (!ix || !kernel_ms || !launches) return fail(FUEL_AMD_ERR_INVALID, "NULL argument");
+  std::lock_guard<std::mutex> lock(ix->mu);
+  int32_t rc = check_device(ix->device);
+  if (rc) return rc;
+  double total = 0.0;
+  int64_t cnt = 0;
+  for (size_t i = 0; i + 1 < ix->ev_used; i += 2) {
+    hipError_t e = hipEventSynchronize(ix->ev_pool[i + 1]);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ix->ev_pool[i], ix->ev_pool[i + 1]);
+    if (e != hipSuccess) return hip_fail(e, "event timing");
+    total += ms;
+    ++cnt;
+  }
+  *kernel_ms = total;
+  *launches = cnt;
+  if (name && name_len > 0) {
+    std::strncpy(name, ix->timed_kernel.c_str(), (size_t)name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  ix->ev_used = 0;
+  return FUEL_AMD_OK;
+}
+
+int32_t fuel_amd_last_stats(const fuel_amd_index* ix, fuel_amd_stats* out) {
+  if (!ix || !out) return fail(FUEL_AMD_ERR_INVALID, "NULL argument");
+  *out = ix->stats;
+  return FUEL_AMD_OK;
+}
+
+// ---- kNN --------------------------------------------------------------------
+
+int32_t fuel_amd_knn_device(fuel_amd_index* ix, const void* d_query, int64_t m, int64_t ldq, int32_t k,
+                            int32_t exclude_self, int64_t* d_out_idx, void* d_out_dist, void* stream) {
+  g_last_error.clear();
+  if (!ix) return fail(FUEL_AMD_ERR_INVALID, "index must not be NULL");
+  if (m < 0 || k < 0) return fail(FUEL_AMD_ERR_INVALID, "m and k must be non-negative");
+  if (exclude_self < 0 || exclude_self > 2) return fail(FUEL_AMD_ERR_INVALID, "bad exclude_self");
+  if (ldq < ix->d) return fail(FUEL_AMD_ERR_INVALID, "ldq must be >= d");
+  std::lock_guard<std::mutex> lock(ix->mu);
+  int32_t rc = check_device(ix->device);
+  if (rc) return rc;
+  hipStream_t st = stream ? (hipStream_t)stream : ix->stream;
+  return knn_run(ix, d_query, m, ldq, k, exclude_self, d_out_idx, d_out_dist, st);
+}
+
+int32_t fuel_amd_knn(fuel_amd_index* ix, const void* query, int64_t m, int32_t k, int32_t exclude_self,
+                     int64_t* out_idx, void* out_dist) {
+  g_last_error.clear();
+  if (!ix) return fail(FUEL_AMD_ERR_INVALID, "index must not be NULL");
+  if (m < 0 || k < 0) return fail(FUEL_AMD_ERR_INVALID, "m and k must be non-negative");
+  if (exclude_self < 0 || exclude_self > 2) return fail(FUEL_AMD_ERR_INVALID, "bad exclude_self");
+  if (m == 0 || k == 0) return FUEL_AMD_OK;
+  if (!query || !out_idx || !out_dist) return fail(FUEL_AMD_ERR_INVALID, "NULL buffer");
+  std::lock_guard<std::mutex> lock(ix->mu);
+  int32_t rc = check_device(ix->device);
+  if (rc) return rc;
+  const size_t es = ix->dtype == FUEL_AMD_F32 ? 4 : 8;
+  hipStream_t st = ix->stream;
+  hipError_t e = ix->hq.ensure(size_t(m) * ix->ld * es);
+  if (e == hipSuccess) e = ix->hout_idx.ensure(size_t(m) * k * 8);
+  if (e == hipSuccess) e = ix->hout_dist.ensure(size_t(m) * k * es);
+  if (e != hipSuccess) return hip_fail(e, "query buffers");
+  if (ix->d > 0) {
+    e = hipMemcpy2DAsync(ix->hq.p, ix->ld * es, query, size_t(ix->d) * es, size_t(ix->d) * es, (size_t)m,
+                         hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "query upload");
+  }
+  rc = knn_run(ix, ix->hq.p, m, ix->ld, k, exclude_self, (int64_t*)ix->hout_idx.p, ix->hout_dist.p, st);
+  if (rc) return rc;
+  e = hipMemcpyAsync(out_idx, ix->hout_idx.p, size_t(m) * k * 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(out_dist, ix->hout_dist.p, size_t(m) * k * es, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, "result download");
+  return FUEL_AMD_OK;
+}
+
+// ---- range ------------------------------------------------------------------
+
+void fuel_amd_csr_free(fuel_amd_csr* c) {
+  if (!c) return;
+  std::free(c->offsets);
+  std::free(c->index);
+  std::free(c->dist);
+  c->offsets = nullptr;
+  c->index = nullptr;
+  c->dist = nullptr;
+  c->nnz = 0;
+  c->m = 0;
+}
+
+int32_t fuel_amd_range(fuel_amd_index* ix, const void* query, int64_t m, double radius, int32_t exclude_self,
+                       fuel_amd_csr* out) {
+  g_last_error.clear();
+  if (!ix || !out) return fail(FUEL_AMD_ERR_INVALID, "NULL argument");
+  std::memset(out, 0, sizeof(*out));
+  if (m < 0) return fail(FUEL_AMD_ERR_INVALID, "m must be non-negative");
+  if (m > 0 && !query) return fail(FUEL_AMD_ERR_INVALID, "query must not be NULL");
+  std::lock_guard<std::mutex> lock(ix->mu);
+  int32_t rc = check_device(ix->device);
+  if (rc) return rc;
+  return range_run(ix, query, m, radius, exclude_self, out);
+}
+
+// ---- pairwise ---------------------------------------------------------------
+
+int32_t fuel_amd_pairwise(const void* data, int64_t n, int32_t d, int32_t dtype, int32_t metric, int32_t form,
+                          void* out) {
+  g_last_error.clear();
+  if (form != FUEL_AMD_FORM_SQUARE && form != FUEL_AMD_FORM_CONDENSED)
+    return fail(FUEL_AMD_ERR_INVALID, "form must be SQUARE or CONDENSED");
+  fuel_amd_index* ix = nullptr;
+  int32_t rc = fuel_amd_index_create(data, n, d, dtype, metric, nullptr, &ix);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lock(ix->mu);
+    rc = pairwise_run(ix, form, out);
+  }
+  fuel_amd_index_destroy(ix);
+  return rc;
+}
+
+// ---- outlier scores -----------------------------------------------------------
+
+int32_t fuel_amd_lof(fuel_amd_index* ix, int32_t k, void* out_scores) {
+  g_last_error.clear();
+  if (!ix || (!out_scores && ix->n > 0)) return fail(FUEL_AMD_ERR_INVALID, "NULL argument");
+  if (k <= 0) return fail(FUEL_AMD_ERR_INVALID, "k must be greater than 0");  // src/outlier/lof.rs:22
+  std::lock_guard<std::mutex> lock(ix->mu);
+  int32_t rc = check_device(ix->device);
+  if (rc) return rc;
+  return outlier_run(ix, k, 0, out_scores);
+}
+
+int32_t fuel_amd_knn_outlier(fuel_amd_index* ix, int32_t k, void* out_scores) {
+  g_last_error.clear();
+  if (!ix || (!out_scores && ix->n > 0)) return fail(FUEL_AMD_ERR_INVALID, "NULL argument");
+  if (k <= 0) return fail(FUEL_AMD_ERR_INVALID, "k must be greater than 0");  // src/outlier/knn.rs:22
+  std::lock_guard<std::mutex> lock(ix->mu);
+  int32_t rc = check_device(ix->device);
+  if (rc) return rc;
+  return outlier_run(ix, k, 1, out_scores);
+}
+
+}  // extern "C"

@@ -1,0 +1,85 @@
+// Host-side internals shared by the C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/fuel_amd.h"
+
+namespace fa {
+
+// A grow-only device allocation reused across calls (no allocation in steady state).
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf();
+  void release();
+  hipError_t ensure(size_t want);
+  template <typename T>
+  T* as() const {
+    return reinterpret_cast<T*>(p);
+  }
+};
+
+int32_t fail(int32_t code, const std::string& msg);
+int32_t hip_fail(hipError_t e, const char* what);
+int32_t check_device(int dev);
+int device_cu_count(int dev);
+bool cancelled();
+int choose_splits(int64_t qblocks, int64_t n, int cus, int64_t min_rows);
+
+}  // namespace fa
+
+struct fuel_amd_index {
+  int device = 0;
+  int dtype = 0;
+  int metric = 0;
+  int64_t n = 0;
+  int d = 0;
+  int64_t ld = 0;  // padded row stride of the resident copy (elements)
+  fuel_amd_opts opts{};
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  fa::DevBuf X;                          // resident rows (n x ld)
+  fa::DevBuf hq, hout_idx, hout_dist;    // staging for host-pointer calls
+  fa::DevBuf slots, lcnts, fail_list;    // scan workspaces
+  fa::DevBuf ws_a, ws_b, ws_c, ws_d, ws_e, ws_tmp;  // range / pairwise / outlier workspaces
+  fuel_amd_stats stats{};
+  // profiling: event pairs around the dominant kernel of each call
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::string timed_kernel;
+  void release_all() {
+    X.release();
+    hq.release();
+    hout_idx.release();
+    hout_dist.release();
+    slots.release();
+    lcnts.release();
+    fail_list.release();
+    ws_a.release();
+    ws_b.release();
+    ws_c.release();
+    ws_d.release();
+    ws_e.release();
+    ws_tmp.release();
+  }
+};
+
+namespace fa {
+int32_t knn_run(fuel_amd_index* ix, const void* d_query, int64_t m, int64_t ldq, int k, int exclude,
+                int64_t* d_out_idx, void* d_out_dist, hipStream_t st);
+int32_t range_run(fuel_amd_index* ix, const void* query, int64_t m, double radius, int exclude_self,
+                  fuel_amd_csr* out);
+int32_t pairwise_run(fuel_amd_index* ix, int form, void* out);
+int32_t outlier_run(fuel_amd_index* ix, int k, int knn_outlier, void* out_scores);
+// Record the start (begin=true) or end of a timed kernel on `st` (no-op unless enabled).
+void timing_mark(fuel_amd_index* ix, hipStream_t st, bool begin, const char* name);
+}  // namespace fa

@@ -1,0 +1,278 @@
+// Orchestration of the exact kNN, range, pairwise and outlier paths on one
+// index: workspace sizing, query batching, split choice and launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "api_internal.h"
+#include "kernels.h"
+
+namespace fa {
+
+namespace {
+
+constexpr int EXACT_V = 8;  // slot = 512 composites
+constexpr int EXACT_NS = 64 * EXACT_V;
+constexpr size_t SLOT_BUDGET = size_t(1) << 30;  // bytes of candidate slots per batch
+
+int exact_qw(int64_t nq) { return nq >= 64 ? 16 : (nq >= 8 ? 8 : 4); }
+
+template <typename F>
+int32_t knn_exact(fuel_amd_index* ix, const F* Q, int64_t m, int64_t ldq, int k, int mode, int64_t* out_idx,
+                  F* out_dist, hipStream_t st) {
+  using T = typename Comp<F>::T;
+  const int KK = k + (mode != EXCL_NONE ? 1 : 0);
+  if (KK > EXACT_NS / 2)
+    return fail(FUEL_AMD_ERR_UNSUPPORTED, "k above " + std::to_string(EXACT_NS / 2 - 1) + " is not supported");
+  const int cus = device_cu_count(ix->device);
+  int64_t done = 0;
+  while (done < m) {
+    if (cancelled()) return fail(FUEL_AMD_ERR_INTERRUPTED, "interrupted");
+    int64_t chunk = m - done;
+    const int qw = exact_qw(chunk);
+    int nsplit = choose_splits((chunk + qw - 1) / qw, ix->n, cus, 512);
+    const size_t per_q = size_t(nsplit) * EXACT_NS * sizeof(T);
+    chunk = std::min<int64_t>(chunk, std::max<int64_t>(qw, int64_t(SLOT_BUDGET / per_q)));
+    hipError_t e = ix->slots.ensure(per_q * chunk);
+    if (e == hipSuccess) e = ix->lcnts.ensure(sizeof(int) * size_t(nsplit) * chunk);
+    if (e != hipSuccess) return hip_fail(e, "slot workspace");
+    ScanExactArgs<F> sa{};
+    sa.X = ix->X.as<F>();
+    sa.ldx = ix->ld;
+    sa.n = ix->n;
+    sa.d = ix->d;
+    sa.Q = Q + done * ldq;
+    sa.ldq = ldq;
+    sa.qrows = nullptr;
+    sa.nq = chunk;
+    sa.metric = ix->metric;
+    sa.KP = KK;
+    int64_t rps = (ix->n + nsplit - 1) / nsplit;
+    rps = std::max<int64_t>(256, (rps + 255) & ~int64_t(255));
+    sa.rows_per_split = rps;
+    nsplit = (int)std::max<int64_t>(1, (ix->n + rps - 1) / rps);
+    sa.slots = ix->slots.as<T>();
+    sa.lcnts = ix->lcnts.as<int>();
+    timing_mark(ix, st, true, "scan_exact_kernel");
+    e = launch_scan_exact<F>(sa, qw, nsplit, st);
+    timing_mark(ix, st, false, nullptr);
+    if (e != hipSuccess) return hip_fail(e, "scan_exact launch");
+    FinalizeArgs<F> fa_{};
+    fa_.slots = sa.slots;
+    fa_.lcnts = sa.lcnts;
+    fa_.slot_stride = EXACT_NS;
+    fa_.nsplit = nsplit;
+    fa_.nq = chunk;
+    fa_.KK = KK;
+    fa_.k = k;
+    fa_.mode = mode;
+    fa_.qrows = nullptr;
+    fa_.qbase = done;
+    fa_.out_idx = out_idx + done * k;
+    fa_.out_dist = out_dist + done * k;
+    e = launch_finalize_exact<F>(fa_, st);
+    if (e != hipSuccess) return hip_fail(e, "finalize launch");
+    ix->stats.splits = nsplit;
+    ix->stats.kp = KK;
+    done += chunk;
+  }
+  return FUEL_AMD_OK;
+}
+
+template <typename F>
+int32_t range_impl(fuel_amd_index* ix, const void* query, int64_t m, double radius, int exclude_self,
+                   fuel_amd_csr* out) {
+  using K = typename OrdKey<F>::T;
+  hipStream_t st = ix->stream;
+  const size_t es = sizeof(F);
+  out->m = m;
+  out->offsets = (int64_t*)std::malloc(sizeof(int64_t) * (m + 1));
+  if (!out->offsets) return fail(FUEL_AMD_ERR_NOMEM, "host allocation failed");
+  if (m == 0) {
+    out->offsets[0] = 0;
+    return FUEL_AMD_OK;
+  }
+  const int QW = 16;
+  const int cus = device_cu_count(ix->device);
+  int nsplit = choose_splits((m + QW - 1) / QW, ix->n, cus, 1024);
+  int64_t rps = (ix->n + nsplit - 1) / nsplit;
+  rps = std::max<int64_t>(256, (rps + 255) & ~int64_t(255));
+  nsplit = (int)std::max<int64_t>(1, (ix->n + rps - 1) / rps);
+  const size_t ncs = size_t(m) * nsplit;
+  hipError_t e = ix->hq.ensure(size_t(m) * ix->ld * es);
+  if (e == hipSuccess) e = ix->ws_a.ensure(sizeof(int64_t) * (ncs + 1));  // counts
+  if (e == hipSuccess) e = ix->ws_b.ensure(sizeof(int64_t) * (ncs + 1));  // offsets
+  if (e == hipSuccess) e = ix->ws_c.ensure(sizeof(int64_t) * (m + 1));    // per-query offsets
+  if (e != hipSuccess) return hip_fail(e, "range workspace");
+  if (ix->d > 0) {
+    e = hipMemcpy2DAsync(ix->hq.p, ix->ld * es, query, ix->d * es, ix->d * es, (size_t)m, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "query upload");
+  }
+  RangeArgs<F> ra{};
+  ra.X = ix->X.as<F>();
+  ra.ldx = ix->ld;
+  ra.n = ix->n;
+  ra.d = ix->d;
+  ra.Q = ix->hq.as<F>();
+  ra.ldq = ix->ld;
+  ra.nq = m;
+  ra.qbase = 0;
+  ra.metric = ix->metric;
+  ra.radius = (F)radius;  // python/search.rs:785 (`radius as f32`)
+  ra.exclude_self = exclude_self ? 1 : 0;
+  ra.nsplit = nsplit;
+  ra.rows_per_split = rps;
+  ra.counts = ix->ws_a.as<int64_t>();
+  e = launch_range<F>(ra, 0, nsplit, st);
+  if (e != hipSuccess) return hip_fail(e, "range count launch");
+  size_t tmp_bytes = 0;
+  e = scan_exclusive_i64(ra.counts, ix->ws_b.as<int64_t>(), (int64_t)ncs, nullptr, tmp_bytes, st);
+  if (e == hipSuccess) e = ix->ws_tmp.ensure(tmp_bytes);
+  if (e == hipSuccess)
+    e = scan_exclusive_i64(ra.counts, ix->ws_b.as<int64_t>(), (int64_t)ncs, ix->ws_tmp.p, tmp_bytes, st);
+  int64_t tail[2] = {0, 0};
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&tail[0], ix->ws_b.as<int64_t>() + ncs - 1, 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(&tail[1], ra.counts + ncs - 1, 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, "range count");
+  const int64_t nnz = tail[0] + tail[1];
+  out->nnz = nnz;
+  out->index = (int64_t*)std::malloc(sizeof(int64_t) * std::max<int64_t>(1, nnz));
+  out->dist = std::malloc(es * std::max<int64_t>(1, nnz));
+  if (!out->index || !out->dist) return fail(FUEL_AMD_ERR_NOMEM, "host allocation failed");
+  e = launch_gather_offsets(ix->ws_b.as<int64_t>(), m, nsplit, nnz, ix->ws_c.as<int64_t>(), st);
+  if (e != hipSuccess) return hip_fail(e, "range offsets");
+  if (nnz > 0) {
+    if (cancelled()) return fail(FUEL_AMD_ERR_INTERRUPTED, "interrupted");
+    if (nnz > int64_t(0xFFFFFFFFll)) return fail(FUEL_AMD_ERR_UNSUPPORTED, "more than 2^32 range results");
+    // keys, idx (unsorted) then sorted
+    e = ix->ws_d.ensure((sizeof(K) + 8) * size_t(nnz));
+    if (e == hipSuccess) e = ix->ws_e.ensure((sizeof(K) + 8) * size_t(nnz));
+    if (e != hipSuccess) return hip_fail(e, "range result workspace");
+    K* k_in = ix->ws_d.as<K>();
+    int64_t* i_in = reinterpret_cast<int64_t*>(ix->ws_d.as<char>() + sizeof(K) * nnz + (8 - (sizeof(K) * nnz) % 8) % 8);
+    K* k_out = ix->ws_e.as<K>();
+    int64_t* i_out = reinterpret_cast<int64_t*>(ix->ws_e.as<char>() + sizeof(K) * nnz + (8 - (sizeof(K) * nnz) % 8) % 8);
+    // (buffers are sized with 8 bytes of slack for the alignment above)
+    ra.offsets = ix->ws_b.as<int64_t>();
+    ra.out_keys = k_in;
+    ra.out_idx = i_in;
+    e = launch_range<F>(ra, 1, nsplit, st);
+    if (e != hipSuccess) return hip_fail(e, "range fill launch");
+    tmp_bytes = 0;
+    e = segmented_sort_pairs<K>(k_in, k_out, i_in, i_out, nnz, m, ix->ws_c.as<int64_t>(), nullptr, tmp_bytes, st);
+    if (e == hipSuccess) e = ix->ws_tmp.ensure(tmp_bytes);
+    if (e == hipSuccess)
+      e = segmented_sort_pairs<K>(k_in, k_out, i_in, i_out, nnz, m, ix->ws_c.as<int64_t>(), ix->ws_tmp.p, tmp_bytes,
+                                  st);
+    if (e == hipSuccess) e = launch_unord<F>(k_out, nnz, reinterpret_cast<F*>(k_in), st);
+    if (e == hipSuccess) e = hipMemcpyAsync(out->index, i_out, 8 * size_t(nnz), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(out->dist, k_in, es * size_t(nnz), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return hip_fail(e, "range sort");
+  }
+  e = hipMemcpyAsync(out->offsets, ix->ws_c.p, sizeof(int64_t) * (m + 1), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, "range download");
+  return FUEL_AMD_OK;
+}
+
+template <typename F>
+int32_t pairwise_impl(fuel_amd_index* ix, int form, void* out) {
+  hipStream_t st = ix->stream;
+  const int64_t n = ix->n;
+  const size_t count = form == 0 ? size_t(n) * n : (n >= 2 ? size_t(n) * (n - 1) / 2 : 0);
+  if (count == 0) return FUEL_AMD_OK;
+  hipError_t e = ix->ws_a.ensure(count * sizeof(F));
+  if (e != hipSuccess) return hip_fail(e, "pairwise output");
+  const int cus = device_cu_count(ix->device);
+  int nsplit = choose_splits((n + 15) / 16, n, cus, 1024);
+  int64_t rps = (n + nsplit - 1) / nsplit;
+  rps = std::max<int64_t>(256, (rps + 255) & ~int64_t(255));
+  nsplit = (int)std::max<int64_t>(1, (n + rps - 1) / rps);
+  PairwiseArgs<F> pa{};
+  pa.X = ix->X.as<F>();
+  pa.ldx = ix->ld;
+  pa.n = n;
+  pa.d = ix->d;
+  pa.metric = ix->metric;
+  pa.rows_per_split = rps;
+  pa.out = ix->ws_a.as<F>();
+  e = launch_pairwise<F>(pa, form, nsplit, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, pa.out, count * sizeof(F), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, "pairwise");
+  return FUEL_AMD_OK;
+}
+
+template <typename F>
+int32_t outlier_impl(fuel_amd_index* ix, int k, int knn_outlier, void* out_scores) {
+  hipStream_t st = ix->stream;
+  const int64_t n = ix->n;
+  if (n == 0) return FUEL_AMD_OK;
+  const int k_eff = (int)std::min<int64_t>(k, n - 1);  // src/outlier/lof.rs:25
+  hipError_t e = ix->ws_a.ensure(8 * size_t(n) * std::max(1, k_eff));
+  if (e == hipSuccess) e = ix->ws_b.ensure(sizeof(F) * size_t(n) * std::max(1, k_eff));
+  if (e == hipSuccess) e = ix->ws_c.ensure(sizeof(F) * size_t(n) * 3);
+  if (e != hipSuccess) return hip_fail(e, "outlier workspace");
+  if (k_eff > 0) {
+    int32_t rc = knn_run(ix, ix->X.p, n, ix->ld, k_eff, EXCL_FOR_EACH_KNN, ix->ws_a.as<int64_t>(), ix->ws_b.p, st);
+    if (rc) return rc;
+  }
+  LofArgs<F> la{};
+  la.nb_idx = ix->ws_a.as<int64_t>();
+  la.nb_dist = ix->ws_b.as<F>();
+  la.n = n;
+  la.k = k_eff;
+  la.kdist = ix->ws_c.as<F>();
+  la.lrd = la.kdist + n;
+  la.out = la.lrd + n;
+  e = launch_lof<F>(la, knn_outlier, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(out_scores, la.out, sizeof(F) * n, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, "outlier scores");
+  return FUEL_AMD_OK;
+}
+
+}  // namespace
+
+int32_t knn_run(fuel_amd_index* ix, const void* d_query, int64_t m, int64_t ldq, int k, int exclude,
+                int64_t* d_out_idx, void* d_out_dist, hipStream_t st) {
+  if (m == 0 || k == 0) return FUEL_AMD_OK;
+  if (!d_query || !d_out_idx || !d_out_dist) return fail(FUEL_AMD_ERR_INVALID, "NULL buffer");
+  auto t0 = std::chrono::steady_clock::now();
+  ix->stats = fuel_amd_stats{};
+  ix->stats.queries = m;
+  ix->stats.mode_used = FUEL_AMD_MODE_EXACT;
+  int32_t rc;
+  if (ix->dtype == FUEL_AMD_F32)
+    rc = knn_exact<float>(ix, (const float*)d_query, m, ldq, k, exclude, d_out_idx, (float*)d_out_dist, st);
+  else
+    rc = knn_exact<double>(ix, (const double*)d_query, m, ldq, k, exclude, d_out_idx, (double*)d_out_dist, st);
+  ix->stats.ms_total =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return rc;
+}
+
+int32_t range_run(fuel_amd_index* ix, const void* query, int64_t m, double radius, int exclude_self,
+                  fuel_amd_csr* out) {
+  int32_t rc = ix->dtype == FUEL_AMD_F32 ? range_impl<float>(ix, query, m, radius, exclude_self, out)
+                                         : range_impl<double>(ix, query, m, radius, exclude_self, out);
+  if (rc) fuel_amd_csr_free(out);
+  return rc;
+}
+
+int32_t pairwise_run(fuel_amd_index* ix, int form, void* out) {
+  return ix->dtype == FUEL_AMD_F32 ? pairwise_impl<float>(ix, form, out) : pairwise_impl<double>(ix, form, out);
+}
+
+int32_t outlier_run(fuel_amd_index* ix, int k, int knn_outlier, void* out_scores) {
+  return ix->dtype == FUEL_AMD_F32 ? outlier_impl<float>(ix, k, knn_outlier, out_scores)
+                                   : outlier_impl<double>(ix, k, knn_outlier, out_scores);
+}
+
+}  // namespace fa

@@ -1,0 +1,179 @@
+// Kernel argument blocks and launchers shared between the HIP translation units
+// and the C-ABI host code.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.h"
+
+namespace fa {
+
+// Exclusion semantics for self-queries (query slot q is data row qrow):
+//   0 none; 1 fuel's for_each_knn (k+1, drop self, truncate: src/outlier/common.rs:27,41-44);
+//   2 the Python linear-scan path (k, drop self: python/search.rs:485-488).
+enum ExcludeMode { EXCL_NONE = 0, EXCL_FOR_EACH_KNN = 1, EXCL_PY_LINEAR = 2 };
+
+template <typename F>
+struct ScanExactArgs {
+  const F* X;
+  int64_t ldx;
+  int64_t n;
+  int d;
+  const F* Q;
+  int64_t ldq;
+  const int64_t* qrows;  // slot -> query row (nullptr: identity)
+  int64_t nq;            // number of query slots
+  int metric;            // 0 euclidean, 1 sqeuclidean
+  int KP;
+  int64_t rows_per_split;
+  typename Comp<F>::T* slots;  // [nsplit][nq][NS]
+  int* lcnts;                  // [nsplit][nq]
+};
+
+// Order-preserving unsigned keys of a distance (u32 for f32, u64 for f64).
+template <typename F>
+struct OrdKey;
+template <>
+struct OrdKey<float> {
+  using T = uint32_t;
+};
+template <>
+struct OrdKey<double> {
+  using T = uint64_t;
+};
+__device__ __forceinline__ uint32_t ord_key(float f) { return ord_f32(f); }
+__device__ __forceinline__ uint64_t ord_key(double f) { return ord_f64(f); }
+template <typename F>
+__device__ __forceinline__ F unord_key(typename OrdKey<F>::T k);
+template <>
+__device__ __forceinline__ float unord_key<float>(uint32_t k) { return unord_f32(k); }
+template <>
+__device__ __forceinline__ double unord_key<double>(uint64_t k) { return unord_f64(k); }
+
+template <typename F>
+struct RangeArgs {
+  const F* X;
+  int64_t ldx;
+  int64_t n;
+  int d;
+  const F* Q;
+  int64_t ldq;
+  int64_t nq;
+  int64_t qbase;  // data row of query slot 0 (self-exclusion)
+  int metric;
+  F radius;
+  int exclude_self;
+  int nsplit;
+  int64_t rows_per_split;
+  int64_t* counts;         // pass 0: [nq][nsplit]
+  const int64_t* offsets;  // pass 1: exclusive scan of counts
+  typename OrdKey<F>::T* out_keys;
+  int64_t* out_idx;
+};
+
+template <typename F>
+struct PairwiseArgs {
+  const F* X;
+  int64_t ldx;
+  int64_t n;
+  int d;
+  int metric;
+  int64_t rows_per_split;
+  F* out;
+};
+
+template <typename F>
+struct LofArgs {
+  const int64_t* nb_idx;  // n x k (for_each_knn semantics, no padding)
+  const F* nb_dist;
+  int64_t n;
+  int k;
+  F* kdist;
+  F* lrd;
+  F* out;
+};
+
+template <typename F>
+struct FinalizeArgs {
+  const typename Comp<F>::T* slots;
+  const int* lcnts;
+  int64_t slot_stride;  // NS of the scan that produced the slots
+  int nsplit;
+  int64_t nq;
+  int KK;
+  int k;
+  int mode;
+  const int64_t* qrows;  // slot -> query row within this batch (nullptr: identity)
+  int64_t qbase;         // data row of batch query 0 (self-exclusion)
+  int64_t* out_idx;      // rows indexed by query row
+  F* out_dist;
+};
+
+// Write one output row from a wave-resident, ascending, lexicographically sorted
+// candidate set x (count valid entries) with fuel's exclusion/truncation/padding
+// semantics (python/search.rs:485-488, :1195-1216; src/outlier/common.rs:41-44).
+template <typename F, int V>
+__device__ __forceinline__ void write_knn_row(const typename Comp<F>::T (&x)[V], int count, int64_t qrow,
+                                              int k, int mode, int64_t* __restrict__ oidx,
+                                              F* __restrict__ odist) {
+  using C = Comp<F>;
+  const int lane = __lane_id();
+  // position of the query's own index (at most one entry)
+  int sp = 64 * V;
+  if (mode != EXCL_NONE) {
+#pragma unroll
+    for (int v = V - 1; v >= 0; --v) {
+      const int e = v * 64 + lane;
+      const bool m = e < count && (int64_t)C::idx(x[v]) == qrow;
+      const uint64_t b = __ballot(m);
+      if (b) sp = v * 64 + __builtin_ctzll(b);
+    }
+  }
+  F kth = F(INFINITY);
+  if (mode == EXCL_PY_LINEAR && k >= 1 && count >= k) kth = C::key(wave_get<typename C::T, V>(x, k - 1));
+  int written = 0;
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    const int e = v * 64 + lane;
+    bool keep = e < count;
+    if (mode != EXCL_NONE) keep = keep && e != sp;
+    if (mode == EXCL_PY_LINEAR) keep = keep && C::key(x[v]) <= kth;
+    const int j = e - (e > sp ? 1 : 0);
+    if (keep && j < k) {
+      oidx[j] = (int64_t)C::idx(x[v]);
+      odist[j] = C::key(x[v]);
+    }
+    written += __popcll(__ballot(keep));
+  }
+  if (written > k) written = k;
+  for (int j = written + lane; j < k; j += 64) {
+    oidx[j] = -1;
+    odist[j] = F(INFINITY);
+  }
+}
+
+template <typename F>
+hipError_t launch_scan_exact(const ScanExactArgs<F>& a, int qw, int nsplit, hipStream_t st);
+template <typename F>
+hipError_t launch_finalize_exact(const FinalizeArgs<F>& a, hipStream_t st);
+template <typename F>
+hipError_t launch_range(const RangeArgs<F>& a, int pass, int nsplit, hipStream_t st);
+template <typename F>
+hipError_t launch_pairwise(const PairwiseArgs<F>& a, int form, int nsplit, hipStream_t st);
+template <typename F>
+hipError_t launch_unord(const typename OrdKey<F>::T* keys, int64_t nnz, F* dist, hipStream_t st);
+hipError_t launch_gather_offsets(const int64_t* qs_offsets, int64_t m, int nsplit, int64_t total, int64_t* out,
+                                 hipStream_t st);
+template <typename F>
+hipError_t launch_lof(const LofArgs<F>& a, int knn_outlier, hipStream_t st);
+
+// rocPRIM wrappers (sort.hip): exclusive scan of int64 counts; stable segmented
+// sort of (order key, index) pairs.
+hipError_t scan_exclusive_i64(const int64_t* in, int64_t* out, int64_t count, void* tmp, size_t& tmp_bytes,
+                              hipStream_t st);
+template <typename K>
+hipError_t segmented_sort_pairs(const K* keys_in, K* keys_out, const int64_t* vals_in, int64_t* vals_out,
+                                int64_t nnz, int64_t nseg, const int64_t* offsets, void* tmp, size_t& tmp_bytes,
+                                hipStream_t st);
+
+}  // namespace fa

@@ -1,0 +1,33 @@
+// rocPRIM-backed helpers: exclusive scan of per-(query, split) counts and the
+// stable segmented sort that orders range results by (distance, index).
+#include <cstring>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_segmented_radix_sort.hpp>
+
+#include "kernels.h"
+
+namespace fa {
+
+hipError_t scan_exclusive_i64(const int64_t* in, int64_t* out, int64_t count, void* tmp, size_t& tmp_bytes,
+                              hipStream_t st) {
+  return rocprim::exclusive_scan(tmp, tmp_bytes, in, out, int64_t(0), (size_t)count, rocprim::plus<int64_t>(),
+                                 st);
+}
+
+// Input pairs arrive in index order within each segment, so a stable sort by
+// the order-preserving key yields fuel's (distance, index) order.
+template <typename K>
+hipError_t segmented_sort_pairs(const K* keys_in, K* keys_out, const int64_t* vals_in, int64_t* vals_out,
+                                int64_t nnz, int64_t nseg, const int64_t* offsets, void* tmp, size_t& tmp_bytes,
+                                hipStream_t st) {
+  return rocprim::segmented_radix_sort_pairs(tmp, tmp_bytes, keys_in, keys_out, vals_in, vals_out,
+                                             (unsigned int)nnz, (unsigned int)nseg, offsets, offsets + 1, 0,
+                                             (unsigned int)(8 * sizeof(K)), st);
+}
+
+template hipError_t segmented_sort_pairs<uint32_t>(const uint32_t*, uint32_t*, const int64_t*, int64_t*, int64_t,
+                                                   int64_t, const int64_t*, void*, size_t&, hipStream_t);
+template hipError_t segmented_sort_pairs<uint64_t>(const uint64_t*, uint64_t*, const int64_t*, int64_t*, int64_t,
+                                                   int64_t, const int64_t*, void*, size_t&, hipStream_t);
+
+}  // namespace fa

@@ -1,0 +1,182 @@
+/*
+ * fuel_amd — MI355X (gfx950) exact linear-scan kNN / range / pairwise
+ * squared-Euclidean engine behind a C ABI.
+ *
+ * Drop-in boundary for fuel's linear-scan search path.  Every entry point below
+ * names the reference interface it replaces (paths relative to the fuel
+ * repository).  Conventions shared by all entry points:
+ *   - return value: FUEL_AMD_OK (0) or an error code; the message of the last
+ *     error on the calling thread is available from fuel_amd_last_error().
+ *     Nothing aborts the process.  (fuel: Result<_, String> -> PyRuntimeError,
+ *     src/api/parallel.rs:167-171; bad names -> ValueError, python/mod.rs:64-69.)
+ *   - data and queries are C-contiguous row-major (rows x d) arrays of the
+ *     index dtype, like the numpy arrays fuel takes (src/api/ndarray.rs:96-99).
+ *   - results are identical to fuel's CPU linear scan: distances are the f32
+ *     (or f64) values of fuel's own AVX2 kernel
+ *     (src/distance/squared_euclidean.rs:13-67), sqrt'd for 'euclidean'
+ *     (src/distance/euclidean.rs:7-9), and neighbours are ordered by
+ *     (distance, index) (src/api/search.rs:23-30).
+ *   - host-pointer entry points are synchronous; *_device entry points take
+ *     device pointers and a hipStream_t (as void*) and are asynchronous.
+ *   - an index is immutable after creation; calls on one index from several
+ *     threads are serialized internally.
+ */
+#ifndef FUEL_AMD_H
+#define FUEL_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define FUEL_AMD_OK 0
+#define FUEL_AMD_ERR_INVALID 1     /* bad argument (ValueError in fuel's Python layer) */
+#define FUEL_AMD_ERR_UNSUPPORTED 2 /* valid request outside this engine's scope */
+#define FUEL_AMD_ERR_DEVICE 3      /* HIP runtime error or no usable GPU */
+#define FUEL_AMD_ERR_INTERRUPTED 4 /* fuel_amd_request_cancel() was called (src/api/parallel.rs:73) */
+#define FUEL_AMD_ERR_NOMEM 5       /* device or host allocation failed */
+
+/* dtypes (fuel: build_linear_scan_f32 / _f64, python/search.rs:1353-1389) */
+#define FUEL_AMD_F32 0
+#define FUEL_AMD_F64 1
+
+/* metrics (fuel: parse_distance_fn "euclidean"|"l2" and
+ * "sqeuclidean"|"squared_euclidean", python/mod.rs:49-50) */
+#define FUEL_AMD_EUCLIDEAN 0
+#define FUEL_AMD_SQEUCLIDEAN 1
+
+/* self-exclusion for self-queries (query row i is data row i) */
+#define FUEL_AMD_EXCLUDE_NONE 0
+#define FUEL_AMD_EXCLUDE_FOR_EACH_KNN 1 /* search k+1, drop self, truncate: src/outlier/common.rs:27,41-44 */
+#define FUEL_AMD_EXCLUDE_PY_LINEAR 2    /* search k, drop self: python/search.rs:485-488 */
+
+/* candidate-generation precision.  Every mode returns the exact reference
+ * answer; the modes differ only in how candidates are generated before the
+ * exact rerank and certification. */
+#define FUEL_AMD_MODE_AUTO 0
+#define FUEL_AMD_MODE_EXACT 1 /* restated distance for every pair */
+#define FUEL_AMD_MODE_FP32 2  /* FP32 MFMA ||x||^2-2x.y candidates + exact rerank */
+#define FUEL_AMD_MODE_BF16 3  /* BF16 MFMA candidates + exact FP32 rerank + FP64 bound */
+
+/* pairwise output forms (fuel: compute_pairwise_dense / _condensed) */
+#define FUEL_AMD_FORM_SQUARE 0
+#define FUEL_AMD_FORM_CONDENSED 1 /* fuel's internal order: pair (i>j) at i(i-1)/2+j */
+
+typedef struct fuel_amd_index fuel_amd_index;
+
+typedef struct fuel_amd_opts {
+  int32_t mode;   /* FUEL_AMD_MODE_* */
+  int32_t widen;  /* candidate widening factor for the MFMA modes (default 4) */
+  int32_t device; /* HIP device ordinal (default 0) */
+  int32_t flags;  /* reserved, 0 */
+  int64_t reserved[6];
+} fuel_amd_opts;
+
+/* Variable-length range results as CSR; allocated by the library, released by
+ * fuel_amd_csr_free.  dist has the index dtype. */
+typedef struct fuel_amd_csr {
+  int64_t* offsets; /* m + 1 */
+  int64_t* index;   /* nnz */
+  void* dist;       /* nnz */
+  int64_t nnz;
+  int64_t m;
+} fuel_amd_csr;
+
+/* Statistics of the last kNN call on an index (certification / fallback). */
+typedef struct fuel_amd_stats {
+  int64_t queries;
+  int64_t fallback_queries; /* queries rescanned exactly because the bound did not certify */
+  int32_t mode_used;
+  int32_t splits;
+  int32_t kp;
+  int32_t reserved;
+  double ms_total;
+} fuel_amd_stats;
+
+/* Version string of the library. */
+const char* fuel_amd_version(void);
+
+/* Number of usable GPUs (0 when none: every compute entry point then fails with
+ * FUEL_AMD_ERR_DEVICE; there is no CPU fallback). */
+int32_t fuel_amd_device_count(void);
+
+/* Message of the last error on this thread ("" if none). */
+const char* fuel_amd_last_error(void);
+
+/* Cooperative cancellation (fuel: SHUTDOWN_REQUESTED, src/api/mod.rs:29, polled
+ * in par_try_map, src/api/parallel.rs:73).  Running calls poll the flag between
+ * launches and return FUEL_AMD_ERR_INTERRUPTED. */
+void fuel_amd_request_cancel(void);
+void fuel_amd_clear_cancel(void);
+
+/* Build a resident linear-scan index: copies the (n x d) rows to HBM.
+ * Replaces build_linear_scan_f32 / build_linear_scan_f64
+ * (python/search.rs:1353-1389) and LinearScanSearcher::new
+ * (src/search/linear_scan.rs:8-24).  opts may be NULL. */
+int32_t fuel_amd_index_create(const void* data, int64_t n, int32_t d, int32_t dtype, int32_t metric,
+                              const fuel_amd_opts* opts, fuel_amd_index** out);
+/* Same, from rows already resident in device memory (row stride ld elements). */
+int32_t fuel_amd_index_create_device(const void* d_data, int64_t n, int32_t d, int64_t ld,
+                                     int32_t dtype, int32_t metric, const fuel_amd_opts* opts,
+                                     fuel_amd_index** out);
+int32_t fuel_amd_index_destroy(fuel_amd_index* index);
+
+/* kNN of m query rows.  out_idx (m x k int64, -1 padded) and out_dist
+ * (m x k of the index dtype, +inf padded) are caller-allocated host arrays.
+ * Replaces SearchIndex.knn for the LinearScan variants (python/search.rs:560-569,
+ * LinearScanSearcher::query_search_knn :524-539, linear_scan_knn_batch
+ * :471-491, knn_results_to_arrays :1195-1216) and, with
+ * FUEL_AMD_EXCLUDE_FOR_EACH_KNN, the search loop of for_each_knn
+ * (src/outlier/common.rs:12-49). */
+int32_t fuel_amd_knn(fuel_amd_index* index, const void* query, int64_t m, int32_t k,
+                     int32_t exclude_self, int64_t* out_idx, void* out_dist);
+/* Device-pointer form (query rows with stride ldq elements); asynchronous on
+ * `stream` except for the certification fallback, which synchronizes. */
+int32_t fuel_amd_knn_device(fuel_amd_index* index, const void* d_query, int64_t m, int64_t ldq,
+                            int32_t k, int32_t exclude_self, int64_t* d_out_idx, void* d_out_dist,
+                            void* stream);
+
+/* All neighbours within `radius` (inclusive), sorted by (distance, index).
+ * Replaces SearchIndex.radius_search for the LinearScan variants
+ * (python/search.rs:776-787, linear_scan_range_batch :494-514,
+ * range_results_to_py :1218-1235; src/api/search.rs:154-166).  The radius is
+ * cast to the index dtype first, like fuel (python/search.rs:785). */
+int32_t fuel_amd_range(fuel_amd_index* index, const void* query, int64_t m, double radius,
+                       int32_t exclude_self, fuel_amd_csr* out);
+void fuel_amd_csr_free(fuel_amd_csr* csr);
+
+/* Pairwise distance matrix of n rows (square n x n, or condensed n(n-1)/2 in
+ * fuel's internal lower-triangular order), caller-allocated host output.
+ * Replaces _compute_pairwise_distances_{f32,f64} and the _condensed_ variants
+ * (python/mod.rs:73-123; src/api/square_distance_matrix.rs:12-42;
+ * src/api/condensed_distance_matrix.rs:22-54). */
+int32_t fuel_amd_pairwise(const void* data, int64_t n, int32_t d, int32_t dtype, int32_t metric,
+                          int32_t form, void* out);
+
+/* Local outlier factor over the self-kNN of the index rows (f32 or f64 scores,
+ * n entries).  Replaces local_outlier_factor over a LinearScanSearcher
+ * (src/outlier/lof.rs:14-81 with for_each_knn, src/outlier/common.rs:12-49). */
+int32_t fuel_amd_lof(fuel_amd_index* index, int32_t k, void* out_scores);
+
+/* kNN outlier score (distance to the k-th neighbour, self excluded).
+ * Replaces k_nearest_neighbors_outlier (src/outlier/knn.rs:14-32). */
+int32_t fuel_amd_knn_outlier(fuel_amd_index* index, int32_t k, void* out_scores);
+
+/* Statistics of the last kNN call on this index. */
+int32_t fuel_amd_last_stats(const fuel_amd_index* index, fuel_amd_stats* out);
+
+/* Profiling aid (no counterpart in fuel, which has no GPU): when enabled, the
+ * dominant scan kernel of every kNN call on this index is bracketed by HIP
+ * events recorded on the call's stream.  fuel_amd_timing_read waits for the
+ * recorded events, returns the summed kernel time, the number of timed
+ * launches and the kernel's name, and resets the record. */
+int32_t fuel_amd_timing_enable(fuel_amd_index* index, int32_t enable);
+int32_t fuel_amd_timing_read(fuel_amd_index* index, double* kernel_ms, int64_t* launches, char* name,
+                             int32_t name_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FUEL_AMD_H */

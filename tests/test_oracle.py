@@ -1,0 +1,226 @@
+"""Pins the CPU oracle (a restatement of fuel's linear-scan path) against every
+known answer fuel's own tests hold for this path, against closed forms and
+against brute force, so that a dropped term, a wrong order or a wrong tie rule
+fails at least one check.  CPU only."""
+import itertools
+
+import numpy as np
+import pytest
+
+from conftest import load_gaussian4d, load_reference_scores
+
+
+# -- squared distance kernel (src/distance/squared_euclidean.rs:13-67) -----------
+
+def test_pairwise_known_answers_f32_f64(oracle):
+    # src/math/mod.rs:672-690
+    p = np.array([[0, 1], [2, 3], [1, 4]], np.float32)
+    assert oracle.pairwise(p, "sqeuclidean").ravel().tolist() == [0, 8, 10, 8, 0, 2, 10, 2, 0]
+    p = np.array([[1, 2, 3, 4], [4, 3, 2, 1], [0, 0, 0, 0]], np.float64)
+    assert oracle.pairwise(p, "sqeuclidean").ravel().tolist() == [0, 20, 30, 20, 0, 30, 30, 30, 0]
+
+
+def test_avx_matches_scalar_lane_restatement(oracle):
+    rng = np.random.default_rng(0)
+    for d in [0, 1, 3, 7, 8, 9, 15, 16, 17, 31, 64, 100, 128, 784]:
+        for _ in range(20):
+            a = (rng.standard_normal(d) * rng.choice([1e-3, 1, 1e3])).astype(np.float32)
+            b = (rng.standard_normal(d) * rng.choice([1e-3, 1, 1e3])).astype(np.float32)
+            x = oracle.sqdist(a, b, "avx")
+            y = oracle.sqdist(a, b, "lanes")
+            assert x.tobytes() == y.tobytes(), d
+
+
+def _numpy_restated(a, b):
+    """The same arithmetic written with numpy float32 scalars; fma is emulated in
+    float64 with a single rounding, exact because the product of two f32 fits in
+    f64 and the sum is rounded once (double rounding cannot occur for a single
+    f32 product plus f32 addend when computed exactly and rounded to f32 once
+    via np.float32 of the exact f64 sum whenever that sum is exact in f64)."""
+    d = a.shape[0]
+    sd = d & ~7
+    acc = [np.float32(0)] * 8
+    for i in range(0, sd, 8):
+        for j in range(8):
+            t = np.float32(a[i + j] - b[i + j])
+            acc[j] = np.float32(np.float64(t) * np.float64(t) + np.float64(acc[j]))
+    s = np.float32(-0.0)
+    for j in range(8):
+        s = np.float32(s + acc[j])
+    for i in range(sd, d):
+        t = np.float32(a[i] - b[i])
+        s = np.float32(np.float64(t) * np.float64(t) + np.float64(s))
+    return s
+
+
+def test_restatement_on_small_integers_is_exact(oracle):
+    # integer inputs: every intermediate is an exact integer below 2^24, so the
+    # restated value must equal the exact sum, whatever the rounding order
+    rng = np.random.default_rng(1)
+    for d in [5, 8, 13, 16, 33, 128, 784]:
+        a = rng.integers(0, 256, d).astype(np.float32)
+        b = rng.integers(0, 256, d).astype(np.float32)
+        exact = int(((a.astype(np.int64) - b.astype(np.int64)) ** 2).sum())
+        assert exact < 2 ** 24
+        assert float(oracle.sqdist(a, b)) == exact
+        assert float(_numpy_restated(a, b)) == exact
+
+
+def test_lane_order_is_observable(oracle):
+    # Pins the 8-lane interleave and the left-fold lane sum with exact values:
+    # lane 0 accumulates 1 + 4096^2 = 2^24 + 1 -> 2^24 (ties to even), lanes 1..7
+    # hold 1 each, and each +1 of the left fold is absorbed: 2^24.
+    a = np.array([1.0] * 8 + [4096.0] + [0.0] * 7, np.float32)
+    b = np.zeros_like(a)
+    assert float(oracle.sqdist(a, b)) == 2.0 ** 24
+    # a single sequential accumulator would give 8 + 2^24 (exact), and a
+    # right-to-left lane fold 7 + 2^24 -> 2^24 + 8: both differ
+    assert float(np.float32(np.float64(8.0) + 2.0 ** 24)) != 2.0 ** 24
+    # the tail runs after the fold: d = 9 puts 4096 in the tail, fused onto the
+    # folded 8 -> 2^24 + 8 exactly
+    a9 = np.array([1.0] * 8 + [4096.0], np.float32)
+    assert float(oracle.sqdist(a9, np.zeros_like(a9))) == 2.0 ** 24 + 8
+    assert _numpy_restated(a, b).tobytes() == oracle.sqdist(a, b).tobytes()
+
+
+def test_restated_close_to_fp64(oracle):
+    rng = np.random.default_rng(2)
+    X = rng.standard_normal((200, 37)).astype(np.float32)
+    Q = rng.standard_normal((50, 37)).astype(np.float32)
+    qi = np.repeat(np.arange(50), 200)
+    xi = np.tile(np.arange(200), 50)
+    ref = oracle.sqdist_fp64(X, Q, qi, xi)
+    got = np.array([oracle.sqdist(Q[i], X[j]) for i, j in zip(qi, xi)], np.float64)
+    # error bound of the restated sum: (d/8 + 8 + d%8 + 2) u relative
+    gamma = (37 // 8 + 8 + 37 % 8 + 2) * 2.0 ** -24
+    assert np.all(np.abs(got - ref) <= gamma * ref + 1e-30)
+
+
+def test_fallback_kernel_differs_only_by_rounding(oracle):
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal(100).astype(np.float32)
+    b = rng.standard_normal(100).astype(np.float32)
+    x, y = float(oracle.sqdist(a, b)), float(oracle.sqdist(a, b, "fallback"))
+    assert abs(x - y) <= 1e-5 * x
+
+
+# -- linear scan semantics (src/api/search.rs, src/search/linear_scan.rs) -------
+
+def test_linear_scan_knn_known_answer(oracle):
+    # src/search/linear_scan.rs:247-265: k=2 returns 3 entries (tie), order [1,0,2]
+    X = np.array([[0, 0], [1, 0], [2, 0]], np.float32)
+    offs, idx, dist = oracle.knn_ties(X, np.array([[1, 0]], np.float32), 2)
+    assert idx.tolist() == [1, 0, 2]
+    assert list(offs) == [0, 3]
+    assert dist[0] <= dist[1] <= dist[2]
+
+
+def test_linear_scan_range_known_answer(oracle):
+    # src/search/linear_scan.rs:267-279
+    X = np.array([[0, 0], [1, 0], [2, 0]], np.float64)
+    offs, idx, dist = oracle.range_search(X, np.array([[0.5, 0]]), 1.0)
+    assert idx.tolist() == [0, 1]
+
+
+def _brute_force_knn(X, q, k, metric):
+    """src/search/vptree/tests/shared.rs:20-42: sort by (distance, index), keep
+    every entry <= the k-th distance."""
+    d = np.array([oracle_distance(q, x, metric) for x in X])
+    order = sorted(range(len(X)), key=lambda i: (d[i], i))
+    if k == 0 or len(X) == 0:
+        return []
+    kth = d[order[min(k, len(X)) - 1]]
+    return [(d[i], i) for i in order if d[i] <= kth]
+
+
+_ORACLE = None
+
+
+def oracle_distance(a, b, metric):
+    s = _ORACLE.sqdist(a, b)
+    return np.sqrt(s) if metric == "euclidean" else s
+
+
+@pytest.mark.parametrize("metric", ["euclidean", "sqeuclidean"])
+def test_knn_heap_equals_brute_force_on_grids(oracle, metric):
+    # integer grids give heavy index-ordered ties (src/search/vptree/tests/knn.rs)
+    global _ORACLE
+    _ORACLE = oracle
+    for g in (5, 10):
+        X = np.array(list(itertools.product(range(g), range(g))), np.float32)
+        for k in (1, 2, 3, 5, 8, 13):
+            offs, idx, dist = oracle.knn_ties(X, X, k, metric)
+            for qi in range(0, len(X), 7):
+                exp = _brute_force_knn(X, X[qi], k, metric)
+                got = list(zip(dist[offs[qi]:offs[qi + 1]].tolist(), idx[offs[qi]:offs[qi + 1]].tolist()))
+                assert [(float(a), b) for a, b in exp] == [(float(a), b) for a, b in got]
+
+
+def test_python_packing_and_exclusion_modes(oracle):
+    # duplicates at distance 0 make ties broken by index
+    X = np.array([[0, 0], [0, 0], [1, 0], [1, 0], [2, 0]], np.float32)
+    idx, dist = oracle.knn(X, X, 2, exclude="none")
+    assert idx.tolist() == [[0, 1], [0, 1], [2, 3], [2, 3], [4, 2]]
+    # python linear path: search k, drop self -> k-1 real neighbours + ties
+    idx2, dist2 = oracle.knn(X, X, 2, exclude="python_linear")
+    assert idx2.tolist() == [[1, -1], [0, -1], [3, -1], [2, -1], [2, 3]]
+    assert np.isinf(dist2[0, 1])
+    # for_each_knn: search k+1, drop self, truncate k
+    idx1, _ = oracle.knn(X, X, 2, exclude="for_each_knn")
+    assert idx1.tolist() == [[1, 2], [0, 2], [3, 0], [2, 0], [2, 3]]
+    # k > n pads with -1 / inf
+    idx3, dist3 = oracle.knn(X, X[:1], 7)
+    assert idx3.tolist() == [[0, 1, 2, 3, 4, -1, -1]] and np.isinf(dist3[0, 5:]).all()
+
+
+def test_precomputed_subset_with_ties(oracle):
+    # src/search/precomputed.rs:136-165: subsets of a max_k table equal a fresh scan
+    X = np.array([[0, 0], [0, 0], [1, 0], [1, 0], [2, 0]], np.float64)
+    offs, idx, dist = oracle.knn_ties(X, X, 2)
+    for qi in range(5):
+        row = dist[offs[qi]:offs[qi + 1]]
+        for k in (1, 2):
+            c = oracle.subset_with_ties(row, k)
+            o2, i2, d2 = oracle.knn_ties(X, X[qi:qi + 1], k)
+            assert idx[offs[qi]:offs[qi] + c].tolist() == i2.tolist()
+
+
+# -- pairwise forms ---------------------------------------------------------------
+
+def test_pairwise_forms(oracle):
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((40, 11)).astype(np.float32)
+    sq = oracle.pairwise(X, "euclidean", "square")
+    assert (sq == sq.T).all() and (np.diag(sq) == 0).all()
+    cd = oracle.pairwise(X, "euclidean", "condensed")
+    for i in range(1, 40):
+        for j in range(i):
+            assert cd[i * (i - 1) // 2 + j] == sq[i, j]
+
+
+# -- LOF / kNN outlier against ELKI (src/outlier/lof.rs:192-211, knn.rs:78-96) ----
+
+def test_lof_and_knn_outlier_match_elki(oracle):
+    X = load_gaussian4d()
+    ref = load_reference_scores()
+    assert X.shape == (200, 4)
+    np.testing.assert_allclose(oracle.lof(X, 10), ref["LOF-10"], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(oracle.knn_outlier(X, 10), ref["KNN-10"], rtol=0, atol=1e-6)
+    # f32 data: within 1e-5 of the f64 ELKI values
+    np.testing.assert_allclose(oracle.lof(X.astype(np.float32), 10), ref["LOF-10"], rtol=1e-5, atol=1e-5)
+
+
+def test_lof_closed_form(oracle):
+    # src/outlier/lof.rs:119-138
+    P = np.array([[1, 1], [1, 2], [2, 1]], np.float64)
+    s = oracle.lof(P, 2)
+    r2 = np.sqrt(2.0)
+    assert abs(s[0] - 2 * r2 / (1 + r2)) < 1e-12
+    s1 = (1 + r2) * (1 / (4 * r2) + 1 / (2 + 2 * r2))
+    assert abs(s[1] - s1) < 1e-12 and abs(s[2] - s1) < 1e-12
+
+
+def test_lof_degenerate_cases(oracle):
+    assert oracle.lof(np.zeros((1, 3)), 5).tolist() == [1.0]
+    # all duplicates: reachability sums 0 -> lrd inf -> score 1
+    assert oracle.lof(np.zeros((6, 2)), 3).tolist() == [1.0] * 6
