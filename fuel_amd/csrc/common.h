@@ -193,6 +193,31 @@ __device__ __forceinline__ int merge_slot(T* __restrict__ slot, int KP, int lcnt
   return total < KP ? total : KP;
 }
 
+// Two-segment variant: the append buffer is split between the two halves of a
+// wave that share a query ([KP, KP+csub) and [KP+csub, KP+2*csub)).
+template <typename T, int V>
+__device__ __forceinline__ int merge_slot2(T* __restrict__ slot, int KP, int lcnt, int csub, int c0, int c1, T& kth) {
+  const int lane = __lane_id();
+  constexpr T TMAX = ~T(0);
+  T x[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    const int e = v * 64 + lane;
+    const bool valid = (e < lcnt) || (e >= KP && e < KP + c0) || (e >= KP + csub && e < KP + csub + c1);
+    x[v] = valid ? slot[e] : TMAX;
+  }
+  wave_bitonic_sort<T, V>(x);
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    const int e = v * 64 + lane;
+    if (e < KP) slot[e] = x[v];
+  }
+  __threadfence_block();
+  kth = wave_get<T, V>(x, KP - 1);
+  const int total = lcnt + c0 + c1;
+  return total < KP ? total : KP;
+}
+
 template <typename T, int V>
 __device__ __forceinline__ T wave_get(const T (&x)[V], int e) {
   // Broadcast element e (uniform) to all lanes.

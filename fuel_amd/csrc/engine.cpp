@@ -21,9 +21,11 @@ constexpr size_t SLOT_BUDGET = size_t(1) << 30;  // bytes of candidate slots per
 
 int exact_qw(int64_t nq) { return nq >= 64 ? 16 : (nq >= 8 ? 8 : 4); }
 
+// Exact kNN of m query slots.  With qrows, slot i is query row qrows[i] of Q
+// (a device array) and its result goes to output row qrows[i].
 template <typename F>
 int32_t knn_exact(fuel_amd_index* ix, const F* Q, int64_t m, int64_t ldq, int k, int mode, int64_t* out_idx,
-                  F* out_dist, hipStream_t st) {
+                  F* out_dist, hipStream_t st, const int64_t* qrows = nullptr, int64_t qbase = 0) {
   using T = typename Comp<F>::T;
   const int KK = k + (mode != EXCL_NONE ? 1 : 0);
   if (KK > EXACT_NS / 2)
@@ -45,9 +47,9 @@ int32_t knn_exact(fuel_amd_index* ix, const F* Q, int64_t m, int64_t ldq, int k,
     sa.ldx = ix->ld;
     sa.n = ix->n;
     sa.d = ix->d;
-    sa.Q = Q + done * ldq;
+    sa.Q = qrows ? Q : Q + done * ldq;
     sa.ldq = ldq;
-    sa.qrows = nullptr;
+    sa.qrows = qrows ? qrows + done : nullptr;
     sa.nq = chunk;
     sa.metric = ix->metric;
     sa.KP = KK;
@@ -70,10 +72,10 @@ int32_t knn_exact(fuel_amd_index* ix, const F* Q, int64_t m, int64_t ldq, int k,
     fa_.KK = KK;
     fa_.k = k;
     fa_.mode = mode;
-    fa_.qrows = nullptr;
-    fa_.qbase = done;
-    fa_.out_idx = out_idx + done * k;
-    fa_.out_dist = out_dist + done * k;
+    fa_.qrows = qrows ? qrows + done : nullptr;
+    fa_.qbase = qrows ? qbase : qbase + done;
+    fa_.out_idx = qrows ? out_idx : out_idx + done * k;
+    fa_.out_dist = qrows ? out_dist : out_dist + done * k;
     e = launch_finalize_exact<F>(fa_, st);
     if (e != hipSuccess) return hip_fail(e, "finalize launch");
     ix->stats.splits = nsplit;
@@ -240,6 +242,11 @@ int32_t outlier_impl(fuel_amd_index* ix, int k, int knn_outlier, void* out_score
 
 }  // namespace
 
+int32_t knn_exact_rows(fuel_amd_index* ix, const float* Q, int64_t ldq, const int64_t* d_rows, int64_t nrows,
+                       int64_t qbase, int k, int exclude, int64_t* out_idx, float* out_dist, hipStream_t st) {
+  return knn_exact<float>(ix, Q, nrows, ldq, k, exclude, out_idx, out_dist, st, d_rows, qbase);
+}
+
 int32_t knn_run(fuel_amd_index* ix, const void* d_query, int64_t m, int64_t ldq, int k, int exclude,
                 int64_t* d_out_idx, void* d_out_dist, hipStream_t st) {
   if (m == 0 || k == 0) return FUEL_AMD_OK;
@@ -247,12 +254,21 @@ int32_t knn_run(fuel_amd_index* ix, const void* d_query, int64_t m, int64_t ldq,
   auto t0 = std::chrono::steady_clock::now();
   ix->stats = fuel_amd_stats{};
   ix->stats.queries = m;
-  ix->stats.mode_used = FUEL_AMD_MODE_EXACT;
+  int mode = ix->opts.mode;
+  if (mode == FUEL_AMD_MODE_AUTO) mode = FUEL_AMD_MODE_BF16;
+  int kind = mode == FUEL_AMD_MODE_BF16 ? 0 : (mode == FUEL_AMD_MODE_FP32 ? 1 : -1);
+  if (kind >= 0 && !mfma_supported(ix, kind, k, exclude)) kind = -1;
   int32_t rc;
-  if (ix->dtype == FUEL_AMD_F32)
+  if (kind >= 0) {
+    ix->stats.mode_used = kind == 0 ? FUEL_AMD_MODE_BF16 : FUEL_AMD_MODE_FP32;
+    rc = knn_mfma(ix, (const float*)d_query, m, ldq, k, exclude, kind, d_out_idx, (float*)d_out_dist, st);
+  } else if (ix->dtype == FUEL_AMD_F32) {
+    ix->stats.mode_used = FUEL_AMD_MODE_EXACT;
     rc = knn_exact<float>(ix, (const float*)d_query, m, ldq, k, exclude, d_out_idx, (float*)d_out_dist, st);
-  else
+  } else {
+    ix->stats.mode_used = FUEL_AMD_MODE_EXACT;
     rc = knn_exact<double>(ix, (const double*)d_query, m, ldq, k, exclude, d_out_idx, (double*)d_out_dist, st);
+  }
   ix->stats.ms_total =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return rc;

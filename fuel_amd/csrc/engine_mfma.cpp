@@ -1,0 +1,229 @@
+// Host orchestration of the mixed-precision kNN path:
+//   prep (once per index) -> per query batch: prep queries -> MFMA candidate
+//   scan with fused selection -> exact rerank + certification -> exact
+//   fallback scan for the queries whose bound did not certify.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "api_internal.h"
+#include "kernels.h"
+
+namespace fa {
+
+namespace {
+
+constexpr size_t MFMA_SLOT_BUDGET = size_t(2) << 30;  // bytes of candidate slots per batch
+constexpr int KP_MAX = 128;
+
+int kp_for(const fuel_amd_index* ix, int k, int exclude) {
+  const int KK = k + (exclude != EXCL_NONE ? 1 : 0);
+  const int w = ix->opts.widen > 0 ? ix->opts.widen : 4;
+  int kp = std::max(KK, std::min(KP_MAX, w * k));
+  return kp;
+}
+
+int32_t ensure_prepared(fuel_amd_index* ix, int kind, hipStream_t st) {
+  if (ix->mfma_kind == kind) return FUEL_AMD_OK;
+  const int ks = mfma_ksteps_bucket(kind, ix->d);
+  if (ks < 0) return fail(FUEL_AMD_ERR_UNSUPPORTED, "dimension too large for the MFMA candidate scan");
+  const int kp = ks * (kind == 0 ? 16 : 8);
+  const int64_t n_pad = std::max<int64_t>(256, (ix->n + 255) & ~int64_t(255));
+  const size_t es = kind == 0 ? 2 : 4;
+  hipError_t e = ix->Xop.ensure(size_t(n_pad) * kp * es);
+  if (e == hipSuccess) e = ix->hx.ensure(sizeof(float) * n_pad);
+  if (e == hipSuccess) e = ix->consts.ensure(64);
+  if (e != hipSuccess) return hip_fail(e, "MFMA operand copy");
+  unsigned long long hc[4] = {0, 0, 0, 0};
+  int hf[2] = {1, 1};
+  char init[64];
+  std::memset(init, 0, sizeof(init));
+  std::memcpy(init, hc, sizeof(hc));
+  std::memcpy(init + 32, hf, sizeof(hf));
+  e = hipMemcpyAsync(ix->consts.p, init, 64, hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return hip_fail(e, "bound constants");
+  PrepDbArgs pa{};
+  pa.X = ix->X.as<float>();
+  pa.ldx = ix->ld;
+  pa.n = ix->n;
+  pa.n_pad = n_pad;
+  pa.d = ix->d;
+  pa.Kp = kp;
+  pa.bf16 = kind == 0;
+  pa.Xop = ix->Xop.p;
+  pa.hx = ix->hx.as<float>();
+  pa.consts = ix->consts.as<unsigned long long>();
+  pa.flags = reinterpret_cast<int*>(ix->consts.as<char>() + 32);
+  e = launch_prep_db(pa, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, "MFMA operand preparation");
+  ix->mfma_kind = kind;
+  ix->mfma_ks = ks;
+  ix->mfma_kp = kp;
+  ix->n_pad = n_pad;
+  return FUEL_AMD_OK;
+}
+
+// Number of DB splits: fill the resident workgroup slots (one workgroup per CU
+// at this kernel's register use) with as little tail as possible.
+int mfma_splits(int64_t nqb, int64_t n_pad, int cus, int64_t rows_tile) {
+  const int64_t slots = cus;
+  const int64_t max_s = std::max<int64_t>(1, std::min<int64_t>(32, n_pad / (rows_tile * 4)));
+  int best = 1;
+  double best_eff = -1.0;
+  for (int64_t s = 1; s <= max_s; ++s) {
+    const int64_t units = nqb * s;
+    const int64_t rounds = (units + slots - 1) / slots;
+    const double eff = double(units) / double(rounds * slots);
+    // prefer fewer splits unless more splits fill the machine clearly better
+    if (eff > best_eff + 0.03) {
+      best_eff = eff;
+      best = (int)s;
+    }
+    if (eff > 0.93) break;
+  }
+  return best;
+}
+
+}  // namespace
+
+bool mfma_supported(const fuel_amd_index* ix, int kind, int k, int exclude) {
+  if (ix->dtype != FUEL_AMD_F32) return false;
+  if (mfma_ksteps_bucket(kind, ix->d) < 0) return false;
+  const int KK = k + (exclude != EXCL_NONE ? 1 : 0);
+  return KK <= KP_MAX;
+}
+
+int32_t knn_mfma(fuel_amd_index* ix, const float* Q, int64_t m, int64_t ldq, int k, int exclude, int kind,
+                 int64_t* out_idx, float* out_dist, hipStream_t st) {
+  int32_t rc = ensure_prepared(ix, kind, st);
+  if (rc) return rc;
+  const int KK = k + (exclude != EXCL_NONE ? 1 : 0);
+  const int KP = kp_for(ix, k, exclude);
+  const int V = KP <= 64 ? 4 : 8;
+  const int NS = 64 * V;
+  const int csub = (NS - KP) / 2;
+  const int ks = ix->mfma_ks;
+  const int Kp = ix->mfma_kp;
+  const int qpb = mfma_queries_per_block(kind, ks);
+  const int rows_tile = mfma_rows_per_tile(kind, ks);
+  const int cus = device_cu_count(ix->device);
+  const size_t es = kind == 0 ? 2 : 4;
+  const int64_t ld32 = ix->ld;
+
+  int64_t done = 0;
+  while (done < m) {
+    if (cancelled()) return fail(FUEL_AMD_ERR_INTERRUPTED, "interrupted");
+    int64_t chunk = m - done;
+    int64_t nqb = (chunk + qpb - 1) / qpb;
+    int nsplit = mfma_splits(nqb, ix->n_pad, cus, rows_tile);
+    const size_t per_q = size_t(nsplit) * NS * 8;
+    const int64_t max_chunk = std::max<int64_t>(qpb, int64_t(MFMA_SLOT_BUDGET / per_q) / qpb * qpb);
+    if (chunk > max_chunk) {
+      chunk = max_chunk;
+      nqb = chunk / qpb;
+    }
+    const int64_t nq_pad = nqb * qpb;
+    int64_t rps = (ix->n_pad + nsplit - 1) / nsplit;
+    rps = (rps + rows_tile - 1) / rows_tile * rows_tile;
+    nsplit = (int)((ix->n_pad + rps - 1) / rps);
+
+    hipError_t e = ix->Qop.ensure(size_t(nq_pad) * Kp * es);
+    if (e == hipSuccess) e = ix->Q32.ensure(size_t(chunk) * ld32 * 4);
+    if (e == hipSuccess) e = ix->qnorm.ensure(sizeof(double) * 3 * size_t(chunk));
+    if (e == hipSuccess) e = ix->qint.ensure(sizeof(int) * size_t(chunk));
+    if (e == hipSuccess) e = ix->slots.ensure(per_q * nq_pad);
+    if (e == hipSuccess) e = ix->lcnts.ensure(sizeof(int) * size_t(nsplit) * nq_pad);
+    if (e == hipSuccess) e = ix->fail_count.ensure(64);
+    if (e == hipSuccess) e = ix->fail_list.ensure(sizeof(int64_t) * size_t(chunk));
+    if (e != hipSuccess) return hip_fail(e, "MFMA workspaces");
+
+    PrepQArgs qa{};
+    qa.Q = Q + done * ldq;
+    qa.ldq = ldq;
+    qa.m = chunk;
+    qa.m_pad = nq_pad;
+    qa.d = ix->d;
+    qa.Kp = Kp;
+    qa.bf16 = kind == 0;
+    qa.Qop = ix->Qop.p;
+    qa.Q32 = ix->Q32.as<float>();
+    qa.ld32 = ld32;
+    qa.qn2 = ix->qnorm.as<double>();
+    qa.eq2 = qa.qn2 + chunk;
+    qa.qt2 = qa.eq2 + chunk;
+    qa.qint = ix->qint.as<int>();
+    e = launch_prep_queries(qa, st);
+    if (e == hipSuccess) e = hipMemsetAsync(ix->fail_count.p, 0, sizeof(int), st);
+    if (e != hipSuccess) return hip_fail(e, "query preparation");
+
+    ScanMfmaArgs sa{};
+    sa.Xop = ix->Xop.p;
+    sa.hx = ix->hx.as<float>();
+    sa.Kp = Kp;
+    sa.n_pad = ix->n_pad;
+    sa.Qop = ix->Qop.p;
+    sa.nq_pad = nq_pad;
+    sa.nqb = (int)nqb;
+    sa.rows_per_split = rps;
+    sa.KP = KP;
+    sa.Csub = csub;
+    sa.slots = ix->slots.as<uint64_t>();
+    sa.lcnts = ix->lcnts.as<int>();
+    timing_mark(ix, st, true, kind == 0 ? "scan_mfma_kernel<bf16>" : "scan_mfma_kernel<f32>");
+    e = launch_scan_mfma(sa, kind, ks, V, nsplit, st);
+    timing_mark(ix, st, false, nullptr);
+    if (e != hipSuccess) return hip_fail(e, "MFMA scan launch");
+
+    RerankArgs ra{};
+    ra.slots = sa.slots;
+    ra.lcnts = sa.lcnts;
+    ra.nsplit = nsplit;
+    ra.nq = chunk;
+    ra.nq_pad = nq_pad;
+    ra.slot_ns = NS;
+    ra.KP = KP;
+    ra.X = ix->X.as<float>();
+    ra.ldx = ix->ld;
+    ra.Q32 = qa.Q32;
+    ra.ld32 = ld32;
+    ra.d = ix->d;
+    ra.metric = ix->metric;
+    ra.bf16 = kind == 0;
+    ra.qn2 = qa.qn2;
+    ra.eq2 = qa.eq2;
+    ra.qt2 = qa.qt2;
+    ra.qint = qa.qint;
+    ra.consts = ix->consts.as<unsigned long long>();
+    ra.flags = reinterpret_cast<const int*>(ix->consts.as<char>() + 32);
+    ra.KK = KK;
+    ra.k = k;
+    ra.mode = exclude;
+    ra.qbase = done;
+    ra.out_idx = out_idx + done * k;
+    ra.out_dist = out_dist + done * k;
+    ra.fail_count = ix->fail_count.as<int>();
+    ra.fail_list = ix->fail_list.as<int64_t>();
+    e = launch_rerank(ra, V, st);
+    if (e != hipSuccess) return hip_fail(e, "rerank launch");
+
+    int nfail = 0;
+    e = hipMemcpyAsync(&nfail, ix->fail_count.p, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "certification count");
+    if (nfail > 0) {
+      rc = knn_exact_rows(ix, qa.Q32, ld32, ra.fail_list, nfail, done, k, exclude, out_idx + done * k,
+                          out_dist + done * k, st);
+      if (rc) return rc;
+    }
+    ix->stats.fallback_queries += nfail;
+    ix->stats.splits = nsplit;
+    ix->stats.kp = KP;
+    done += chunk;
+  }
+  return FUEL_AMD_OK;
+}
+
+}  // namespace fa

@@ -152,6 +152,100 @@ __device__ __forceinline__ void write_knn_row(const typename Comp<F>::T (&x)[V],
   }
 }
 
+// ---- mixed-precision candidate path ------------------------------------------
+// Index-level bound constants (nonnegative doubles stored as their bit patterns
+// so that atomicMax on u64 orders them): [0] max ||x||^2, [1] max ||x~ - x||^2,
+// [2] max ||x~||^2 (x~ = the MFMA operand copy).  flags[0] = 1 while every
+// coordinate is an integer of magnitude <= 256, flags[1] = 1 while all are >= 0.
+struct PrepDbArgs {
+  const float* X;  // resident f32 rows (ldx)
+  int64_t ldx;
+  int64_t n;
+  int64_t n_pad;
+  int d;
+  int Kp;
+  int bf16;    // operand type: 1 bf16, 0 f32
+  void* Xop;   // n_pad x Kp operand copy
+  float* hx;   // n_pad: fl32(||x||^2 / 2), +inf for padding rows
+  unsigned long long* consts;
+  int* flags;
+};
+
+struct PrepQArgs {
+  const float* Q;
+  int64_t ldq;
+  int64_t m;
+  int64_t m_pad;
+  int d;
+  int Kp;
+  int bf16;
+  void* Qop;    // m_pad x Kp operand copy (bf16 or f32)
+  float* Q32;   // m x ld32 aligned f32 copy for the exact rerank
+  int64_t ld32;
+  double* qn2;  // ||q||^2
+  double* eq2;  // ||q~ - q||^2
+  double* qt2;  // ||q~||^2
+  int* qint;    // 1 if integer-valued with |v| <= 256
+};
+
+struct ScanMfmaArgs {
+  const void* Xop;
+  const float* hx;
+  int Kp;
+  int64_t n_pad;
+  const void* Qop;
+  int64_t nq_pad;          // query slots (multiple of the workgroup's query count)
+  int nqb;                 // query blocks
+  int64_t rows_per_split;  // multiple of the row tile
+  int KP;                  // candidates kept per (split, query)
+  int Csub;                // per-lane-half append buffer capacity
+  uint64_t* slots;         // [nsplit][nq_pad][NS]
+  int* lcnts;              // [nsplit][nq_pad]
+};
+
+struct RerankArgs {
+  const uint64_t* slots;
+  const int* lcnts;
+  int nsplit;
+  int64_t nq;
+  int64_t nq_pad;
+  int slot_ns;
+  int KP;
+  const float* X;
+  int64_t ldx;
+  const float* Q32;
+  int64_t ld32;
+  int d;
+  int metric;
+  int bf16;
+  const double* qn2;
+  const double* eq2;
+  const double* qt2;
+  const int* qint;
+  const unsigned long long* consts;
+  const int* flags;
+  int KK;
+  int k;
+  int mode;
+  int64_t qbase;
+  int64_t* out_idx;
+  float* out_dist;
+  int* fail_count;
+  int64_t* fail_list;
+};
+
+hipError_t launch_prep_db(const PrepDbArgs& a, hipStream_t st);
+hipError_t launch_prep_queries(const PrepQArgs& a, hipStream_t st);
+// kind 0: bf16 MFMA, 1: f32 MFMA.  Returns hipErrorInvalidValue for an
+// unsupported K step count.
+hipError_t launch_scan_mfma(const ScanMfmaArgs& a, int kind, int ksteps, int V, int nsplit, hipStream_t st);
+hipError_t launch_rerank(const RerankArgs& a, int V, hipStream_t st);
+// Supported K step counts of the MFMA scan (bf16: 16 dims per step; f32: 8 dims
+// per step group); the padded K is steps * 16 (bf16) or steps * 8 (f32).
+int mfma_ksteps_bucket(int kind, int d);
+int mfma_queries_per_block(int kind, int ksteps);
+int mfma_rows_per_tile(int kind, int ksteps);
+
 template <typename F>
 hipError_t launch_scan_exact(const ScanExactArgs<F>& a, int qw, int nsplit, hipStream_t st);
 template <typename F>
