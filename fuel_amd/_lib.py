@@ -79,6 +79,14 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with `python -m fuel_amd._build`")
+    # One HIP runtime per process: torch bundles its own libamdhip64 (same soname,
+    # libamdhip64.so.7).  Loading torch first lets the dynamic loader satisfy this
+    # library's dependency with the copy torch already mapped, so torch device
+    # buffers and streams can be passed to the *_device entry points.
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     P = ctypes.c_void_p
     I32 = ctypes.c_int32

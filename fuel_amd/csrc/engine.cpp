@@ -153,14 +153,14 @@ int32_t range_impl(fuel_amd_index* ix, const void* query, int64_t m, double radi
     if (cancelled()) return fail(FUEL_AMD_ERR_INTERRUPTED, "interrupted");
     if (nnz > int64_t(0xFFFFFFFFll)) return fail(FUEL_AMD_ERR_UNSUPPORTED, "more than 2^32 range results");
     // keys, idx (unsorted) then sorted
-    e = ix->ws_d.ensure((sizeof(K) + 8) * size_t(nnz));
-    if (e == hipSuccess) e = ix->ws_e.ensure((sizeof(K) + 8) * size_t(nnz));
+    const size_t key_bytes = (sizeof(K) * size_t(nnz) + 15) & ~size_t(15);
+    e = ix->ws_d.ensure(key_bytes + 8 * size_t(nnz));
+    if (e == hipSuccess) e = ix->ws_e.ensure(key_bytes + 8 * size_t(nnz));
     if (e != hipSuccess) return hip_fail(e, "range result workspace");
     K* k_in = ix->ws_d.as<K>();
-    int64_t* i_in = reinterpret_cast<int64_t*>(ix->ws_d.as<char>() + sizeof(K) * nnz + (8 - (sizeof(K) * nnz) % 8) % 8);
+    int64_t* i_in = reinterpret_cast<int64_t*>(ix->ws_d.as<char>() + key_bytes);
     K* k_out = ix->ws_e.as<K>();
-    int64_t* i_out = reinterpret_cast<int64_t*>(ix->ws_e.as<char>() + sizeof(K) * nnz + (8 - (sizeof(K) * nnz) % 8) % 8);
-    // (buffers are sized with 8 bytes of slack for the alignment above)
+    int64_t* i_out = reinterpret_cast<int64_t*>(ix->ws_e.as<char>() + key_bytes);
     ra.offsets = ix->ws_b.as<int64_t>();
     ra.out_keys = k_in;
     ra.out_idx = i_in;
@@ -169,13 +169,15 @@ int32_t range_impl(fuel_amd_index* ix, const void* query, int64_t m, double radi
     tmp_bytes = 0;
     e = segmented_sort_pairs<K>(k_in, k_out, i_in, i_out, nnz, m, ix->ws_c.as<int64_t>(), nullptr, tmp_bytes, st);
     if (e == hipSuccess) e = ix->ws_tmp.ensure(tmp_bytes);
-    if (e == hipSuccess)
-      e = segmented_sort_pairs<K>(k_in, k_out, i_in, i_out, nnz, m, ix->ws_c.as<int64_t>(), ix->ws_tmp.p, tmp_bytes,
-                                  st);
-    if (e == hipSuccess) e = launch_unord<F>(k_out, nnz, reinterpret_cast<F*>(k_in), st);
-    if (e == hipSuccess) e = hipMemcpyAsync(out->index, i_out, 8 * size_t(nnz), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return hip_fail(e, "range sort (workspace query)");
+    e = segmented_sort_pairs<K>(k_in, k_out, i_in, i_out, nnz, m, ix->ws_c.as<int64_t>(), ix->ws_tmp.p, tmp_bytes,
+                                st);
+    if (e != hipSuccess) return hip_fail(e, "range sort (segmented radix sort)");
+    e = launch_unord<F>(k_out, nnz, reinterpret_cast<F*>(k_in), st);
+    if (e != hipSuccess) return hip_fail(e, "range sort (key decode)");
+    e = hipMemcpyAsync(out->index, i_out, 8 * size_t(nnz), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipMemcpyAsync(out->dist, k_in, es * size_t(nnz), hipMemcpyDeviceToHost, st);
-    if (e != hipSuccess) return hip_fail(e, "range sort");
+    if (e != hipSuccess) return hip_fail(e, "range result download");
   }
   e = hipMemcpyAsync(out->offsets, ix->ws_c.p, sizeof(int64_t) * (m + 1), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
