@@ -34,15 +34,14 @@ int32_t ensure_prepared(fuel_amd_index* ix, int kind, hipStream_t st) {
   const size_t es = kind == 0 ? 2 : 4;
   hipError_t e = ix->Xop.ensure(size_t(n_pad) * kp * es);
   if (e == hipSuccess) e = ix->hx.ensure(sizeof(float) * n_pad);
-  if (e == hipSuccess) e = ix->consts.ensure(64);
+  if (e == hipSuccess) e = ix->consts.ensure(128);
   if (e != hipSuccess) return hip_fail(e, "MFMA operand copy");
-  unsigned long long hc[4] = {0, 0, 0, 0};
+  // 8 u64 bound constants (zero) followed by the two flags (one)
   int hf[2] = {1, 1};
-  char init[64];
+  char init[128];
   std::memset(init, 0, sizeof(init));
-  std::memcpy(init, hc, sizeof(hc));
-  std::memcpy(init + 32, hf, sizeof(hf));
-  e = hipMemcpyAsync(ix->consts.p, init, 64, hipMemcpyHostToDevice, st);
+  std::memcpy(init + 64, hf, sizeof(hf));
+  e = hipMemcpyAsync(ix->consts.p, init, 128, hipMemcpyHostToDevice, st);
   if (e != hipSuccess) return hip_fail(e, "bound constants");
   PrepDbArgs pa{};
   pa.X = ix->X.as<float>();
@@ -55,7 +54,7 @@ int32_t ensure_prepared(fuel_amd_index* ix, int kind, hipStream_t st) {
   pa.Xop = ix->Xop.p;
   pa.hx = ix->hx.as<float>();
   pa.consts = ix->consts.as<unsigned long long>();
-  pa.flags = reinterpret_cast<int*>(ix->consts.as<char>() + 32);
+  pa.flags = reinterpret_cast<int*>(ix->consts.as<char>() + 64);
   e = launch_prep_db(pa, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return hip_fail(e, "MFMA operand preparation");
@@ -197,7 +196,7 @@ int32_t knn_mfma(fuel_amd_index* ix, const float* Q, int64_t m, int64_t ldq, int
     ra.qt2 = qa.qt2;
     ra.qint = qa.qint;
     ra.consts = ix->consts.as<unsigned long long>();
-    ra.flags = reinterpret_cast<const int*>(ix->consts.as<char>() + 32);
+    ra.flags = reinterpret_cast<const int*>(ix->consts.as<char>() + 64);
     ra.KK = KK;
     ra.k = k;
     ra.mode = exclude;

@@ -155,8 +155,10 @@ __device__ __forceinline__ void write_knn_row(const typename Comp<F>::T (&x)[V],
 // ---- mixed-precision candidate path ------------------------------------------
 // Index-level bound constants (nonnegative doubles stored as their bit patterns
 // so that atomicMax on u64 orders them): [0] max ||x||^2, [1] max ||x~ - x||^2,
-// [2] max ||x~||^2 (x~ = the MFMA operand copy).  flags[0] = 1 while every
-// coordinate is an integer of magnitude <= 256, flags[1] = 1 while all are >= 0.
+// [2] max ||x~||^2 (x~ = the MFMA operand copy), [3] max |h1+h2+h3 + ||x||^2/2|
+// and [4] max |h1|+|h2|+|h3| for the bf16 norm columns.  flags (8 bytes after
+// the 8 constants): [0] = 1 while every coordinate is an integer of magnitude
+// <= 256, [1] = 1 while all are >= 0.
 struct PrepDbArgs {
   const float* X;  // resident f32 rows (ldx)
   int64_t ldx;

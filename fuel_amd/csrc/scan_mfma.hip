@@ -75,16 +75,24 @@ struct Mfma<1> {  // f32: four chained 32x32x2 MFMAs per 16-byte operand
   } while (0)
 
 template <int KIND, int KS, int A, int R, int V>
-__global__ __launch_bounds__(256) void scan_mfma_kernel(ScanMfmaArgs a) {
+__device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a) {
   using C = Comp<float>;
+  // bf16 rows carry -||x||^2/2 split into three bf16 columns (queries carry 1s
+  // there), so the MFMA result is already acc = q.x - ||x||^2/2 = -key.
+  constexpr bool FOLD = KIND == 0;
   constexpr int NS = 64 * V;
   constexpr int QPW = 32 * A;
   constexpr int QPB = 4 * QPW;
   constexpr int BN = 32 * R;
-  constexpr int ROWB = KS * 32 + 16;  // LDS row stride in bytes
-  constexpr int CHUNKS = KS * 2;      // 16-byte chunks per row
-  __shared__ __attribute__((aligned(16))) char tile[BN * ROWB];
-  __shared__ __attribute__((aligned(16))) float tile_hx[BN];
+  constexpr int CH = KS * 2;                  // 16-byte chunks of a row
+  constexpr int ROWC = CH + 1;                // + one pad chunk (bank spread)
+  constexpr int ROWB = ROWC * 16;             // LDS row stride in bytes
+  constexpr int TILE_CHUNKS = BN * ROWC;
+  constexpr int WAVE_LOADS = (TILE_CHUNKS + 255) / 256;  // 64-chunk loads per wave per tile
+  constexpr int TILE_BYTES = WAVE_LOADS * 256 * 16;       // rounded up to whole wave loads
+  constexpr int NBUF = (2 * TILE_BYTES <= 65536) ? 2 : 1;
+  __shared__ __attribute__((aligned(16))) char tile[NBUF][TILE_BYTES];
+  __shared__ __attribute__((aligned(16))) float tile_hx[FOLD ? 1 : NBUF][FOLD ? 4 : BN];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -100,6 +108,7 @@ __global__ __launch_bounds__(256) void scan_mfma_kernel(ScanMfmaArgs a) {
   const int KP = a.KP;
   const int csub = a.Csub;
   const size_t kp_bytes = size_t(KS) * 32;
+  const char* xop = reinterpret_cast<const char*>(a.Xop);
 
   // query fragments, resident for the whole scan
   uint4 qf[A][KS];
@@ -111,78 +120,110 @@ __global__ __launch_bounds__(256) void scan_mfma_kernel(ScanMfmaArgs a) {
   }
   float thr[A];
   int cnt[A], lcnt[A];
-  uint64_t* slot[A];
 #pragma unroll
   for (int g = 0; g < A; ++g) {
     thr[g] = INFINITY;
     cnt[g] = 0;
     lcnt[g] = 0;
-    slot[g] = a.slots + (size_t(split) * a.nq_pad + q0 + g * 32 + col) * NS;
   }
 
-  for (int64_t t = r0; t < r1; t += BN) {
-    __syncthreads();
-    for (int c = tid; c < BN * CHUNKS; c += 256) {
-      const int row = c / CHUNKS, part = c - row * CHUNKS;
-      const uint4 v = *reinterpret_cast<const uint4*>(reinterpret_cast<const char*>(a.Xop) +
-                                                      size_t(t + row) * kp_bytes + part * 16);
-      *reinterpret_cast<uint4*>(tile + row * ROWB + part * 16) = v;
+  // Tile staging: direct global -> LDS loads (global_load_lds_dwordx4); lane i
+  // of load j of wave w fills LDS chunk p = (j*4 + w)*64 + i = row*ROWC + c.
+  // The pad chunk (c == CH) and the round-up tail re-read a valid chunk.
+  auto stage = [&](int b, int64_t t) {
+#pragma unroll
+    for (int j = 0; j < WAVE_LOADS; ++j) {
+      const int p = (j * 4 + wave) * 64 + lane;
+      int row = p / ROWC, c = p - row * ROWC;
+      if (c >= CH) c = CH - 1;
+      if (row >= BN) {
+        row = BN - 1;
+        c = CH - 1;
+      }
+      const char* src = xop + size_t(t + row) * kp_bytes + c * 16;
+      __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)(&tile[b][(j * 4 + wave) * 1024]),
+                                       16, 0, 0);
     }
-    if (tid < BN) tile_hx[tid] = a.hx[t + tid];
-    __syncthreads();
+    if (!FOLD && tid < BN) tile_hx[FOLD ? 0 : (NBUF == 2 ? b : 0)][FOLD ? 0 : tid] = a.hx[t + tid];
+  };
+
+  if (r0 < r1) stage(0, r0);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  int buf = 0;
+  for (int64_t t = r0; t < r1; t += BN) {
+    const bool has_next = t + BN < r1;
+    if (NBUF == 2 && has_next) stage(buf ^ 1, t + BN);
 
     floatx16 acc[A][R];
 #pragma unroll
     for (int g = 0; g < A; ++g)
 #pragma unroll
       for (int rb = 0; rb < R; ++rb) acc[g][rb] = floatx16{};
+    const char* tb = tile[NBUF == 2 ? buf : 0];
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
       uint4 af[R];
 #pragma unroll
       for (int rb = 0; rb < R; ++rb)
-        af[rb] = *reinterpret_cast<const uint4*>(tile + (rb * 32 + col) * ROWB + s * 32 + half * 16);
+        af[rb] = *reinterpret_cast<const uint4*>(tb + (rb * 32 + col) * ROWB + s * 32 + half * 16);
 #pragma unroll
       for (int g = 0; g < A; ++g)
 #pragma unroll
         for (int rb = 0; rb < R; ++rb) Mfma<KIND>::step(acc[g][rb], af[rb], qf[g][s]);
     }
 
-    // key = hx - q.x ; row of register r: 8*(r/4) + 4*half + (r%4)
-    float hv[R][16];
-#pragma unroll
-    for (int rb = 0; rb < R; ++rb)
-#pragma unroll
-      for (int q4 = 0; q4 < 4; ++q4) {
-        const float4 h4 = *reinterpret_cast<const float4*>(tile_hx + rb * 32 + q4 * 8 + half * 4);
-        hv[rb][q4 * 4 + 0] = h4.x;
-        hv[rb][q4 * 4 + 1] = h4.y;
-        hv[rb][q4 * 4 + 2] = h4.z;
-        hv[rb][q4 * 4 + 3] = h4.w;
-      }
-    bool any = false;
-#pragma unroll
-    for (int g = 0; g < A; ++g)
+    if constexpr (!FOLD) {
+      // acc = q.x - hx = -key.  Register r of a 32x32 tile holds row
+      // 8*(r/4) + 4*half + (r%4).
 #pragma unroll
       for (int rb = 0; rb < R; ++rb)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) any |= (hv[rb][r] - acc[g][rb][r]) < thr[g];
-    if (__ballot(any)) {
+        for (int q4 = 0; q4 < 4; ++q4) {
+          const float4 h4 =
+              *reinterpret_cast<const float4*>(&tile_hx[FOLD ? 0 : (NBUF == 2 ? buf : 0)][rb * 32 + q4 * 8 + half * 4]);
+          const float hv[4] = {h4.x, h4.y, h4.z, h4.w};
 #pragma unroll
-      for (int g = 0; g < A; ++g) {
+          for (int u = 0; u < 4; ++u)
 #pragma unroll
-        for (int rb = 0; rb < R; ++rb)
+            for (int g = 0; g < A; ++g) acc[g][rb][q4 * 4 + u] = acc[g][rb][q4 * 4 + u] - hv[u];
+        }
+    }
+    // Selection: one max-reduction per (query group, row block); only blocks
+    // with a survivor in some lane take the append path.
+#pragma unroll
+    for (int g = 0; g < A; ++g) {
+      const float nt = -thr[g];
+      bool appended = false;
+#pragma unroll
+      for (int rb = 0; rb < R; ++rb) {
+        float mx = acc[g][rb][0];
+#pragma unroll
+        for (int r = 1; r < 16; ++r) mx = fmaxf(mx, acc[g][rb][r]);
+        if (__ballot(mx > nt)) {
+          appended = true;
+          uint64_t* sl = a.slots + (size_t(split) * a.nq_pad + q0 + g * 32 + col) * NS + KP + half * csub;
 #pragma unroll
           for (int r = 0; r < 16; ++r) {
-            const float key = hv[rb][r] - acc[g][rb][r];
-            if (key < thr[g]) {
+            if (acc[g][rb][r] > nt) {
               const uint32_t row = (uint32_t)(t + rb * 32 + (r >> 2) * 8 + half * 4 + (r & 3));
-              slot[g][KP + half * csub + cnt[g]] = C::make(key, row);
+              sl[cnt[g]] = C::make(-acc[g][rb][r], row);
               ++cnt[g];
             }
           }
-        MERGE_WHERE(g, cnt[g] > csub - 16 * R);
+        }
       }
+      if (appended) MERGE_WHERE(g, cnt[g] > csub - 16 * R);
+    }
+    if constexpr (NBUF == 2) {
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      buf ^= 1;
+    } else {
+      __syncthreads();
+      if (has_next) stage(0, t + BN);
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
     }
   }
 #pragma unroll
@@ -193,16 +234,29 @@ __global__ __launch_bounds__(256) void scan_mfma_kernel(ScanMfmaArgs a) {
   }
 }
 
-// K step buckets: bf16 covers 16 coordinates per step, f32 covers 8.  Up to 16
-// steps a wave holds 2 query groups and the tile has 64 rows; beyond, 1 query
-// group and 32-row tiles keep the query fragments at <= 224 registers and the
-// tile within 64 KB of LDS.
-static const int kBf16Steps[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 32, 40, 49, 56};
+// Two entry points with different register budgets: shapes whose working set
+// fits 256 registers run two waves per SIMD (one wave's MFMAs overlap the other's
+// selection epilogue); wider shapes keep one wave per SIMD and no spills.
+template <int KIND, int KS, int A, int R, int V>
+__global__ __launch_bounds__(256, 2) void scan_mfma_kernel(ScanMfmaArgs a) {
+  scan_mfma_body<KIND, KS, A, R, V>(a);
+}
+template <int KIND, int KS, int A, int R, int V>
+__global__ __launch_bounds__(256) void scan_mfma_kernel_wide(ScanMfmaArgs a) {
+  scan_mfma_body<KIND, KS, A, R, V>(a);
+}
+
+// K step buckets: bf16 covers 16 columns per step (d coordinates + the three
+// norm columns), f32 covers 8.  Up to 16 steps a wave holds 2 query groups and
+// the tile has 64 rows; beyond, 1 query group and 32-row tiles keep the query
+// fragments at <= 224 registers and the two tile buffers within 64 KB of LDS.
+static const int kBf16Steps[] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 14, 16, 17, 20, 24, 32, 40, 50, 56};
 static const int kF32Steps[] = {1, 2, 3, 4, 6, 8, 12, 16, 20, 24, 32};
 
 int mfma_ksteps_bucket(int kind, int d) {
   const int per = kind == 0 ? 16 : 8;
-  int need = (d + per - 1) / per;
+  const int k = kind == 0 ? d + 3 : d;  // bf16 rows carry three extra columns
+  int need = (k + per - 1) / per;
   if (need < 1) need = 1;
   const int* tab = kind == 0 ? kBf16Steps : kF32Steps;
   const int nt = kind == 0 ? (int)(sizeof(kBf16Steps) / sizeof(int)) : (int)(sizeof(kF32Steps) / sizeof(int));
@@ -218,11 +272,19 @@ template <int KIND, int KS>
 static hipError_t launch_ks(const ScanMfmaArgs& a, int V, int nsplit, hipStream_t st) {
   constexpr int A = KS <= 16 ? 2 : 1;
   constexpr int R = KS <= 16 ? 2 : 1;
+  constexpr bool two_waves = KIND == 0 ? KS <= 12 : KS <= 8;
   const dim3 grid((unsigned)(a.nqb * nsplit)), block(256);
-  if (V == 4)
-    hipLaunchKernelGGL((scan_mfma_kernel<KIND, KS, A, R, 4>), grid, block, 0, st, a);
-  else
-    hipLaunchKernelGGL((scan_mfma_kernel<KIND, KS, A, R, 8>), grid, block, 0, st, a);
+  if constexpr (two_waves) {
+    if (V == 4)
+      hipLaunchKernelGGL((scan_mfma_kernel<KIND, KS, A, R, 4>), grid, block, 0, st, a);
+    else
+      hipLaunchKernelGGL((scan_mfma_kernel<KIND, KS, A, R, 8>), grid, block, 0, st, a);
+  } else {
+    if (V == 4)
+      hipLaunchKernelGGL((scan_mfma_kernel_wide<KIND, KS, A, R, 4>), grid, block, 0, st, a);
+    else
+      hipLaunchKernelGGL((scan_mfma_kernel_wide<KIND, KS, A, R, 8>), grid, block, 0, st, a);
+  }
   return hipGetLastError();
 }
 
@@ -233,9 +295,9 @@ hipError_t launch_scan_mfma(const ScanMfmaArgs& a, int kind, int ks, int V, int 
 #define KS_CASE(K) \
   case K:          \
     return launch_ks<0, K>(a, V, nsplit, st);
-      KS_CASE(1) KS_CASE(2) KS_CASE(3) KS_CASE(4) KS_CASE(5) KS_CASE(6) KS_CASE(7) KS_CASE(8) KS_CASE(10)
-      KS_CASE(12) KS_CASE(14) KS_CASE(16) KS_CASE(20) KS_CASE(24) KS_CASE(32) KS_CASE(40) KS_CASE(49)
-      KS_CASE(56)
+      KS_CASE(1) KS_CASE(2) KS_CASE(3) KS_CASE(4) KS_CASE(5) KS_CASE(6) KS_CASE(7) KS_CASE(8) KS_CASE(9)
+      KS_CASE(10) KS_CASE(12) KS_CASE(14) KS_CASE(16) KS_CASE(17) KS_CASE(20) KS_CASE(24) KS_CASE(32)
+      KS_CASE(40) KS_CASE(50) KS_CASE(56)
 #undef KS_CASE
       default:
         return hipErrorInvalidValue;
