@@ -38,6 +38,7 @@ def parse():
     p.add_argument("--m", type=int, default=100_000)
     p.add_argument("--d", type=int, default=128)
     p.add_argument("--k", type=int, default=10)
+    p.add_argument("--widen", type=int, default=4, help="candidates kept per query = widen * k (<= 128)")
     p.add_argument("--verify", type=int, default=64, help="queries checked against the oracle")
     p.add_argument("--cpu-sample", type=int, default=1000, help="queries timed on the CPU baseline (0: skip)")
     return p.parse_args()
@@ -66,7 +67,8 @@ def main():
     out_i = torch.empty((m, k), dtype=torch.int64, device=dev)
     out_d = torch.empty((m, k), dtype=torch.float32, device=dev)
     torch.cuda.synchronize()
-    ix = _lib.Index.from_device(Xd.data_ptr(), X.shape[0], d, d, np.float32, "euclidean", mode=args.mode)
+    ix = _lib.Index.from_device(Xd.data_ptr(), X.shape[0], d, d, np.float32, "euclidean", mode=args.mode,
+                                widen=args.widen)
     stream = torch.cuda.current_stream().cuda_stream
 
     def step():
@@ -140,7 +142,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"kNN db {X.shape[0]}x{d} f32 uniform[0,1), {m} queries, k={k}, euclidean",
                    "mode": args.mode, "splits": stats["splits"], "kp": stats["kp"],
-                   "fallback_queries": stats["fallback_queries"]},
+                   "fallback_queries": stats["fallback_queries"], "exact_rescans": stats["exact_queries"]},
         "verify": {"queries": int(len(sample)), "exact_rows": exact_rows, "recall": recall,
                    "max_rel_err_vs_fp64": max_rel_err},
         "roofline": roofline,

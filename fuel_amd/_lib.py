@@ -58,7 +58,7 @@ class Csr(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [("queries", ctypes.c_int64), ("fallback_queries", ctypes.c_int64),
                 ("mode_used", ctypes.c_int32), ("splits", ctypes.c_int32), ("kp", ctypes.c_int32),
-                ("reserved", ctypes.c_int32), ("ms_total", ctypes.c_double)]
+                ("exact_queries", ctypes.c_int32), ("ms_total", ctypes.c_double)]
 
 
 class FuelAmdError(RuntimeError):
@@ -265,7 +265,7 @@ class Index:
     def stats(self):
         s = Stats()
         check(load().fuel_amd_last_stats(self.handle, ctypes.byref(s)))
-        return {f: getattr(s, f) for f, _ in Stats._fields_ if f != "reserved"}
+        return {f: getattr(s, f) for f, _ in Stats._fields_}
 
 
 def pairwise(data, metric="euclidean", form="square"):

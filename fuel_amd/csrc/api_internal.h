@@ -50,13 +50,18 @@ struct fuel_amd_index {
   fa::DevBuf hq, hout_idx, hout_dist;    // staging for host-pointer calls
   fa::DevBuf slots, lcnts, fail_list;    // scan workspaces
   fa::DevBuf ws_a, ws_b, ws_c, ws_d, ws_e, ws_tmp;  // range / pairwise / outlier workspaces
-  // mixed-precision candidate path (prepared on first use)
-  int mfma_kind = -1;  // 0 bf16, 1 f32, -1 not prepared
-  int mfma_ks = 0;     // K steps of the operand copy
-  int mfma_kp = 0;     // padded K (elements)
+  // mixed-precision candidate path: one operand copy per kind (0 bf16, 1 f32),
+  // prepared on first use, and one query workspace per fallback level
+  struct MfmaOperand {
+    bool ready = false;
+    int ks = 0;  // K steps
+    int kp = 0;  // padded K (elements)
+    fa::DevBuf Xop, hx, consts;  // operand copy, ||x||^2/2, bound constants + flags
+  } mop[2];
   int64_t n_pad = 0;
-  fa::DevBuf Xop, hx, consts;                      // operand copy, ||x||^2/2, bound constants + flags
-  fa::DevBuf Qop, Q32, qnorm, qint, fail_count;    // per-call query workspaces
+  struct QueryWs {
+    fa::DevBuf Qop, Q32, qnorm, qint, slots, lcnts, fail_count, fail_list, gthr;
+  } qws[2];
   fuel_amd_stats stats{};
   // profiling: event pairs around the dominant kernel of each call
   bool timing = false;
@@ -77,15 +82,23 @@ struct fuel_amd_index {
     ws_d.release();
     ws_e.release();
     ws_tmp.release();
-    Xop.release();
-    hx.release();
-    consts.release();
-    Qop.release();
-    Q32.release();
-    qnorm.release();
-    qint.release();
-    fail_count.release();
-    mfma_kind = -1;
+    for (auto& m : mop) {
+      m.Xop.release();
+      m.hx.release();
+      m.consts.release();
+      m.ready = false;
+    }
+    for (auto& w : qws) {
+      w.Qop.release();
+      w.Q32.release();
+      w.qnorm.release();
+      w.qint.release();
+      w.slots.release();
+      w.lcnts.release();
+      w.fail_count.release();
+      w.fail_list.release();
+      w.gthr.release();
+    }
   }
 };
 
@@ -98,8 +111,10 @@ int32_t pairwise_run(fuel_amd_index* ix, int form, void* out);
 int32_t outlier_run(fuel_amd_index* ix, int k, int knn_outlier, void* out_scores);
 // Mixed-precision kNN (f32 indexes): kind 0 bf16 MFMA, 1 f32 MFMA.  Returns
 // FUEL_AMD_ERR_UNSUPPORTED when the shape has no MFMA variant.
+// With d_rows, slot i is query row d_rows[i] of Q and writes output row d_rows[i].
 int32_t knn_mfma(fuel_amd_index* ix, const float* Q, int64_t m, int64_t ldq, int k, int exclude, int kind,
-                 int64_t* out_idx, float* out_dist, hipStream_t st);
+                 int64_t* out_idx, float* out_dist, hipStream_t st, const int64_t* d_rows = nullptr,
+                 int64_t qbase = 0, int level = 0);
 bool mfma_supported(const fuel_amd_index* ix, int kind, int k, int exclude);
 // Exact path on an explicit list of query rows (the certification fallback).
 int32_t knn_exact_rows(fuel_amd_index* ix, const float* Q, int64_t ldq, const int64_t* d_rows, int64_t nrows,

@@ -1,7 +1,8 @@
 // Host orchestration of the mixed-precision kNN path:
-//   prep (once per index) -> per query batch: prep queries -> MFMA candidate
-//   scan with fused selection -> exact rerank + certification -> exact
-//   fallback scan for the queries whose bound did not certify.
+//   prep (once per index and kind) -> per query batch: prep queries -> MFMA
+//   candidate scan with fused selection -> exact rerank + certification ->
+//   the queries whose bound did not certify go one level down the chain
+//   bf16 MFMA -> f32 MFMA -> exact restated scan, which always answers.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,27 +22,27 @@ constexpr int KP_MAX = 128;
 int kp_for(const fuel_amd_index* ix, int k, int exclude) {
   const int KK = k + (exclude != EXCL_NONE ? 1 : 0);
   const int w = ix->opts.widen > 0 ? ix->opts.widen : 4;
-  int kp = std::max(KK, std::min(KP_MAX, w * k));
-  return kp;
+  return std::max(KK, std::min(KP_MAX, w * k));
 }
 
 int32_t ensure_prepared(fuel_amd_index* ix, int kind, hipStream_t st) {
-  if (ix->mfma_kind == kind) return FUEL_AMD_OK;
+  auto& op = ix->mop[kind];
+  if (op.ready) return FUEL_AMD_OK;
   const int ks = mfma_ksteps_bucket(kind, ix->d);
   if (ks < 0) return fail(FUEL_AMD_ERR_UNSUPPORTED, "dimension too large for the MFMA candidate scan");
   const int kp = ks * (kind == 0 ? 16 : 8);
   const int64_t n_pad = std::max<int64_t>(256, (ix->n + 255) & ~int64_t(255));
   const size_t es = kind == 0 ? 2 : 4;
-  hipError_t e = ix->Xop.ensure(size_t(n_pad) * kp * es);
-  if (e == hipSuccess) e = ix->hx.ensure(sizeof(float) * n_pad);
-  if (e == hipSuccess) e = ix->consts.ensure(128);
+  hipError_t e = op.Xop.ensure(size_t(n_pad) * kp * es);
+  if (e == hipSuccess) e = op.hx.ensure(sizeof(float) * n_pad);
+  if (e == hipSuccess) e = op.consts.ensure(128);
   if (e != hipSuccess) return hip_fail(e, "MFMA operand copy");
   // 8 u64 bound constants (zero) followed by the two flags (one)
   int hf[2] = {1, 1};
   char init[128];
   std::memset(init, 0, sizeof(init));
   std::memcpy(init + 64, hf, sizeof(hf));
-  e = hipMemcpyAsync(ix->consts.p, init, 128, hipMemcpyHostToDevice, st);
+  e = hipMemcpyAsync(op.consts.p, init, 128, hipMemcpyHostToDevice, st);
   if (e != hipSuccess) return hip_fail(e, "bound constants");
   PrepDbArgs pa{};
   pa.X = ix->X.as<float>();
@@ -51,24 +52,25 @@ int32_t ensure_prepared(fuel_amd_index* ix, int kind, hipStream_t st) {
   pa.d = ix->d;
   pa.Kp = kp;
   pa.bf16 = kind == 0;
-  pa.Xop = ix->Xop.p;
-  pa.hx = ix->hx.as<float>();
-  pa.consts = ix->consts.as<unsigned long long>();
-  pa.flags = reinterpret_cast<int*>(ix->consts.as<char>() + 64);
+  pa.Xop = op.Xop.p;
+  pa.hx = op.hx.as<float>();
+  pa.consts = op.consts.as<unsigned long long>();
+  pa.flags = reinterpret_cast<int*>(op.consts.as<char>() + 64);
   e = launch_prep_db(pa, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return hip_fail(e, "MFMA operand preparation");
-  ix->mfma_kind = kind;
-  ix->mfma_ks = ks;
-  ix->mfma_kp = kp;
+  op.ready = true;
+  op.ks = ks;
+  op.kp = kp;
   ix->n_pad = n_pad;
   return FUEL_AMD_OK;
 }
 
-// Number of DB splits: fill the resident workgroup slots (one workgroup per CU
-// at this kernel's register use) with as little tail as possible.
-int mfma_splits(int64_t nqb, int64_t n_pad, int cus, int64_t rows_tile) {
-  const int64_t slots = cus;
+// Number of DB splits: fill the resident workgroup slots with as little tail as
+// possible (the splits of a query share its threshold, so extra splits cost
+// only the per-split warm-up).
+int mfma_splits(int64_t nqb, int64_t n_pad, int cus, int64_t rows_tile, int wg_per_cu) {
+  const int64_t slots = int64_t(cus) * wg_per_cu;
   const int64_t max_s = std::max<int64_t>(1, std::min<int64_t>(32, n_pad / (rows_tile * 4)));
   int best = 1;
   double best_eff = -1.0;
@@ -76,7 +78,6 @@ int mfma_splits(int64_t nqb, int64_t n_pad, int cus, int64_t rows_tile) {
     const int64_t units = nqb * s;
     const int64_t rounds = (units + slots - 1) / slots;
     const double eff = double(units) / double(rounds * slots);
-    // prefer fewer splits unless more splits fill the machine clearly better
     if (eff > best_eff + 0.03) {
       best_eff = eff;
       best = (int)s;
@@ -96,28 +97,33 @@ bool mfma_supported(const fuel_amd_index* ix, int kind, int k, int exclude) {
 }
 
 int32_t knn_mfma(fuel_amd_index* ix, const float* Q, int64_t m, int64_t ldq, int k, int exclude, int kind,
-                 int64_t* out_idx, float* out_dist, hipStream_t st) {
+                 int64_t* out_idx, float* out_dist, hipStream_t st, const int64_t* d_rows, int64_t qbase,
+                 int level) {
   int32_t rc = ensure_prepared(ix, kind, st);
   if (rc) return rc;
+  auto& op = ix->mop[kind];
+  auto& ws = ix->qws[level];
   const int KK = k + (exclude != EXCL_NONE ? 1 : 0);
   const int KP = kp_for(ix, k, exclude);
   const int V = KP <= 64 ? 4 : 8;
   const int NS = 64 * V;
   const int csub = (NS - KP) / 2;
-  const int ks = ix->mfma_ks;
-  const int Kp = ix->mfma_kp;
+  const int ks = op.ks;
+  const int Kp = op.kp;
   const int qpb = mfma_queries_per_block(kind, ks);
   const int rows_tile = mfma_rows_per_tile(kind, ks);
+  const int wg_per_cu = mfma_workgroups_per_cu(kind, ks);
   const int cus = device_cu_count(ix->device);
   const size_t es = kind == 0 ? 2 : 4;
   const int64_t ld32 = ix->ld;
+  const bool timed = level == 0;
 
   int64_t done = 0;
   while (done < m) {
     if (cancelled()) return fail(FUEL_AMD_ERR_INTERRUPTED, "interrupted");
     int64_t chunk = m - done;
     int64_t nqb = (chunk + qpb - 1) / qpb;
-    int nsplit = mfma_splits(nqb, ix->n_pad, cus, rows_tile);
+    int nsplit = mfma_splits(nqb, ix->n_pad, cus, rows_tile, wg_per_cu);
     const size_t per_q = size_t(nsplit) * NS * 8;
     const int64_t max_chunk = std::max<int64_t>(qpb, int64_t(MFMA_SLOT_BUDGET / per_q) / qpb * qpb);
     if (chunk > max_chunk) {
@@ -129,51 +135,59 @@ int32_t knn_mfma(fuel_amd_index* ix, const float* Q, int64_t m, int64_t ldq, int
     rps = (rps + rows_tile - 1) / rows_tile * rows_tile;
     nsplit = (int)((ix->n_pad + rps - 1) / rps);
 
-    hipError_t e = ix->Qop.ensure(size_t(nq_pad) * Kp * es);
-    if (e == hipSuccess) e = ix->Q32.ensure(size_t(chunk) * ld32 * 4);
-    if (e == hipSuccess) e = ix->qnorm.ensure(sizeof(double) * 3 * size_t(chunk));
-    if (e == hipSuccess) e = ix->qint.ensure(sizeof(int) * size_t(chunk));
-    if (e == hipSuccess) e = ix->slots.ensure(per_q * nq_pad);
-    if (e == hipSuccess) e = ix->lcnts.ensure(sizeof(int) * size_t(nsplit) * nq_pad);
-    if (e == hipSuccess) e = ix->fail_count.ensure(64);
-    if (e == hipSuccess) e = ix->fail_list.ensure(sizeof(int64_t) * size_t(chunk));
+    hipError_t e = ws.Qop.ensure(size_t(nq_pad) * Kp * es);
+    if (e == hipSuccess) e = ws.Q32.ensure(size_t(chunk) * ld32 * 4);
+    if (e == hipSuccess) e = ws.qnorm.ensure(sizeof(double) * 3 * size_t(chunk));
+    if (e == hipSuccess) e = ws.qint.ensure(sizeof(int) * size_t(chunk));
+    if (e == hipSuccess) e = ws.slots.ensure(per_q * nq_pad);
+    if (e == hipSuccess) e = ws.lcnts.ensure(sizeof(int) * size_t(nsplit) * nq_pad);
+    if (e == hipSuccess) e = ws.fail_count.ensure(64);
+    if (e == hipSuccess) e = ws.fail_list.ensure(sizeof(int64_t) * size_t(chunk));
+    if (e == hipSuccess) e = ws.gthr.ensure(sizeof(uint32_t) * size_t(nq_pad));
     if (e != hipSuccess) return hip_fail(e, "MFMA workspaces");
 
+    // rows of this chunk: an explicit list (fallback levels) or done + i
+    const int64_t* rows = d_rows ? d_rows + done : nullptr;
+    const float* Qb = d_rows ? Q : Q + done * ldq;
+
     PrepQArgs qa{};
-    qa.Q = Q + done * ldq;
+    qa.Q = Qb;
     qa.ldq = ldq;
     qa.m = chunk;
     qa.m_pad = nq_pad;
     qa.d = ix->d;
     qa.Kp = Kp;
     qa.bf16 = kind == 0;
-    qa.Qop = ix->Qop.p;
-    qa.Q32 = ix->Q32.as<float>();
+    qa.Qop = ws.Qop.p;
+    qa.Q32 = ws.Q32.as<float>();
     qa.ld32 = ld32;
-    qa.qn2 = ix->qnorm.as<double>();
+    qa.qn2 = ws.qnorm.as<double>();
     qa.eq2 = qa.qn2 + chunk;
     qa.qt2 = qa.eq2 + chunk;
-    qa.qint = ix->qint.as<int>();
+    qa.qint = ws.qint.as<int>();
+    qa.gthr = ws.gthr.as<uint32_t>();
+    qa.rows = rows;
     e = launch_prep_queries(qa, st);
-    if (e == hipSuccess) e = hipMemsetAsync(ix->fail_count.p, 0, sizeof(int), st);
+    if (e == hipSuccess) e = hipMemsetAsync(ws.fail_count.p, 0, sizeof(int), st);
     if (e != hipSuccess) return hip_fail(e, "query preparation");
 
     ScanMfmaArgs sa{};
-    sa.Xop = ix->Xop.p;
-    sa.hx = ix->hx.as<float>();
+    sa.Xop = op.Xop.p;
+    sa.hx = op.hx.as<float>();
     sa.Kp = Kp;
     sa.n_pad = ix->n_pad;
-    sa.Qop = ix->Qop.p;
+    sa.Qop = ws.Qop.p;
     sa.nq_pad = nq_pad;
     sa.nqb = (int)nqb;
     sa.rows_per_split = rps;
     sa.KP = KP;
     sa.Csub = csub;
-    sa.slots = ix->slots.as<uint64_t>();
-    sa.lcnts = ix->lcnts.as<int>();
-    timing_mark(ix, st, true, kind == 0 ? "scan_mfma_kernel<bf16>" : "scan_mfma_kernel<f32>");
+    sa.slots = ws.slots.as<uint64_t>();
+    sa.lcnts = ws.lcnts.as<int>();
+    sa.gthr = ws.gthr.as<uint32_t>();
+    if (timed) timing_mark(ix, st, true, kind == 0 ? "scan_mfma_kernel<bf16>" : "scan_mfma_kernel<f32>");
     e = launch_scan_mfma(sa, kind, ks, V, nsplit, st);
-    timing_mark(ix, st, false, nullptr);
+    if (timed) timing_mark(ix, st, false, nullptr);
     if (e != hipSuccess) return hip_fail(e, "MFMA scan launch");
 
     RerankArgs ra{};
@@ -195,31 +209,40 @@ int32_t knn_mfma(fuel_amd_index* ix, const float* Q, int64_t m, int64_t ldq, int
     ra.eq2 = qa.eq2;
     ra.qt2 = qa.qt2;
     ra.qint = qa.qint;
-    ra.consts = ix->consts.as<unsigned long long>();
-    ra.flags = reinterpret_cast<const int*>(ix->consts.as<char>() + 64);
+    ra.consts = op.consts.as<unsigned long long>();
+    ra.flags = reinterpret_cast<const int*>(op.consts.as<char>() + 64);
     ra.KK = KK;
     ra.k = k;
     ra.mode = exclude;
-    ra.qbase = done;
-    ra.out_idx = out_idx + done * k;
-    ra.out_dist = out_dist + done * k;
-    ra.fail_count = ix->fail_count.as<int>();
-    ra.fail_list = ix->fail_list.as<int64_t>();
+    ra.qbase = d_rows ? qbase : qbase + done;
+    ra.rows = rows;
+    ra.out_idx = d_rows ? out_idx : out_idx + done * k;
+    ra.out_dist = d_rows ? out_dist : out_dist + done * k;
+    ra.fail_count = ws.fail_count.as<int>();
+    ra.fail_list = ws.fail_list.as<int64_t>();
     e = launch_rerank(ra, V, st);
     if (e != hipSuccess) return hip_fail(e, "rerank launch");
 
     int nfail = 0;
-    e = hipMemcpyAsync(&nfail, ix->fail_count.p, sizeof(int), hipMemcpyDeviceToHost, st);
+    e = hipMemcpyAsync(&nfail, ws.fail_count.p, sizeof(int), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(e, "certification count");
     if (nfail > 0) {
-      rc = knn_exact_rows(ix, qa.Q32, ld32, ra.fail_list, nfail, done, k, exclude, out_idx + done * k,
-                          out_dist + done * k, st);
+      // the fail list holds query rows relative to Qb and to this chunk's outputs
+      if (kind == 0 && level == 0 && mfma_supported(ix, 1, k, exclude)) {
+        rc = knn_mfma(ix, Qb, nfail, ldq, k, exclude, 1, ra.out_idx, ra.out_dist, st, ra.fail_list, ra.qbase, 1);
+      } else {
+        rc = knn_exact_rows(ix, Qb, ldq, ra.fail_list, nfail, ra.qbase, k, exclude, ra.out_idx, ra.out_dist, st);
+      }
       if (rc) return rc;
     }
-    ix->stats.fallback_queries += nfail;
-    ix->stats.splits = nsplit;
-    ix->stats.kp = KP;
+    if (level == 0) {
+      ix->stats.fallback_queries += nfail;
+      ix->stats.splits = nsplit;
+      ix->stats.kp = KP;
+    } else {
+      ix->stats.exact_queries += nfail;  // queries that needed the exact scan after the f32 level
+    }
     done += chunk;
   }
   return FUEL_AMD_OK;

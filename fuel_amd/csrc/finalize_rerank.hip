@@ -174,12 +174,13 @@ __global__ __launch_bounds__(64) void rerank_kernel(RerankArgs a) {
     const float kk = C::key(wave_get<T, V>(y, a.KK - 1));
     cert = certify(a, qs, T_C, kk);
   }
+  const int64_t orow = a.rows ? a.rows[qs] : qs;
   if (cert) {
     const int cnt = ncand < a.KK ? ncand : a.KK;
-    write_knn_row<float, V>(y, cnt, a.qbase + qs, a.k, a.mode, a.out_idx + qs * a.k, a.out_dist + qs * a.k);
+    write_knn_row<float, V>(y, cnt, a.qbase + orow, a.k, a.mode, a.out_idx + orow * a.k, a.out_dist + orow * a.k);
   } else if (lane == 0) {
     const int pos = atomicAdd(a.fail_count, 1);
-    a.fail_list[pos] = qs;
+    a.fail_list[pos] = orow;
   }
 }
 

@@ -188,6 +188,8 @@ struct PrepQArgs {
   double* eq2;  // ||q~ - q||^2
   double* qt2;  // ||q~||^2
   int* qint;    // 1 if integer-valued with |v| <= 256
+  uint32_t* gthr;  // m_pad shared thresholds, initialised to +inf here
+  const int64_t* rows;  // optional: slot i reads query row rows[i] of Q
 };
 
 struct ScanMfmaArgs {
@@ -203,6 +205,7 @@ struct ScanMfmaArgs {
   int Csub;                // per-lane-half append buffer capacity
   uint64_t* slots;         // [nsplit][nq_pad][NS]
   int* lcnts;              // [nsplit][nq_pad]
+  uint32_t* gthr;          // [nq_pad] per-query threshold shared by the splits (order-preserving bits)
 };
 
 struct RerankArgs {
@@ -230,10 +233,11 @@ struct RerankArgs {
   int k;
   int mode;
   int64_t qbase;
+  const int64_t* rows;  // optional: slot i is query row rows[i] (output row and self index)
   int64_t* out_idx;
   float* out_dist;
   int* fail_count;
-  int64_t* fail_list;
+  int64_t* fail_list;  // receives the query rows (not slots) that did not certify
 };
 
 hipError_t launch_prep_db(const PrepDbArgs& a, hipStream_t st);
@@ -247,6 +251,7 @@ hipError_t launch_rerank(const RerankArgs& a, int V, hipStream_t st);
 int mfma_ksteps_bucket(int kind, int d);
 int mfma_queries_per_block(int kind, int ksteps);
 int mfma_rows_per_tile(int kind, int ksteps);
+int mfma_workgroups_per_cu(int kind, int ksteps);
 
 template <typename F>
 hipError_t launch_scan_exact(const ScanExactArgs<F>& a, int qw, int nsplit, hipStream_t st);

@@ -125,6 +125,7 @@ __global__ __launch_bounds__(256) void prep_query_kernel(PrepQArgs a) {
   const int lane = __lane_id();
   const size_t es = a.bf16 ? 2 : 4;
   char* dst = reinterpret_cast<char*>(a.Qop) + size_t(row) * a.Kp * es;
+  if (lane == 0 && a.gthr) a.gthr[row] = 0xFF800000u;  // order-preserving bits of +inf
   if (row >= a.m) {
     for (int c = lane; c < a.Kp; c += 64) {
       if (a.bf16)
@@ -134,7 +135,7 @@ __global__ __launch_bounds__(256) void prep_query_kernel(PrepQArgs a) {
     }
     return;
   }
-  const float* src = a.Q + row * a.ldq;
+  const float* src = a.Q + (a.rows ? a.rows[row] : row) * a.ldq;
   for (int c = lane; c < a.ld32; c += 64) a.Q32[row * a.ld32 + c] = c < a.d ? src[c] : 0.0f;
   const RowStats r = prep_row(src, a.d, a.Kp, a.bf16, dst);
   if (lane == 0 && a.bf16) {  // 1 in the three norm columns

@@ -65,6 +65,7 @@ struct Mfma<1> {  // f32: four chained 32x32x2 MFMAs per 16-byte operand
       uint64_t kth_;                                                                      \
       const int nl_ = merge_slot2<uint64_t, V>(sl_, KP, lc_, csub, c0_, c1_, kth_);       \
       const float nthr_ = nl_ == KP ? C::key(kth_) : INFINITY;                            \
+      if (nl_ == KP && lane == j_) atomicMin(&a.gthr[q0 + (g) * 32 + j_], (uint32_t)(kth_ >> 32)); \
       if (col == j_) {                                                                    \
         cnt[g] = 0;                                                                       \
         lcnt[g] = nl_;                                                                    \
@@ -190,9 +191,15 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a) {
         }
     }
     // Selection: one max-reduction per (query group, row block); only blocks
-    // with a survivor in some lane take the append path.
+    // with a survivor in some lane take the append path.  The threshold is the
+    // smaller of this split's KP-th key and the one other splits of the query
+    // published: any split's KP-th key bounds the KP-th key of the union of
+    // all splits' lists, so rows above it can never be needed.
 #pragma unroll
     for (int g = 0; g < A; ++g) {
+      const uint32_t shared = __hip_atomic_load(&a.gthr[q0 + g * 32 + col], __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+      thr[g] = fminf(thr[g], unord_f32(shared));
       const float nt = -thr[g];
       bool appended = false;
 #pragma unroll
@@ -267,6 +274,7 @@ int mfma_ksteps_bucket(int kind, int d) {
 
 int mfma_queries_per_block(int kind, int ksteps) { return 4 * 32 * (ksteps <= 16 ? 2 : 1); }
 int mfma_rows_per_tile(int kind, int ksteps) { return 32 * (ksteps <= 16 ? 2 : 1); }
+int mfma_workgroups_per_cu(int kind, int ksteps) { return (kind == 0 ? ksteps <= 12 : ksteps <= 8) ? 2 : 1; }
 
 template <int KIND, int KS>
 static hipError_t launch_ks(const ScanMfmaArgs& a, int V, int nsplit, hipStream_t st) {

@@ -87,11 +87,11 @@ typedef struct fuel_amd_csr {
 /* Statistics of the last kNN call on an index (certification / fallback). */
 typedef struct fuel_amd_stats {
   int64_t queries;
-  int64_t fallback_queries; /* queries rescanned exactly because the bound did not certify */
+  int64_t fallback_queries; /* queries whose first-level bound did not certify */
   int32_t mode_used;
   int32_t splits;
   int32_t kp;
-  int32_t reserved;
+  int32_t exact_queries;    /* of those, queries that needed the exact restated scan */
   double ms_total;
 } fuel_amd_stats;
 
