@@ -41,11 +41,28 @@ def parse():
     p.add_argument("--widen", type=int, default=4, help="candidates kept per query = widen * k (<= 128)")
     p.add_argument("--verify", type=int, default=64, help="queries checked against the oracle")
     p.add_argument("--cpu-sample", type=int, default=1000, help="queries timed on the CPU baseline (0: skip)")
+    p.add_argument("--copy-probe", type=int, default=0,
+                   help="only run a plain device copy of this many bytes 3 times (calibrates HBM counters)")
     return p.parse_args()
+
+
+def copy_probe(nbytes):
+    """A plain device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
+    readings per dispatch convert the counters of the kNN kernels into bytes."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes}), flush=True)
 
 
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported",
                           "n_gpus": args.gpus}), flush=True)

@@ -71,7 +71,7 @@ int32_t ensure_prepared(fuel_amd_index* ix, int kind, hipStream_t st) {
 // only the per-split warm-up).
 int mfma_splits(int64_t nqb, int64_t n_pad, int cus, int64_t rows_tile, int wg_per_cu) {
   const int64_t slots = int64_t(cus) * wg_per_cu;
-  const int64_t max_s = std::max<int64_t>(1, std::min<int64_t>(32, n_pad / (rows_tile * 4)));
+  const int64_t max_s = std::max<int64_t>(1, std::min<int64_t>(128, n_pad / (rows_tile * 4)));
   int best = 1;
   double best_eff = -1.0;
   for (int64_t s = 1; s <= max_s; ++s) {
