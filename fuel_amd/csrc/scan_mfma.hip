@@ -75,6 +75,34 @@ struct Mfma<1> {  // f32: four chained 32x32x2 MFMAs per 16-byte operand
     }                                                                                     \
   } while (0)
 
+// Asynchronous global -> LDS copy of 16 (or 4) bytes per lane into
+// lds_base + 16*lane (4*lane).  Issued from inline assembly so that the compiler
+// does not serialise later LDS reads of *other* buffers behind it; completion is
+// awaited explicitly (s_waitcnt) before the barrier that publishes the buffer.
+// M0 (the LDS base operand, latched at issue) is saved and restored.
+__device__ __forceinline__ void lds_dma16(uint32_t lds_base, const void* src) {
+  uint32_t tmp;
+  asm volatile(
+      "s_mov_b32 %0, m0\n\t"
+      "s_mov_b32 m0, %1\n\t"
+      "global_load_lds_dwordx4 %2, off\n\t"
+      "s_mov_b32 m0, %0"
+      : "=&s"(tmp)
+      : "s"(lds_base), "v"(src)
+      : "memory");
+}
+__device__ __forceinline__ void lds_dma4(uint32_t lds_base, const void* src) {
+  uint32_t tmp;
+  asm volatile(
+      "s_mov_b32 %0, m0\n\t"
+      "s_mov_b32 m0, %1\n\t"
+      "global_load_lds_dword %2, off\n\t"
+      "s_mov_b32 m0, %0"
+      : "=&s"(tmp)
+      : "s"(lds_base), "v"(src)
+      : "memory");
+}
+
 template <int KIND, int KS, int A, int R, int V>
 __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a) {
   using C = Comp<float>;
@@ -93,7 +121,8 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a) {
   constexpr int TILE_BYTES = WAVE_LOADS * 256 * 16;       // rounded up to whole wave loads
   constexpr int NBUF = (2 * TILE_BYTES <= 65536) ? 2 : 1;
   __shared__ __attribute__((aligned(16))) char tile[NBUF][TILE_BYTES];
-  __shared__ __attribute__((aligned(16))) float tile_hx[FOLD ? 1 : NBUF][FOLD ? 4 : BN];
+  // 64 floats per buffer: the hx DMA writes one dword per lane of a wave
+  __shared__ __attribute__((aligned(16))) float tile_hx[FOLD ? 1 : NBUF][FOLD ? 4 : 64];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -128,34 +157,41 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a) {
     lcnt[g] = 0;
   }
 
-  // Tile staging: direct global -> LDS loads (global_load_lds_dwordx4); lane i
-  // of load j of wave w fills LDS chunk p = (j*4 + w)*64 + i = row*ROWC + c.
-  // The pad chunk (c == CH) and the round-up tail re-read a valid chunk.
-  auto stage = [&](int b, int64_t t) {
+  // Tile staging: lane i of DMA j of wave w fills LDS chunk p = (j*4 + w)*64 + i
+  // = row*ROWC + c; the pad chunk (c == CH) and the round-up tail re-read a
+  // valid chunk.  The per-lane source offsets do not depend on the tile.
+  uint32_t soff[WAVE_LOADS];
 #pragma unroll
-    for (int j = 0; j < WAVE_LOADS; ++j) {
-      const int p = (j * 4 + wave) * 64 + lane;
-      int row = p / ROWC, c = p - row * ROWC;
-      if (c >= CH) c = CH - 1;
-      if (row >= BN) {
-        row = BN - 1;
-        c = CH - 1;
-      }
-      const char* src = xop + size_t(t + row) * kp_bytes + c * 16;
-      __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)(&tile[b][(j * 4 + wave) * 1024]),
-                                       16, 0, 0);
+  for (int j = 0; j < WAVE_LOADS; ++j) {
+    const int p = (j * 4 + wave) * 64 + lane;
+    int row = p / ROWC, c = p - row * ROWC;
+    if (c >= CH) c = CH - 1;
+    if (row >= BN) {
+      row = BN - 1;
+      c = CH - 1;
     }
-    if (!FOLD && tid < BN) tile_hx[FOLD ? 0 : (NBUF == 2 ? b : 0)][FOLD ? 0 : tid] = a.hx[t + tid];
+    soff[j] = (uint32_t)(row * (KS * 32) + c * 16);
+  }
+  const uint32_t lds_tile = (uint32_t)(uintptr_t)&tile[0][0];
+  const uint32_t lds_hx = (uint32_t)(uintptr_t)&tile_hx[0][0];
+  auto stage = [&](int b, int64_t t) {
+    const char* base = xop + size_t(t) * kp_bytes;
+#pragma unroll
+    for (int j = 0; j < WAVE_LOADS; ++j)
+      lds_dma16(__builtin_amdgcn_readfirstlane(lds_tile + b * TILE_BYTES + (j * 4 + wave) * 1024), base + soff[j]);
+    if (!FOLD && wave == 0) {
+      const int r = lane < BN ? lane : BN - 1;
+      lds_dma4(__builtin_amdgcn_readfirstlane(lds_hx + (NBUF == 2 ? b : 0) * 64 * 4), a.hx + t + r);
+    }
   };
 
+  int64_t it = 0;
   if (r0 < r1) stage(0, r0);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
+  if (NBUF == 2 && r0 + BN < r1) stage(1, r0 + BN);
   int buf = 0;
-  for (int64_t t = r0; t < r1; t += BN) {
-    const bool has_next = t + BN < r1;
-    if (NBUF == 2 && has_next) stage(buf ^ 1, t + BN);
-
+  for (int64_t t = r0; t < r1; t += BN, ++it) {
     floatx16 acc[A][R];
 #pragma unroll
     for (int g = 0; g < A; ++g)
@@ -197,9 +233,6 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a) {
     // all splits' lists, so rows above it can never be needed.
 #pragma unroll
     for (int g = 0; g < A; ++g) {
-      const uint32_t shared = __hip_atomic_load(&a.gthr[q0 + g * 32 + col], __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_AGENT);
-      thr[g] = fminf(thr[g], unord_f32(shared));
       const float nt = -thr[g];
       bool appended = false;
 #pragma unroll
@@ -222,13 +255,29 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a) {
       }
       if (appended) MERGE_WHERE(g, cnt[g] > csub - 16 * R);
     }
+    // Every 8 tiles, adopt the thresholds the query's other splits published
+    // (read here, where no tile DMA is in flight, so the load does not hold up
+    // the prefetch).
+    auto refresh = [&]() {
+      if ((it & 7) == 7) {
+#pragma unroll
+        for (int g = 0; g < A; ++g)
+          thr[g] = fminf(thr[g], unord_f32(__hip_atomic_load(&a.gthr[q0 + g * 32 + col], __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT)));
+      }
+    };
     if constexpr (NBUF == 2) {
+      // the next tile's DMA (issued one iteration ago) has landed for every
+      // wave; this tile's buffer is free again and takes the tile after next
       __builtin_amdgcn_s_waitcnt(0);
+      refresh();
       __syncthreads();
+      if (t + 2 * BN < r1) stage(buf, t + 2 * BN);
       buf ^= 1;
     } else {
+      refresh();
       __syncthreads();
-      if (has_next) stage(0, t + BN);
+      if (t + BN < r1) stage(0, t + BN);
       __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
     }
