@@ -54,6 +54,7 @@ struct fuel_amd_index {
   // prepared on first use, and one query workspace per fallback level
   struct MfmaOperand {
     bool ready = false;
+    bool usable = true;  // false when some squared norm reaches MFMA_NORM2_LIMIT
     int ks = 0;  // K steps
     int kp = 0;  // padded K (elements)
     fa::DevBuf Xop, hx, consts;  // operand copy, ||x||^2/2, bound constants + flags

@@ -57,8 +57,13 @@ int32_t ensure_prepared(fuel_amd_index* ix, int kind, hipStream_t st) {
   pa.consts = op.consts.as<unsigned long long>();
   pa.flags = reinterpret_cast<int*>(op.consts.as<char>() + 64);
   e = launch_prep_db(pa, st);
+  unsigned long long x2bits = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&x2bits, op.consts.p, sizeof(x2bits), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return hip_fail(e, "MFMA operand preparation");
+  double x2;
+  std::memcpy(&x2, &x2bits, sizeof(x2));
+  op.usable = x2 < MFMA_NORM2_LIMIT;
   op.ready = true;
   op.ks = ks;
   op.kp = kp;
@@ -102,6 +107,11 @@ int32_t knn_mfma(fuel_amd_index* ix, const float* Q, int64_t m, int64_t ldq, int
   int32_t rc = ensure_prepared(ix, kind, st);
   if (rc) return rc;
   auto& op = ix->mop[kind];
+  if (!op.usable) {
+    // norms beyond the scan's finite sentinels: every query takes the exact scan
+    ix->stats.exact_queries += m;
+    return knn_exact_rows(ix, Q, ldq, d_rows, m, qbase, k, exclude, out_idx, out_dist, st);
+  }
   auto& ws = ix->qws[level];
   const int KK = k + (exclude != EXCL_NONE ? 1 : 0);
   const int KP = kp_for(ix, k, exclude);

@@ -17,14 +17,15 @@
 //     (sqrt(B) > key for 'euclidean', B > key for 'sqeuclidean'), no excluded
 //     row can enter the answer, ties included, and the top KK is fuel's exact
 //     answer.  Otherwise the query is queued for the exact fallback scan.
-//  E_s collects, for the bf16 form (key = -MFMA([x~, h] . [q~, 1]) with the three
-//  bf16 pieces h of -||x||^2/2): operand rounding |q~.x~ - q.x| <= |q| |x~-x| +
-//  |q~-q| |x| + |q~-q| |x~-x|; MFMA accumulation <= 4 (d+5) u (|q~| |x~| + |h|)
-//  (zero when every partial sum is a multiple of 1/2 below 2^23); the split
-//  error |sum(h) + ||x||^2/2|.  For the f32 form (key = fl(hx - g)): accumulation
-//  <= 4 (d+2) u |q| |x| and the f32 roundings of ||x||^2/2 and of the
-//  subtraction.  u = 2^-24.  All of it is evaluated in f64 with directed slack;
-//  the index-wide maxima make the bound uniform over the excluded rows.
+//  E_s collects, for key~ = -(MFMA of x~ . q~ with C input -fl32(||x||^2/2)),
+//  low 4 bits replaced by the row tag: operand rounding |q~.x~ - q.x| <= |q|
+//  |x~-x| + |q~-q| |x| + |q~-q| |x~-x| (zero for the f32 operands); the
+//  rounding of fl32(||x||^2/2); MFMA accumulation <= 4 (d+2) u (|q~| |x~| + H),
+//  H = max fl32(||x||^2/2) (zero when every partial sum is a multiple of 1/2
+//  below 2^23, which holds for integer data with small norms); the tag, <= 15
+//  ulps of the result, <= 2^-19 (|q~| |x~| + H) + 15 * 2^-149.  u = 2^-24.  All of it is
+//  evaluated in f64 with directed slack; the index-wide maxima make the bound
+//  uniform over the excluded rows.
 #include "common.h"
 #include "kernels.h"
 
@@ -65,32 +66,21 @@ __device__ __forceinline__ bool certify(const RerankArgs& a, int64_t qs, float T
   const double ex = sqrt(__longlong_as_double((long long)a.consts[1])) * (1 + 0x1p-50);
   const double xt = sqrt(__longlong_as_double((long long)a.consts[2])) * (1 + 0x1p-50);
   const double xn = sqrt(X2) * (1 + 0x1p-50);
+  // beyond these norms keys may reach the scan's finite threshold sentinel
+  if (!(X2 < MFMA_NORM2_LIMIT) || !(a.qn2[qs] < MFMA_NORM2_LIMIT)) return false;
   const double qn2_lo = a.qn2[qs] * (1 - sl);
   const double qn = sqrt(a.qn2[qs] * (1 + sl)) * (1 + 0x1p-50);
   const double eq = sqrt(a.eq2[qs] * (1 + sl)) * (1 + 0x1p-50);
   const double qt = sqrt(a.qt2[qs] * (1 + sl)) * (1 + 0x1p-50);
   const bool int_regime = a.flags[0] != 0 && a.qint[qs] != 0;
   const double dot_mag = qt * xt * (1 + 0x1p-50);
-  double e_s;
-  if (a.bf16) {
-    // key = -(MFMA of [x~, h1, h2, h3] . [q~, 1, 1, 1]) with h1+h2+h3 ~ -||x||^2/2
-    const double e_hx = __longlong_as_double((long long)a.consts[3]);
-    const double H = __longlong_as_double((long long)a.consts[4]);
-    const bool mfma_exact = int_regime && e_hx == 0.0 && dot_mag + H < 0x1p23;
-    const double gacc = mfma_exact ? 0.0 : 4.0 * (d + 5) * u;
-    const double e_round = qn * ex + eq * xn + eq * ex;
-    e_s = e_round + gacc * (dot_mag + H) + e_hx + (mfma_exact ? 0.0 : 2.0 * (d + 5) * 0x1p-126);
-  } else {
-    // key = fl(hx - g), g = f32 MFMA of q.x, hx = fl32(||x||^2 / 2)
-    const bool mfma_exact = int_regime && dot_mag < 0x1p24;
-    const double gacc = mfma_exact ? 0.0 : 4.0 * (d + 2) * u;
-    const double e_g = gacc * dot_mag + (mfma_exact ? 0.0 : 2.0 * (d + 2) * 0x1p-126);
-    const double hx_err = (mfma_exact && X2 < 0x1p24) ? 0.0 : 2.0 * u * X2 * 0.5;
-    const double sub_err = (mfma_exact && X2 * 0.5 + dot_mag < 0x1p23)
-                               ? 0.0
-                               : u * (X2 * 0.5 * (1 + 2 * u) + dot_mag * (1 + gacc) + e_g);
-    e_s = hx_err + e_g + sub_err;
-  }
+  const double e_hx = __longlong_as_double((long long)a.consts[3]);
+  const double H = __longlong_as_double((long long)a.consts[4]);
+  const bool mfma_exact = int_regime && e_hx == 0.0 && dot_mag + H < 0x1p23;
+  const double gacc = mfma_exact ? 0.0 : 4.0 * (d + 2) * u;
+  const double e_round = qn * ex + eq * xn + eq * ex;
+  const double e_tag = 0x1p-19 * (dot_mag + H) * (1 + 0x1p-10) + 0x1p-145;
+  double e_s = e_round + e_hx + gacc * (dot_mag + H) + e_tag + (mfma_exact ? 0.0 : 2.0 * (d + 2) * 0x1p-126);
   e_s *= (1 + 0x1p-40);
   double L = 2.0 * (double)T + qn2_lo;
   L -= (fabs(2.0 * (double)T) + qn2_lo) * 0x1p-50;

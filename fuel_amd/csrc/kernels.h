@@ -155,8 +155,8 @@ __device__ __forceinline__ void write_knn_row(const typename Comp<F>::T (&x)[V],
 // ---- mixed-precision candidate path ------------------------------------------
 // Index-level bound constants (nonnegative doubles stored as their bit patterns
 // so that atomicMax on u64 orders them): [0] max ||x||^2, [1] max ||x~ - x||^2,
-// [2] max ||x~||^2 (x~ = the MFMA operand copy), [3] max |h1+h2+h3 + ||x||^2/2|
-// and [4] max |h1|+|h2|+|h3| for the bf16 norm columns.  flags (8 bytes after
+// [2] max ||x~||^2 (x~ = the MFMA operand copy), [3] max |fl32(||x||^2/2) -
+// ||x||^2/2| and [4] max fl32(||x||^2/2) (the MFMA's C input).  flags (8 bytes after
 // the 8 constants): [0] = 1 while every coordinate is an integer of magnitude
 // <= 256, [1] = 1 while all are >= 0.
 struct PrepDbArgs {
@@ -168,7 +168,7 @@ struct PrepDbArgs {
   int Kp;
   int bf16;    // operand type: 1 bf16, 0 f32
   void* Xop;   // n_pad x Kp operand copy
-  float* hx;   // n_pad: fl32(||x||^2 / 2), +inf for padding rows
+  float* hx;   // n_pad: -fl32(||x||^2 / 2), MFMA_PAD_HXN for padding rows
   unsigned long long* consts;
   int* flags;
 };
@@ -191,6 +191,16 @@ struct PrepQArgs {
   uint32_t* gthr;  // m_pad shared thresholds, initialised to +inf here
   const int64_t* rows;  // optional: slot i reads query row rows[i] of Q
 };
+
+// Finite sentinels of the MFMA scan.  Thresholds start at KEY_LIMIT instead of
+// +inf and padding rows carry PAD_HXN as their C input, so a padding row's
+// result (PAD_HXN) never beats a threshold and every value stays finite, which
+// the row tag in the low mantissa bits requires (a tagged infinity would be a
+// NaN).  The scan is used only while every squared norm stays below NORM2_LIMIT,
+// so that real keys stay far below KEY_LIMIT.
+constexpr float MFMA_KEY_LIMIT = 1.0e38f;
+constexpr float MFMA_PAD_HXN = -3.0e38f;
+constexpr double MFMA_NORM2_LIMIT = 1.0e36;
 
 struct ScanMfmaArgs {
   const void* Xop;

@@ -1,9 +1,11 @@
 // Preparation of the MFMA operands and of the quantities the error bound needs.
 //
-// DB rows (once per index): the bf16 operand copy x~ (round to nearest even),
-// hx = fl32(||x||^2 / 2) (+inf for padding rows, which then never qualify), and
-// index-wide maxima of ||x||^2, ||x~ - x||^2 and ||x~||^2 in f64, plus flags
-// "all coordinates are integers of magnitude <= 256" and "all >= 0".
+// DB rows (once per index and operand kind): the operand copy x~ (bf16 by round
+// to nearest even, or the f32 values), hxn = -fl32(||x||^2 / 2) (MFMA_PAD_HXN
+// for padding rows, whose result then never beats a threshold),
+// and index-wide maxima of ||x||^2, ||x~ - x||^2, ||x~||^2, |fl32(||x||^2/2) -
+// ||x||^2/2| and |fl32(||x||^2/2)| in f64, plus the flags "all coordinates are
+// integers of magnitude <= 256" and "all >= 0".
 // Queries (per call): the operand copy q~, an aligned f32 copy for the exact
 // rerank, and ||q||^2, ||q~ - q||^2, ||q~||^2, the integer flag.
 // Sums of squares of f32 values are accumulated in f64; each product is exact
@@ -79,41 +81,22 @@ __global__ __launch_bounds__(256) void prep_db_kernel(PrepDbArgs a) {
       else
         reinterpret_cast<float*>(dst)[c] = 0.0f;
     }
-    if (lane == 0) {
-      a.hx[row] = INFINITY;
-      // padding rows: -inf in the norm column makes their MFMA result -inf, so
-      // their key (+inf) never passes a threshold
-      if (a.bf16) reinterpret_cast<uint16_t*>(dst)[a.d] = 0xFF80;
-    }
+    if (lane == 0) a.hx[row] = MFMA_PAD_HXN;
     return;
   }
   const RowStats r = prep_row(a.X + row * a.ldx, a.d, a.Kp, a.bf16, dst);
   if (lane == 0) {
     const double v = r.s2 * 0.5;
-    a.hx[row] = (float)v;
+    const float h = (float)v;
+    a.hx[row] = -h;
     const double slack = 1.0 + (a.d + 4) * 0x1p-52;
     atomic_max_nonneg(&a.consts[0], r.s2 * slack);
     atomic_max_nonneg(&a.consts[1], r.e2 * slack);
     atomic_max_nonneg(&a.consts[2], r.t2 * slack);
-    if (a.bf16) {
-      // -||x||^2/2 as three bf16 pieces (exact for values of <= 24 significant
-      // bits); the queries hold 1 in these columns.
-      uint16_t* h = reinterpret_cast<uint16_t*>(dst) + a.d;
-      const uint16_t h1 = bf16_rne((float)-v);
-      const double r1 = -v - (double)bf16_val(h1);
-      const uint16_t h2 = bf16_rne((float)r1);
-      const double r2 = r1 - (double)bf16_val(h2);
-      const uint16_t h3 = bf16_rne((float)r2);
-      h[0] = h1;
-      h[1] = h2;
-      h[2] = h3;
-      const double hs = (double)bf16_val(h1) + (double)bf16_val(h2) + (double)bf16_val(h3);
-      // |split - ||x||^2/2| including the f64 error of the sum of squares
-      const double err = fabs(hs + v) + (v > 0 ? v : 0) * (a.d + 4) * 0x1p-53;
-      atomic_max_nonneg(&a.consts[3], err);
-      atomic_max_nonneg(&a.consts[4], fabs((double)bf16_val(h1)) + fabs((double)bf16_val(h2)) +
-                                          fabs((double)bf16_val(h3)));
-    }
+    // |fl32(||x||^2/2) - ||x||^2/2|, including the f64 error of the sum of
+    // squares (none for integer rows: every partial sum is an integer < 2^53)
+    atomic_max_nonneg(&a.consts[3], fabs((double)h - v) + (r.isint ? 0.0 : v * (a.d + 4) * 0x1p-53));
+    atomic_max_nonneg(&a.consts[4], (double)h);
     if (!r.isint) atomicAnd(&a.flags[0], 0);
     if (!r.nonneg) atomicAnd(&a.flags[1], 0);
   }
@@ -138,12 +121,6 @@ __global__ __launch_bounds__(256) void prep_query_kernel(PrepQArgs a) {
   const float* src = a.Q + (a.rows ? a.rows[row] : row) * a.ldq;
   for (int c = lane; c < a.ld32; c += 64) a.Q32[row * a.ld32 + c] = c < a.d ? src[c] : 0.0f;
   const RowStats r = prep_row(src, a.d, a.Kp, a.bf16, dst);
-  if (lane == 0 && a.bf16) {  // 1 in the three norm columns
-    uint16_t* h = reinterpret_cast<uint16_t*>(dst) + a.d;
-    h[0] = 0x3F80;
-    h[1] = 0x3F80;
-    h[2] = 0x3F80;
-  }
   if (lane == 0) {  // raw f64 sums; the bound applies directed slack
     a.qn2[row] = r.s2;
     a.eq2[row] = r.e2;

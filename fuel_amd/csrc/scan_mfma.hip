@@ -1,5 +1,5 @@
 // Candidate generation on the matrix cores: key(q, x) = ||x||^2/2 - q.x, the
-// query-independent part of ||q - x||^2 / 2, with the inner products on MFMA
+// part of ||q - x||^2 / 2 that varies with x, with the inner products on MFMA
 // (bf16 x bf16 -> f32, or f32 x f32 -> f32), tiled through LDS, and a fused
 // per-query selection that keeps the KP smallest keys of every DB split.  The
 // distance matrix is never written.  Exactness is recovered afterwards by the
@@ -11,17 +11,29 @@
 //   B operand = 32 queries, held in registers for the whole scan;
 //   C = 32x32 f32: lane l holds query (l & 31) against 16 DB rows
 //       8*(r/4) + 4*(l/32) + (r%4), r = 0..15.
+//   The C input of the first MFMA of a tile is hxn = -fl32(||x||^2/2) of those
+//   rows (staged beside the tile), so the result is acc = q.x - ||x||^2/2 = -key
+//   with no extra pass over the accumulators.
 // Every lane-half h = l/32 supplies, per K step, the 16 bytes at byte offset
 // 32*s + 16*h of its row (8 bf16, or 4 f32 feeding 4 chained 32x32x2 MFMAs);
 // A and B use the same assignment, so the sums run over every coordinate once.
 //
 // Workgroup: 4 waves, each owning A*32 queries; the tile of R*32 DB rows is
-// shared through LDS by the 4 waves.  blockIdx.x = split * nqb + query block.
+// shared through LDS by the 4 waves and double-buffered: the global->LDS copy
+// of tile t+2 is issued as soon as tile t's buffer is free.
 //
-// Selection: lanes compare key < thr(query) (strict; an excluded row has
-// key >= the KP-th kept key); survivors are appended to the query's slot in
-// HBM, one segment per lane-half, and merged by a wave-cooperative bitonic
-// sort before a segment can overflow.
+// Selection: a row block is skipped when no lane's maximum beats its query's
+// threshold (strict: an excluded row has key >= the KP-th kept key).  Otherwise
+// the 4-bit row position is written into the low mantissa bits (a perturbation
+// of at most 15 ulps, covered by the rerank's error bound), so one max tree
+// yields the best value and its row; survivors are appended to the query's slot
+// in HBM, one segment per lane-half, peeling one per lane per step.  Before a
+// segment can overflow, the query's kept list and both segments are merged by a
+// radix select (no sort): a 32-step ballot/popcount search finds the KP-th key,
+// the keys below it (and enough equal ones) become the new kept list, and the
+// KP-th key is the new threshold.  Thresholds are shared by the splits of a
+// query through a per-query atomicMin; every tile's copy brings the values the
+// other splits published.
 #include "common.h"
 #include "kernels.h"
 
@@ -51,30 +63,6 @@ struct Mfma<1> {  // f32: four chained 32x32x2 MFMAs per 16-byte operand
   }
 };
 
-// Merge every query of group g whose lanes flag `need` (wave-uniform loop; the
-// merge of query j resets the counters held by lanes j and j + 32).
-#define MERGE_WHERE(g, need)                                                              \
-  do {                                                                                    \
-    uint64_t mask_ = __ballot(need);                                                      \
-    while (mask_) {                                                                       \
-      const int j_ = __builtin_ctzll(mask_) & 31;                                         \
-      const int c0_ = __builtin_amdgcn_readlane(cnt[g], j_);                              \
-      const int c1_ = __builtin_amdgcn_readlane(cnt[g], j_ + 32);                         \
-      const int lc_ = __builtin_amdgcn_readlane(lcnt[g], j_);                             \
-      uint64_t* sl_ = a.slots + (size_t(split) * a.nq_pad + q0 + (g) * 32 + j_) * NS;     \
-      uint64_t kth_;                                                                      \
-      const int nl_ = merge_slot2<uint64_t, V>(sl_, KP, lc_, csub, c0_, c1_, kth_);       \
-      const float nthr_ = nl_ == KP ? C::key(kth_) : INFINITY;                            \
-      if (nl_ == KP && lane == j_) atomicMin(&a.gthr[q0 + (g) * 32 + j_], (uint32_t)(kth_ >> 32)); \
-      if (col == j_) {                                                                    \
-        cnt[g] = 0;                                                                       \
-        lcnt[g] = nl_;                                                                    \
-        thr[g] = nthr_;                                                                   \
-      }                                                                                   \
-      mask_ &= ~((1ull << j_) | (1ull << (j_ + 32)));                                     \
-    }                                                                                     \
-  } while (0)
-
 // Asynchronous global -> LDS copy of 16 (or 4) bytes per lane into
 // lds_base + 16*lane (4*lane).  Issued from inline assembly so that the compiler
 // does not serialise later LDS reads of *other* buffers behind it; completion is
@@ -103,12 +91,95 @@ __device__ __forceinline__ void lds_dma4(uint32_t lds_base, const void* src) {
       : "memory");
 }
 
+__device__ __forceinline__ int lane_prefix(uint64_t m) {
+  return (int)__builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0));
+}
+
+// Merge of one query's slot (one wave): kept list slot[0, lcnt) and the two
+// append segments slot[KP, KP + c0), slot[KP + csub, KP + csub + c1).  When the
+// total reaches KP, t = the KP-th smallest key (high words, order-preserving
+// bits) is found by a 32-step search for the smallest t with #(key <= t) >= KP;
+// slot[0, KP) then receives every key < t and as many keys == t as fit.  The
+// list is left unsorted.  Returns the new kept count; t_out = t (all-ones when
+// fewer than KP keys exist).
+template <int V>
+__device__ __forceinline__ int merge_select(uint64_t* __restrict__ slot, int KP, int lcnt, int csub, int c0, int c1,
+                                            uint32_t& t_out) {
+  const int lane = __lane_id();
+  uint64_t x[V];
+  uint64_t vm[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    const int e = v * 64 + lane;
+    const bool valid = (e < lcnt) || (e >= KP && e < KP + c0) || (e >= KP + csub && e < KP + csub + c1);
+    x[v] = valid ? slot[e] : ~0ull;
+    vm[v] = __ballot(valid);
+  }
+  const int total = lcnt + c0 + c1;
+  uint32_t t = 0xFFFFFFFFu;
+  if (total >= KP) {
+    uint32_t p = 0;
+    for (int b = 31; b >= 0; --b) {
+      const uint32_t probe = p | ((1u << b) - 1u);
+      int c = 0;
+#pragma unroll
+      for (int v = 0; v < V; ++v) c += __popcll(__ballot(uint32_t(x[v] >> 32) <= probe) & vm[v]);
+      if (c < KP) p |= (1u << b);
+    }
+    t = p;
+  }
+  // all loads above have been consumed; the stores below may overwrite slot
+  int base = 0;
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    const bool f = ((vm[v] >> lane) & 1) && (total < KP || uint32_t(x[v] >> 32) < t);
+    const uint64_t m = __ballot(f);
+    if (f) slot[base + lane_prefix(m)] = x[v];
+    base += __popcll(m);
+  }
+  if (total >= KP) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const bool f = ((vm[v] >> lane) & 1) && uint32_t(x[v] >> 32) == t;
+      const uint64_t m = __ballot(f);
+      const int pos = base + lane_prefix(m);
+      if (f && pos < KP) slot[pos] = x[v];
+      base += __popcll(m);
+    }
+  }
+  t_out = t;
+  return total < KP ? total : KP;
+}
+
+// Merge every query of group g whose lanes flag `need` (wave-uniform loop; the
+// merge of query j resets the counters held by lanes j and j + 32).  The slot
+// stores and loads of one wave are ordered by the hardware (wavefront scope),
+// so the appends of this tile are visible to the merge's loads.
+#define MERGE_WHERE(g, need)                                                              \
+  do {                                                                                    \
+    uint64_t mask_ = __ballot(need);                                                      \
+    while (mask_) {                                                                       \
+      const int j_ = __builtin_ctzll(mask_) & 31;                                         \
+      const int c0_ = __builtin_amdgcn_readlane(cnt[g], j_);                              \
+      const int c1_ = __builtin_amdgcn_readlane(cnt[g], j_ + 32);                         \
+      const int lc_ = __builtin_amdgcn_readlane(lcnt[g], j_);                             \
+      uint64_t* sl_ = a.slots + (size_t(split) * a.nq_pad + q0 + (g) * 32 + j_) * NS;     \
+      uint32_t kt_;                                                                       \
+      const int nl_ = merge_select<V>(sl_, KP, lc_, csub, c0_, c1_, kt_);                 \
+      const float nthr_ = nl_ == KP ? unord_f32(kt_) : MFMA_KEY_LIMIT;                    \
+      if (nl_ == KP && lane == j_) atomicMin(&a.gthr[q0 + (g) * 32 + j_], kt_);           \
+      if (col == j_) {                                                                    \
+        cnt[g] = 0;                                                                       \
+        lcnt[g] = nl_;                                                                    \
+        thr[g] = nthr_;                                                                   \
+      }                                                                                   \
+      mask_ &= ~((1ull << j_) | (1ull << (j_ + 32)));                                     \
+    }                                                                                     \
+  } while (0)
+
 template <int KIND, int KS, int A, int R, int V>
 __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a) {
   using C = Comp<float>;
-  // bf16 rows carry -||x||^2/2 split into three bf16 columns (queries carry 1s
-  // there), so the MFMA result is already acc = q.x - ||x||^2/2 = -key.
-  constexpr bool FOLD = KIND == 0;
   constexpr int NS = 64 * V;
   constexpr int QPW = 32 * A;
   constexpr int QPB = 4 * QPW;
@@ -119,16 +190,17 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a) {
   constexpr int TILE_CHUNKS = BN * ROWC;
   constexpr int WAVE_LOADS = (TILE_CHUNKS + 255) / 256;  // 64-chunk loads per wave per tile
   constexpr int TILE_BYTES = WAVE_LOADS * 256 * 16;       // rounded up to whole wave loads
-  constexpr int NBUF = (2 * TILE_BYTES <= 65536) ? 2 : 1;
-  __shared__ __attribute__((aligned(16))) char tile[NBUF][TILE_BYTES];
-  // 64 floats per buffer: the hx DMA writes one dword per lane of a wave
-  __shared__ __attribute__((aligned(16))) float tile_hx[FOLD ? 1 : NBUF][FOLD ? 4 : 64];
+  __shared__ __attribute__((aligned(16))) char tile[2][TILE_BYTES];
+  __shared__ __attribute__((aligned(16))) float tile_hx[2][64];      // one dword per lane of wave 0
+  __shared__ __attribute__((aligned(16))) uint32_t thr_lds[2][4][64];  // published thresholds, per wave
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int half = lane >> 5;
   const int col = lane & 31;
   const int wave = tid >> 6;
+  // Split-major order: all query blocks of split 0 start first, so that the
+  // later splits of a query begin with the thresholds the earlier ones reached.
   const int qb = (int)(blockIdx.x % (unsigned)a.nqb);
   const int split = (int)(blockIdx.x / (unsigned)a.nqb);
   const int64_t q0 = int64_t(qb) * QPB + wave * QPW;
@@ -152,7 +224,7 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a) {
   int cnt[A], lcnt[A];
 #pragma unroll
   for (int g = 0; g < A; ++g) {
-    thr[g] = INFINITY;
+    thr[g] = MFMA_KEY_LIMIT;
     cnt[g] = 0;
     lcnt[g] = 0;
   }
@@ -174,30 +246,45 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a) {
   }
   const uint32_t lds_tile = (uint32_t)(uintptr_t)&tile[0][0];
   const uint32_t lds_hx = (uint32_t)(uintptr_t)&tile_hx[0][0];
+  const uint32_t lds_thr = (uint32_t)(uintptr_t)&thr_lds[0][0][0];
+  const uint32_t* gthr_src = a.gthr + q0 + (lane < QPW ? lane : QPW - 1);
   auto stage = [&](int b, int64_t t) {
     const char* base = xop + size_t(t) * kp_bytes;
 #pragma unroll
     for (int j = 0; j < WAVE_LOADS; ++j)
       lds_dma16(__builtin_amdgcn_readfirstlane(lds_tile + b * TILE_BYTES + (j * 4 + wave) * 1024), base + soff[j]);
-    if (!FOLD && wave == 0) {
+    if (wave == 0) {
       const int r = lane < BN ? lane : BN - 1;
-      lds_dma4(__builtin_amdgcn_readfirstlane(lds_hx + (NBUF == 2 ? b : 0) * 64 * 4), a.hx + t + r);
+      lds_dma4(__builtin_amdgcn_readfirstlane(lds_hx + b * 64 * 4), a.hx + t + r);
     }
+    // the thresholds other splits published for this wave's queries ride along
+    lds_dma4(__builtin_amdgcn_readfirstlane(lds_thr + (b * 4 + wave) * 256), gthr_src);
   };
 
-  int64_t it = 0;
   if (r0 < r1) stage(0, r0);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  if (NBUF == 2 && r0 + BN < r1) stage(1, r0 + BN);
+  if (r0 + BN < r1) stage(1, r0 + BN);
   int buf = 0;
-  for (int64_t t = r0; t < r1; t += BN, ++it) {
+  for (int64_t t = r0; t < r1; t += BN) {
+    const char* tb = tile[buf];
+    const float* hb = tile_hx[buf];
+    // C input: hxn of the 16 rows this lane holds, shared by the query groups
     floatx16 acc[A][R];
 #pragma unroll
-    for (int g = 0; g < A; ++g)
+    for (int rb = 0; rb < R; ++rb) {
+      floatx16 h;
 #pragma unroll
-      for (int rb = 0; rb < R; ++rb) acc[g][rb] = floatx16{};
-    const char* tb = tile[NBUF == 2 ? buf : 0];
+      for (int q4 = 0; q4 < 4; ++q4) {
+        const float4 h4 = *reinterpret_cast<const float4*>(hb + rb * 32 + q4 * 8 + half * 4);
+        h[q4 * 4 + 0] = h4.x;
+        h[q4 * 4 + 1] = h4.y;
+        h[q4 * 4 + 2] = h4.z;
+        h[q4 * 4 + 3] = h4.w;
+      }
+#pragma unroll
+      for (int g = 0; g < A; ++g) acc[g][rb] = h;
+    }
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
       uint4 af[R];
@@ -210,27 +297,6 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a) {
         for (int rb = 0; rb < R; ++rb) Mfma<KIND>::step(acc[g][rb], af[rb], qf[g][s]);
     }
 
-    if constexpr (!FOLD) {
-      // acc = q.x - hx = -key.  Register r of a 32x32 tile holds row
-      // 8*(r/4) + 4*half + (r%4).
-#pragma unroll
-      for (int rb = 0; rb < R; ++rb)
-#pragma unroll
-        for (int q4 = 0; q4 < 4; ++q4) {
-          const float4 h4 =
-              *reinterpret_cast<const float4*>(&tile_hx[FOLD ? 0 : (NBUF == 2 ? buf : 0)][rb * 32 + q4 * 8 + half * 4]);
-          const float hv[4] = {h4.x, h4.y, h4.z, h4.w};
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int g = 0; g < A; ++g) acc[g][rb][q4 * 4 + u] = acc[g][rb][q4 * 4 + u] - hv[u];
-        }
-    }
-    // Selection: one max-reduction per (query group, row block); only blocks
-    // with a survivor in some lane take the append path.  The threshold is the
-    // smaller of this split's KP-th key and the one other splits of the query
-    // published: any split's KP-th key bounds the KP-th key of the union of
-    // all splits' lists, so rows above it can never be needed.
 #pragma unroll
     for (int g = 0; g < A; ++g) {
       const float nt = -thr[g];
@@ -243,44 +309,42 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a) {
         if (__ballot(mx > nt)) {
           appended = true;
           uint64_t* sl = a.slots + (size_t(split) * a.nq_pad + q0 + g * 32 + col) * NS + KP + half * csub;
+          float w[16];
 #pragma unroll
-          for (int r = 0; r < 16; ++r) {
-            if (acc[g][rb][r] > nt) {
+          for (int r = 0; r < 16; ++r)
+            w[r] = __uint_as_float((__float_as_uint(acc[g][rb][r]) & ~15u) | uint32_t(r));
+          float mt = w[0];
+#pragma unroll
+          for (int r = 1; r < 16; ++r) mt = fmaxf(mt, w[r]);
+          // at most 16 rounds: each removes one survivor of every active lane
+          while (__ballot(mt > nt)) {
+            if (mt > nt) {
+              const uint32_t r = __float_as_uint(mt) & 15u;
               const uint32_t row = (uint32_t)(t + rb * 32 + (r >> 2) * 8 + half * 4 + (r & 3));
-              sl[cnt[g]] = C::make(-acc[g][rb][r], row);
+              sl[cnt[g]] = C::make(-mt, row);
               ++cnt[g];
             }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) w[r] = w[r] < mt ? w[r] : -INFINITY;
+            mt = w[0];
+#pragma unroll
+            for (int r = 1; r < 16; ++r) mt = fmaxf(mt, w[r]);
           }
         }
       }
+      // a merge leaves room for a whole tile of appends (16 R per lane)
       if (appended) MERGE_WHERE(g, cnt[g] > csub - 16 * R);
     }
-    // Every 8 tiles, adopt the thresholds the query's other splits published
-    // (read here, where no tile DMA is in flight, so the load does not hold up
-    // the prefetch).
-    auto refresh = [&]() {
-      if ((it & 7) == 7) {
+    // The next tile (and the thresholds that came with it) have landed for
+    // every wave; this tile's buffer is free again and takes the tile after next.
+    __builtin_amdgcn_s_waitcnt(0);
+    if (t + BN < r1) {
 #pragma unroll
-        for (int g = 0; g < A; ++g)
-          thr[g] = fminf(thr[g], unord_f32(__hip_atomic_load(&a.gthr[q0 + g * 32 + col], __ATOMIC_RELAXED,
-                                                             __HIP_MEMORY_SCOPE_AGENT)));
-      }
-    };
-    if constexpr (NBUF == 2) {
-      // the next tile's DMA (issued one iteration ago) has landed for every
-      // wave; this tile's buffer is free again and takes the tile after next
-      __builtin_amdgcn_s_waitcnt(0);
-      refresh();
-      __syncthreads();
-      if (t + 2 * BN < r1) stage(buf, t + 2 * BN);
-      buf ^= 1;
-    } else {
-      refresh();
-      __syncthreads();
-      if (t + BN < r1) stage(0, t + BN);
-      __builtin_amdgcn_s_waitcnt(0);
-      __syncthreads();
+      for (int g = 0; g < A; ++g) thr[g] = fminf(thr[g], unord_f32(thr_lds[buf ^ 1][wave][g * 32 + col]));
     }
+    __syncthreads();
+    if (t + 2 * BN < r1) stage(buf, t + 2 * BN);
+    buf ^= 1;
   }
 #pragma unroll
   for (int g = 0; g < A; ++g) MERGE_WHERE(g, cnt[g] > 0);
@@ -302,17 +366,16 @@ __global__ __launch_bounds__(256) void scan_mfma_kernel_wide(ScanMfmaArgs a) {
   scan_mfma_body<KIND, KS, A, R, V>(a);
 }
 
-// K step buckets: bf16 covers 16 columns per step (d coordinates + the three
-// norm columns), f32 covers 8.  Up to 16 steps a wave holds 2 query groups and
-// the tile has 64 rows; beyond, 1 query group and 32-row tiles keep the query
-// fragments at <= 224 registers and the two tile buffers within 64 KB of LDS.
-static const int kBf16Steps[] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 14, 16, 17, 20, 24, 32, 40, 50, 56};
+// K step buckets: bf16 covers 16 coordinates per step, f32 covers 8.  Up to 16
+// steps a wave holds 2 query groups and the tile has 64 rows; beyond, 1 query
+// group and 32-row tiles keep the query fragments at <= 256 registers and the
+// two tile buffers within one CU's LDS.
+static const int kBf16Steps[] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 56, 64};
 static const int kF32Steps[] = {1, 2, 3, 4, 6, 8, 12, 16, 20, 24, 32};
 
 int mfma_ksteps_bucket(int kind, int d) {
   const int per = kind == 0 ? 16 : 8;
-  const int k = kind == 0 ? d + 3 : d;  // bf16 rows carry three extra columns
-  int need = (k + per - 1) / per;
+  int need = (d + per - 1) / per;
   if (need < 1) need = 1;
   const int* tab = kind == 0 ? kBf16Steps : kF32Steps;
   const int nt = kind == 0 ? (int)(sizeof(kBf16Steps) / sizeof(int)) : (int)(sizeof(kF32Steps) / sizeof(int));
@@ -353,8 +416,8 @@ hipError_t launch_scan_mfma(const ScanMfmaArgs& a, int kind, int ks, int V, int 
   case K:          \
     return launch_ks<0, K>(a, V, nsplit, st);
       KS_CASE(1) KS_CASE(2) KS_CASE(3) KS_CASE(4) KS_CASE(5) KS_CASE(6) KS_CASE(7) KS_CASE(8) KS_CASE(9)
-      KS_CASE(10) KS_CASE(12) KS_CASE(14) KS_CASE(16) KS_CASE(17) KS_CASE(20) KS_CASE(24) KS_CASE(32)
-      KS_CASE(40) KS_CASE(50) KS_CASE(56)
+      KS_CASE(10) KS_CASE(12) KS_CASE(14) KS_CASE(16) KS_CASE(20) KS_CASE(24) KS_CASE(28) KS_CASE(32)
+      KS_CASE(40) KS_CASE(48) KS_CASE(56) KS_CASE(64)
 #undef KS_CASE
       default:
         return hipErrorInvalidValue;

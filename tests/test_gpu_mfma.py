@@ -111,3 +111,27 @@ def test_result_independent_of_batch_composition(gpu):
     for sl in (slice(0, 1), slice(13, 14), slice(100, 357), slice(600, 700)):
         si, sd = ix.knn(Q[sl], 10, 0)
         assert np.array_equal(si, gi[sl]) and _same(sd, gd[sl])
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("n", [5000, 6000, 6001, 6047])
+def test_neighbours_in_partial_last_tile(gpu, oracle, mode, n):
+    # the last tile mixes real rows with padding rows; queries sitting on the
+    # last rows must find them (a padding row's sentinel must not hide its
+    # neighbours in the same accumulator block)
+    rng = np.random.default_rng(n)
+    X = rng.random((n, 16), dtype=np.float32)
+    Q = X[-60:] + np.float32(1e-3) * rng.standard_normal((60, 16)).astype(np.float32)
+    _run(gpu, oracle, X, Q, 10, "euclidean", 0, mode)
+    _run(gpu, oracle, X, X[-60:], 5, "euclidean", 1, mode)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_huge_norms_take_the_exact_scan(gpu, oracle, mode):
+    # squared norms beyond the scan's sentinel range: every query is answered
+    # by the exact scan, still bit-identical
+    rng = np.random.default_rng(11)
+    X = (rng.random((3000, 8)) * 2e18).astype(np.float32)
+    Q = (rng.random((50, 8)) * 2e18).astype(np.float32)
+    st = _run(gpu, oracle, X, Q, 7, "euclidean", 0, mode)
+    assert st["exact_queries"] == 50, st
