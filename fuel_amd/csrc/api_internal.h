@@ -61,8 +61,8 @@ struct fuel_amd_index {
   } mop[2];
   int64_t n_pad = 0;
   struct QueryWs {
-    fa::DevBuf Qop, Q32, qnorm, qint, slots, lcnts, fail_count, fail_list, gthr;
-  } qws[2];
+    fa::DevBuf Qop, Q32, qnorm, qint, slots, lcnts, fail_count, fail_list, fail_theta, gthr;
+  } qws[3];  // one per level of the certification chain
   fuel_amd_stats stats{};
   // profiling: event pairs around the dominant kernel of each call
   bool timing = false;
@@ -113,9 +113,13 @@ int32_t outlier_run(fuel_amd_index* ix, int k, int knn_outlier, void* out_scores
 // Mixed-precision kNN (f32 indexes): kind 0 bf16 MFMA, 1 f32 MFMA.  Returns
 // FUEL_AMD_ERR_UNSUPPORTED when the shape has no MFMA variant.
 // With d_rows, slot i is query row d_rows[i] of Q and writes output row d_rows[i].
+// kp = candidates kept per split (0: from the index's widening factor); level =
+// position in the certification chain (which also selects the query
+// workspace); d_theta = optional per-slot starting thresholds (key units), the
+// certified second pass of queries that failed a previous level.
 int32_t knn_mfma(fuel_amd_index* ix, const float* Q, int64_t m, int64_t ldq, int k, int exclude, int kind,
                  int64_t* out_idx, float* out_dist, hipStream_t st, const int64_t* d_rows = nullptr,
-                 int64_t qbase = 0, int level = 0);
+                 int64_t qbase = 0, int level = 0, int kp = 0, const float* d_theta = nullptr);
 bool mfma_supported(const fuel_amd_index* ix, int kind, int k, int exclude);
 // Exact path on an explicit list of query rows (the certification fallback).
 int32_t knn_exact_rows(fuel_amd_index* ix, const float* Q, int64_t ldq, const int64_t* d_rows, int64_t nrows,

@@ -1,8 +1,14 @@
 // Host orchestration of the mixed-precision kNN path:
 //   prep (once per index and kind) -> per query batch: prep queries -> MFMA
 //   candidate scan with fused selection -> exact rerank + certification ->
-//   the queries whose bound did not certify go one level down the chain
-//   bf16 MFMA -> f32 MFMA -> exact restated scan, which always answers.
+//   the queries whose bound did not certify go one level down the chain:
+//   level 0  the requested kind (bf16 or f32), KP from the widening factor;
+//   level 1  the same kind, started at the certified threshold theta the
+//            rerank computed from the query's current KK-th exact key (every
+//            row that can still enter the answer lies below it, so the scan
+//            keeps only those and its answer certifies with theta);
+//   level 2  f32 MFMA (after bf16), again from theta;
+//   then the exact restated scan, which always answers.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -103,7 +109,7 @@ bool mfma_supported(const fuel_amd_index* ix, int kind, int k, int exclude) {
 
 int32_t knn_mfma(fuel_amd_index* ix, const float* Q, int64_t m, int64_t ldq, int k, int exclude, int kind,
                  int64_t* out_idx, float* out_dist, hipStream_t st, const int64_t* d_rows, int64_t qbase,
-                 int level) {
+                 int level, int kp, const float* d_theta) {
   int32_t rc = ensure_prepared(ix, kind, st);
   if (rc) return rc;
   auto& op = ix->mop[kind];
@@ -114,7 +120,7 @@ int32_t knn_mfma(fuel_amd_index* ix, const float* Q, int64_t m, int64_t ldq, int
   }
   auto& ws = ix->qws[level];
   const int KK = k + (exclude != EXCL_NONE ? 1 : 0);
-  const int KP = kp_for(ix, k, exclude);
+  const int KP = kp > 0 ? std::max(kp, k + (exclude != EXCL_NONE ? 1 : 0)) : kp_for(ix, k, exclude);
   const int V = KP <= 64 ? 4 : 8;
   const int NS = 64 * V;
   const int csub = (NS - KP) / 2;
@@ -153,6 +159,7 @@ int32_t knn_mfma(fuel_amd_index* ix, const float* Q, int64_t m, int64_t ldq, int
     if (e == hipSuccess) e = ws.lcnts.ensure(sizeof(int) * size_t(nsplit) * nq_pad);
     if (e == hipSuccess) e = ws.fail_count.ensure(64);
     if (e == hipSuccess) e = ws.fail_list.ensure(sizeof(int64_t) * size_t(chunk));
+    if (e == hipSuccess) e = ws.fail_theta.ensure(sizeof(float) * size_t(chunk));
     if (e == hipSuccess) e = ws.gthr.ensure(sizeof(uint32_t) * size_t(nq_pad));
     if (e != hipSuccess) return hip_fail(e, "MFMA workspaces");
 
@@ -177,6 +184,7 @@ int32_t knn_mfma(fuel_amd_index* ix, const float* Q, int64_t m, int64_t ldq, int
     qa.qint = ws.qint.as<int>();
     qa.gthr = ws.gthr.as<uint32_t>();
     qa.rows = rows;
+    qa.theta = d_theta ? d_theta + done : nullptr;
     e = launch_prep_queries(qa, st);
     if (e == hipSuccess) e = hipMemsetAsync(ws.fail_count.p, 0, sizeof(int), st);
     if (e != hipSuccess) return hip_fail(e, "query preparation");
@@ -230,6 +238,8 @@ int32_t knn_mfma(fuel_amd_index* ix, const float* Q, int64_t m, int64_t ldq, int
     ra.out_dist = d_rows ? out_dist : out_dist + done * k;
     ra.fail_count = ws.fail_count.as<int>();
     ra.fail_list = ws.fail_list.as<int64_t>();
+    ra.gthr = sa.gthr;
+    ra.fail_theta = ws.fail_theta.as<float>();
     e = launch_rerank(ra, V, st);
     if (e != hipSuccess) return hip_fail(e, "rerank launch");
 
@@ -239,9 +249,16 @@ int32_t knn_mfma(fuel_amd_index* ix, const float* Q, int64_t m, int64_t ldq, int
     if (e != hipSuccess) return hip_fail(e, "certification count");
     if (nfail > 0) {
       // the fail list holds query rows relative to Qb and to this chunk's outputs
-      if (kind == 0 && level == 0 && mfma_supported(ix, 1, k, exclude)) {
-        rc = knn_mfma(ix, Qb, nfail, ldq, k, exclude, 1, ra.out_idx, ra.out_dist, st, ra.fail_list, ra.qbase, 1);
+      int next_kind = -1;
+      if (level == 0)
+        next_kind = kind;
+      else if (level == 1 && kind == 0 && mfma_supported(ix, 1, k, exclude))
+        next_kind = 1;
+      if (next_kind >= 0) {
+        rc = knn_mfma(ix, Qb, nfail, ldq, k, exclude, next_kind, ra.out_idx, ra.out_dist, st, ra.fail_list,
+                      ra.qbase, level + 1, KP_MAX, ra.fail_theta);
       } else {
+        ix->stats.exact_queries += nfail;  // queries answered by the exact scan
         rc = knn_exact_rows(ix, Qb, ldq, ra.fail_list, nfail, ra.qbase, k, exclude, ra.out_idx, ra.out_dist, st);
       }
       if (rc) return rc;
@@ -250,8 +267,6 @@ int32_t knn_mfma(fuel_amd_index* ix, const float* Q, int64_t m, int64_t ldq, int
       ix->stats.fallback_queries += nfail;
       ix->stats.splits = nsplit;
       ix->stats.kp = KP;
-    } else {
-      ix->stats.exact_queries += nfail;  // queries that needed the exact scan after the f32 level
     }
     done += chunk;
   }

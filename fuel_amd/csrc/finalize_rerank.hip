@@ -7,8 +7,10 @@
 //  2. Exact rerank: the restated fuel distance of every kept candidate, sorted
 //     by (distance, index) -> the top KK.
 //  3. Certification: a proven lower bound on the restated distance of every
-//     excluded row.  With s~ = fl(hx - g) the approximate key and E_s a bound on
-//     |s~ - (||x||^2/2 - q.x)|, every excluded x has
+//     excluded row.  T below is the KP-th kept key when this kernel cut the
+//     union; otherwise the scan's final shared threshold, below which every
+//     row a split dropped cannot lie.  With key~ the approximate key and E_s
+//     a bound on |key~ - (||x||^2/2 - q.x)|, every excluded x has
 //         D(x) = ||q - x||^2 >= ||q||^2 + 2 T - 2 E_s = L,
 //     and its restated f32 value r(x) >= L (1 - gamma_r) (or, when every
 //     coordinate is an integer of magnitude <= 256, r(x) >= min(L, 2^24 (1 -
@@ -16,7 +18,10 @@
 //     float B below that bound already orders after the KK-th exact key
 //     (sqrt(B) > key for 'euclidean', B > key for 'sqeuclidean'), no excluded
 //     row can enter the answer, ties included, and the top KK is fuel's exact
-//     answer.  Otherwise the query is queued for the exact fallback scan.
+//     answer.  Otherwise the query is queued for the next level of the chain,
+//     with the threshold theta for which the certificate would hold given its
+//     current KK-th exact key: a second scan that starts at theta keeps every
+//     row that can still enter the answer, and certifies with T = theta.
 //  E_s collects, for key~ = -(MFMA of x~ . q~ with C input -fl32(||x||^2/2)),
 //  low 4 bits replaced by the row tag: operand rounding |q~.x~ - q.x| <= |q|
 //  |x~-x| + |q~-q| |x| + |q~-q| |x~-x| (zero for the f32 operands); the
@@ -58,44 +63,95 @@ __device__ __forceinline__ float restated_aligned(const float* __restrict__ a, c
   return s;
 }
 
-__device__ __forceinline__ bool certify(const RerankArgs& a, int64_t qs, float T, float kk) {
+// Per-query terms of the bound (shared by the certificate and the threshold of
+// the certified second pass).
+struct BoundTerms {
+  double e_s;     // bound on |key~ - (||x||^2/2 - q.x)| over every row
+  double qn2_lo;  // lower bound of ||q||^2
+  double gr;      // relative error of the restated f32 distance
+  int d;
+  bool int_regime;
+  bool usable;    // norms inside the scan's sentinel range
+};
+
+__device__ __forceinline__ BoundTerms bound_terms(const RerankArgs& a, int64_t qs) {
   const double u = 0x1p-24;
+  BoundTerms b;
   const int d = a.d;
+  b.d = d;
   const double sl = (d + 8) * 0x1p-52;
   const double X2 = __longlong_as_double((long long)a.consts[0]);  // upper bounds already
   const double ex = sqrt(__longlong_as_double((long long)a.consts[1])) * (1 + 0x1p-50);
   const double xt = sqrt(__longlong_as_double((long long)a.consts[2])) * (1 + 0x1p-50);
   const double xn = sqrt(X2) * (1 + 0x1p-50);
   // beyond these norms keys may reach the scan's finite threshold sentinel
-  if (!(X2 < MFMA_NORM2_LIMIT) || !(a.qn2[qs] < MFMA_NORM2_LIMIT)) return false;
-  const double qn2_lo = a.qn2[qs] * (1 - sl);
+  b.usable = X2 < MFMA_NORM2_LIMIT && a.qn2[qs] < MFMA_NORM2_LIMIT;
+  b.qn2_lo = a.qn2[qs] * (1 - sl);
   const double qn = sqrt(a.qn2[qs] * (1 + sl)) * (1 + 0x1p-50);
   const double eq = sqrt(a.eq2[qs] * (1 + sl)) * (1 + 0x1p-50);
   const double qt = sqrt(a.qt2[qs] * (1 + sl)) * (1 + 0x1p-50);
-  const bool int_regime = a.flags[0] != 0 && a.qint[qs] != 0;
+  b.int_regime = a.flags[0] != 0 && a.qint[qs] != 0;
   const double dot_mag = qt * xt * (1 + 0x1p-50);
   const double e_hx = __longlong_as_double((long long)a.consts[3]);
   const double H = __longlong_as_double((long long)a.consts[4]);
-  const bool mfma_exact = int_regime && e_hx == 0.0 && dot_mag + H < 0x1p23;
+  const bool mfma_exact = b.int_regime && e_hx == 0.0 && dot_mag + H < 0x1p23;
   const double gacc = mfma_exact ? 0.0 : 4.0 * (d + 2) * u;
   const double e_round = qn * ex + eq * xn + eq * ex;
   const double e_tag = 0x1p-19 * (dot_mag + H) * (1 + 0x1p-10) + 0x1p-145;
   double e_s = e_round + e_hx + gacc * (dot_mag + H) + e_tag + (mfma_exact ? 0.0 : 2.0 * (d + 2) * 0x1p-126);
-  e_s *= (1 + 0x1p-40);
-  double L = 2.0 * (double)T + qn2_lo;
-  L -= (fabs(2.0 * (double)T) + qn2_lo) * 0x1p-50;
-  L -= 2.0 * e_s;
+  b.e_s = e_s * (1 + 0x1p-40);
   const int n_r = d / 8 + 8 + (d % 8) + 2;
-  const double gr = n_r * u * (1 + 0x1p-10);
+  b.gr = n_r * u * (1 + 0x1p-10);
+  return b;
+}
+
+// Every excluded row has key~ >= T.  True iff no excluded row can order before
+// the KK-th exact key kk (ties included).
+__device__ __forceinline__ bool certify(const RerankArgs& a, const BoundTerms& b, float T, float kk) {
+  if (!b.usable) return false;
+  double L = 2.0 * (double)T + b.qn2_lo;
+  L -= (fabs(2.0 * (double)T) + b.qn2_lo) * 0x1p-50;
+  L -= 2.0 * b.e_s;
   double Lr;
-  if (int_regime)
-    Lr = fmin(L, 0x1p24 * (1 - gr));
+  if (b.int_regime)
+    Lr = fmin(L, 0x1p24 * (1 - b.gr));
   else
-    Lr = L * (1 - gr) - (2.0 * d + 16) * 0x1p-149;
+    Lr = L * (1 - b.gr) - (2.0 * b.d + 16) * 0x1p-149;
   if (!(Lr > 0.0)) return false;
   float B = (float)Lr;
   if ((double)B > Lr) B = nextafterf(B, -INFINITY);
   return a.metric == 0 ? (sqrt_rn(B) > kk) : (B > kk);
+}
+
+// The smallest float T (up to a small margin) for which certify(T, kk) holds:
+// a second scan that keeps every row with key~ < T finds every row that can
+// still order before kk, and its answer is then certified with T itself.
+// +inf when no such T exists (the integer regime's cap).
+__device__ __forceinline__ float certified_threshold(const RerankArgs& a, const BoundTerms& b, float kk) {
+  if (!b.usable || !(kk < INFINITY)) return INFINITY;
+  // target for the restated-distance bound B: strictly above kk (squared form)
+  double target;
+  if (a.metric == 0) {
+    const double kn = (double)nextafterf(kk, INFINITY);
+    target = kn * kn;
+  } else {
+    target = (double)kk;
+  }
+  target = target * (1 + 0x1p-20) + 0x1p-140;
+  double needL;
+  if (b.int_regime) {
+    if (!(target < 0x1p24 * (1 - b.gr))) return INFINITY;
+    needL = target;
+  } else {
+    needL = (target + (2.0 * b.d + 16) * 0x1p-149) / (1 - b.gr);
+  }
+  const double twoT = needL - b.qn2_lo + 2.0 * b.e_s;
+  double T = 0.5 * twoT;
+  T += fabs(T) * 0x1p-40 + (fabs(twoT) + b.qn2_lo) * 0x1p-48;
+  float Tf = (float)T;
+  if ((double)Tf < T) Tf = nextafterf(Tf, INFINITY);
+  for (int i = 0; i < 8 && !certify(a, b, Tf, kk); ++i) Tf = nextafterf(Tf, INFINITY);
+  return certify(a, b, Tf, kk) ? Tf : INFINITY;
 }
 
 template <int V>
@@ -140,7 +196,10 @@ __global__ __launch_bounds__(64) void rerank_kernel(RerankArgs a) {
   }
   wave_bitonic_sort<T, V>(x);
   const int ncand = total < KP ? (int)total : KP;
-  const float T_C = total >= KP ? C::key(wave_get<T, V>(x, KP - 1)) : INFINITY;
+  // Bound on the excluded rows: with a cut here, the KP-th kept key; otherwise
+  // the scan's final shared threshold (every row a split dropped lies at or
+  // above it).  At or above KEY_LIMIT no real row was excluded.
+  const float T_C = total > KP ? C::key(wave_get<T, V>(x, KP - 1)) : unord_f32(a.gthr[qs]);
 
   // 2. exact rerank
   const float* qrow = a.Q32 + qs * a.ld32;
@@ -160,17 +219,24 @@ __global__ __launch_bounds__(64) void rerank_kernel(RerankArgs a) {
 
   // 3. certification
   bool cert = true;
-  if (total >= KP) {
-    const float kk = C::key(wave_get<T, V>(y, a.KK - 1));
-    cert = certify(a, qs, T_C, kk);
+  float kk = INFINITY;
+  BoundTerms bt;
+  if (T_C < MFMA_KEY_LIMIT) {
+    kk = ncand >= a.KK ? C::key(wave_get<T, V>(y, a.KK - 1)) : INFINITY;
+    bt = bound_terms(a, qs);
+    cert = certify(a, bt, T_C, kk);
   }
   const int64_t orow = a.rows ? a.rows[qs] : qs;
   if (cert) {
     const int cnt = ncand < a.KK ? ncand : a.KK;
     write_knn_row<float, V>(y, cnt, a.qbase + orow, a.k, a.mode, a.out_idx + orow * a.k, a.out_dist + orow * a.k);
-  } else if (lane == 0) {
-    const int pos = atomicAdd(a.fail_count, 1);
-    a.fail_list[pos] = orow;
+  } else {
+    const float theta = a.fail_theta ? certified_threshold(a, bt, kk) : INFINITY;
+    if (lane == 0) {
+      const int pos = atomicAdd(a.fail_count, 1);
+      a.fail_list[pos] = orow;
+      if (a.fail_theta) a.fail_theta[pos] = theta;
+    }
   }
 }
 

@@ -188,8 +188,9 @@ struct PrepQArgs {
   double* eq2;  // ||q~ - q||^2
   double* qt2;  // ||q~||^2
   int* qint;    // 1 if integer-valued with |v| <= 256
-  uint32_t* gthr;  // m_pad shared thresholds, initialised to +inf here
+  uint32_t* gthr;  // m_pad shared thresholds (order-preserving bits), initialised here
   const int64_t* rows;  // optional: slot i reads query row rows[i] of Q
+  const float* theta;   // optional: slot i's initial threshold (key units); +inf without
 };
 
 // Finite sentinels of the MFMA scan.  Thresholds start at KEY_LIMIT instead of
@@ -248,6 +249,9 @@ struct RerankArgs {
   float* out_dist;
   int* fail_count;
   int64_t* fail_list;  // receives the query rows (not slots) that did not certify
+  const uint32_t* gthr;  // final shared thresholds of the scan, per slot
+  float* fail_theta;     // optional: beside fail_list, a key threshold below which
+                         // every row that can still enter the query's answer lies
 };
 
 hipError_t launch_prep_db(const PrepDbArgs& a, hipStream_t st);

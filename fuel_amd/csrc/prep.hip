@@ -108,7 +108,8 @@ __global__ __launch_bounds__(256) void prep_query_kernel(PrepQArgs a) {
   const int lane = __lane_id();
   const size_t es = a.bf16 ? 2 : 4;
   char* dst = reinterpret_cast<char*>(a.Qop) + size_t(row) * a.Kp * es;
-  if (lane == 0 && a.gthr) a.gthr[row] = 0xFF800000u;  // order-preserving bits of +inf
+  if (lane == 0 && a.gthr)  // +inf, or the caller's threshold for this slot
+    a.gthr[row] = (a.theta && row < a.m) ? ord_f32(a.theta[row]) : ord_f32(INFINITY);
   if (row >= a.m) {
     for (int c = lane; c < a.Kp; c += 64) {
       if (a.bf16)

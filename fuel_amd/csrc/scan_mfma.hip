@@ -224,7 +224,8 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a) {
   int cnt[A], lcnt[A];
 #pragma unroll
   for (int g = 0; g < A; ++g) {
-    thr[g] = MFMA_KEY_LIMIT;
+    // a caller may start a query at a proven threshold (the certified second pass)
+    thr[g] = fminf(MFMA_KEY_LIMIT, unord_f32(a.gthr[q0 + g * 32 + col]));
     cnt[g] = 0;
     lcnt[g] = 0;
   }
