@@ -1,11 +1,15 @@
-"""Benchmark: exact kNN queries/s on one MI355X.
+"""Benchmark: exact kNN queries/s on one MI355X, mixed (bf16 MFMA + exact rerank
++ certificate) beside the plain FP32 MFMA form.
 
 Workload (default): database 1,000,000 x 128 f32 uniform [0,1), 100,000 queries
 from the same distribution, k = 10, Euclidean.  One step = one kNN pass of all
 queries against the resident database (inputs already in HBM).  Prints one JSON
-line with the throughput, the dominant kernel's roofline fraction (timed live
-with HIP events on its launch stream) and the CPU baseline (the oracle's
-restatement of fuel's AVX2 linear scan on the host cores, on a bounded sample).
+line with the throughput of --mode, the same for the FP32 form and their ratio,
+the dominant kernel's roofline fraction (timed live with HIP events on its
+launch stream; HBM traffic from the committed rocprofv3 summary) and the CPU
+baseline (the oracle's restatement of fuel's AVX2 linear scan on the host
+cores, on a bounded sample).  Every mode returns fuel's exact answer; a sample
+of queries is checked against the oracle bit for bit.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode bf16|fp32|exact]
 """
@@ -41,9 +45,36 @@ def parse():
     p.add_argument("--widen", type=int, default=4, help="candidates kept per query = widen * k (<= 128)")
     p.add_argument("--verify", type=int, default=64, help="queries checked against the oracle")
     p.add_argument("--cpu-sample", type=int, default=1000, help="queries timed on the CPU baseline (0: skip)")
+    p.add_argument("--fp32-steps", type=int, default=2,
+                   help="timed steps of the plain FP32 MFMA form run beside the mixed one (0: skip)")
     p.add_argument("--copy-probe", type=int, default=0,
                    help="only run a plain device copy of this many bytes 3 times (calibrates HBM counters)")
     return p.parse_args()
+
+
+def profiled_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload`, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary committed under profiles/ (calibrated by
+    the copy probe below), or None when no summary covers this pair."""
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return None
+    for name in sorted(os.listdir(pdir)):
+        if not name.endswith((".md", ".txt")):
+            continue
+        fields = {}
+        with open(os.path.join(pdir, name)) as f:
+            for line in f:
+                line = line.strip()
+                if line.startswith("- ") and ": " in line:
+                    key, val = line[2:].split(": ", 1)
+                    fields.setdefault(key.strip(), val.strip())
+        if fields.get("kernel") == kernel and fields.get("workload") == workload:
+            try:
+                return int(fields["traffic_bytes_per_launch"])
+            except (KeyError, ValueError):
+                return None
+    return None
 
 
 def copy_probe(nbytes):
@@ -56,6 +87,54 @@ def copy_probe(nbytes):
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes": nbytes}), flush=True)
+
+
+METRIC = "kNN queries/s, 1M\u00d7128 k=10: mixed vs FP32 at exact recall; p99 serve ms"
+
+
+def run_mode(_lib, Xd, Qd, n, m, d, k, mode, widen, steps, warmup, stream, out_i, out_d):
+    """Warm up, then time `steps` kNN passes of all queries on a resident index.
+    Returns wall time per step, the dominant scan kernel's HIP-event time and the
+    index statistics of the last pass."""
+    import torch
+    ix = _lib.Index.from_device(Xd.data_ptr(), n, d, d, np.float32, "euclidean", mode=mode, widen=widen)
+
+    def step():
+        ix.knn_device(Qd.data_ptr(), m, d, k, _lib.EXCLUDE_NONE, out_i.data_ptr(), out_d.data_ptr(), stream)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ix.timing_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kernel_ms, launches, kname = ix.timing_read()
+    ix.timing_enable(False)
+    res = {"ms_per_step": (t1 - t0) / steps * 1e3, "qps": m * steps / (t1 - t0),
+           "kernel_s": (kernel_ms / max(launches, 1)) / 1e3, "kernel": kname, "stats": ix.stats()}
+    ix.close()
+    return res
+
+
+def verify(oracle, X, Q, gi, gd, k, nverify):
+    """Sample of queries checked against the oracle: identical indices and f32
+    bits, recall, and the largest relative error against an f64 oracle."""
+    m = Q.shape[0]
+    rng = np.random.default_rng(0)
+    sample = np.sort(rng.choice(m, size=min(nverify, m), replace=False))
+    oi, od = oracle.knn(X, Q[sample], k)
+    exact_rows = int(np.sum(np.all(gi[sample] == oi, axis=1) & np.all(gd[sample] == od, axis=1)))
+    recall = float(np.mean([len(set(gi[s]) & set(oi[j])) / k for j, s in enumerate(sample)]))
+    qi = np.repeat(sample, k)
+    xi = gi[sample].ravel()
+    ref64 = np.sqrt(oracle.sqdist_fp64(X, Q, qi, xi))
+    max_rel_err = float(np.max(np.abs(gd[sample].ravel().astype(np.float64) - ref64) / np.maximum(ref64, 1e-300)))
+    return {"queries": int(len(sample)), "exact_rows": exact_rows, "recall": recall,
+            "max_rel_err_vs_fp64": max_rel_err}
 
 
 def main():
@@ -80,56 +159,39 @@ def main():
     dev = torch.device("cuda:0")
     Xd = torch.from_numpy(X).to(dev)
     Qd = torch.from_numpy(Q).to(dev)
-    m, d, k = Q.shape[0], Q.shape[1], args.k
+    n, m, d, k = X.shape[0], Q.shape[0], Q.shape[1], args.k
     out_i = torch.empty((m, k), dtype=torch.int64, device=dev)
     out_d = torch.empty((m, k), dtype=torch.float32, device=dev)
     torch.cuda.synchronize()
-    ix = _lib.Index.from_device(Xd.data_ptr(), X.shape[0], d, d, np.float32, "euclidean", mode=args.mode,
-                                widen=args.widen)
     stream = torch.cuda.current_stream().cuda_stream
 
-    def step():
-        ix.knn_device(Qd.data_ptr(), m, d, k, _lib.EXCLUDE_NONE, out_i.data_ptr(), out_d.data_ptr(), stream)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    print("# warmup done", file=sys.stderr, flush=True)
-    ix.timing_enable(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    kernel_ms, launches, kname = ix.timing_read()
-    ix.timing_enable(False)
-    stats = ix.stats()
-    elapsed = t1 - t0
-    ms_per_step = elapsed / args.steps * 1e3
-    qps = m * args.steps / elapsed
-    print(f"# timed {args.steps} steps: {ms_per_step:.2f} ms/step, {qps:.0f} q/s", file=sys.stderr, flush=True)
-
-    # verification on a sample of queries against the oracle (exact indices and f32 bits)
     import oracle
-    gi = out_i.cpu().numpy()
-    gd = out_d.cpu().numpy()
-    rng = np.random.default_rng(0)
-    sample = np.sort(rng.choice(m, size=min(args.verify, m), replace=False))
-    oi, od = oracle.knn(X, Q[sample], k)
-    exact_rows = int(np.sum(np.all(gi[sample] == oi, axis=1) & np.all(gd[sample] == od, axis=1)))
-    recall = float(np.mean([len(set(gi[s]) & set(oi[j])) / k for j, s in enumerate(sample)]))
-    qi = np.repeat(sample, k)
-    xi = gi[sample].ravel()
-    ref64 = np.sqrt(oracle.sqdist_fp64(X, Q, qi, xi))
-    max_rel_err = float(np.max(np.abs(gd[sample].ravel().astype(np.float64) - ref64) / np.maximum(ref64, 1e-300)))
+    main_run = run_mode(_lib, Xd, Qd, n, m, d, k, args.mode, args.widen, args.steps, args.warmup, stream, out_i,
+                        out_d)
+    print(f"# {args.mode}: {main_run['ms_per_step']:.2f} ms/step, {main_run['qps']:.0f} q/s", file=sys.stderr,
+          flush=True)
+    checks = verify(oracle, X, Q, out_i.cpu().numpy(), out_d.cpu().numpy(), k, args.verify)
 
-    flops = 2.0 * m * X.shape[0] * d
+    # the plain FP32 MFMA form beside it (the metric is mixed vs FP32 at exact recall)
+    fp32 = None
+    if args.mode != "fp32" and args.fp32_steps > 0:
+        r = run_mode(_lib, Xd, Qd, n, m, d, k, "fp32", args.widen, args.fp32_steps, 1, stream, out_i, out_d)
+        print(f"# fp32: {r['ms_per_step']:.2f} ms/step, {r['qps']:.0f} q/s", file=sys.stderr, flush=True)
+        c32 = verify(oracle, X, Q, out_i.cpu().numpy(), out_d.cpu().numpy(), k, args.verify)
+        fp32 = {"value": round(r["qps"], 1), "ms_per_step": round(r["ms_per_step"], 3), "steps": args.fp32_steps,
+                "kernel": r["kernel"], "kernel_ms": round(r["kernel_s"] * 1e3, 3),
+                "roofline_frac": round(2.0 * m * n * d / r["kernel_s"] / PEAKS["fp32"][0], 4),
+                "exact_rows": c32["exact_rows"], "recall": c32["recall"],
+                "mixed_over_fp32": round(main_run["qps"] / r["qps"], 3)}
+
+    flops = 2.0 * m * n * d
     peak, unit, bound = PEAKS[args.mode]
-    avg_kernel_s = (kernel_ms / max(launches, 1)) / 1e3
+    avg_kernel_s = main_run["kernel_s"]
     achieved = flops / avg_kernel_s if avg_kernel_s > 0 else 0.0
+    kname = main_run["kernel"]
+    workload = f"kNN db {n}x{d} f32 uniform[0,1), {m} queries, k={k}, euclidean"
     roofline = {"bound": bound, "achieved": round(achieved / 1e12, 3), "peak": round(peak / 1e12, 1), "unit": unit,
-                "frac": round(achieved / peak, 4), "traffic": None, "kernel": kname,
+                "frac": round(achieved / peak, 4), "traffic": profiled_traffic(kname, workload), "kernel": kname,
                 "kernel_ms": round(avg_kernel_s * 1e3, 3), "algorithmic_flop_per_launch": flops}
 
     cpu = None
@@ -141,27 +203,28 @@ def main():
         oracle.knn(X, Q[:S], k, threads=threads)
         c1 = time.perf_counter()
         cpu = {"value": round(S / (c1 - c0), 2), "unit": "queries/s", "cores": threads, "kind": "port",
-               "sample": f"{S} of the {m} queries against the full {X.shape[0]}x{d} database, k={k}, "
+               "sample": f"{S} of the {m} queries against the full {n}x{d} database, k={k}, "
                          f"{c1 - c0:.1f}s; restated AVX2 kernel + KNNHeap, one thread per query batch"}
 
+    stats = main_run["stats"]
     line = {
-        "metric": "kNN queries/s, 1M x 128 f32, k=10, exact (recall 1.0 vs fuel's linear scan)",
-        "value": round(qps, 1),
+        "metric": METRIC,
+        "value": round(main_run["qps"], 1),
         "unit": "queries/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": round(main_run["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"bf16": "bf16", "fp32": "f32", "exact": "f32"}[args.mode],
         "data": "synthetic",
-        "config": {"workload": f"kNN db {X.shape[0]}x{d} f32 uniform[0,1), {m} queries, k={k}, euclidean",
-                   "mode": args.mode, "splits": stats["splits"], "kp": stats["kp"],
+        "config": {"workload": workload,
+                   "mode": args.mode, "widen": args.widen, "splits": stats["splits"], "kp": stats["kp"],
                    "fallback_queries": stats["fallback_queries"], "exact_rescans": stats["exact_queries"]},
-        "verify": {"queries": int(len(sample)), "exact_rows": exact_rows, "recall": recall,
-                   "max_rel_err_vs_fp64": max_rel_err},
+        "verify": checks,
+        "fp32": fp32,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
