@@ -47,6 +47,8 @@ def parse():
     p.add_argument("--cpu-sample", type=int, default=1000, help="queries timed on the CPU baseline (0: skip)")
     p.add_argument("--fp32-steps", type=int, default=2,
                    help="timed steps of the plain FP32 MFMA form run beside the mixed one (0: skip)")
+    p.add_argument("--host-path", action="store_true",
+                   help="only time the host-pointer entry point (queries and results cross PCIe each call)")
     p.add_argument("--copy-probe", type=int, default=0,
                    help="only run a plain device copy of this many bytes 3 times (calibrates HBM counters)")
     return p.parse_args()
@@ -166,6 +168,22 @@ def main():
     stream = torch.cuda.current_stream().cuda_stream
 
     import oracle
+    if args.host_path:
+        # fuel_amd_knn with host arrays: H2D of the queries, kNN, D2H of the results
+        ix = _lib.Index.from_device(Xd.data_ptr(), n, d, d, np.float32, "euclidean", mode=args.mode,
+                                    widen=args.widen)
+        for _ in range(args.warmup):
+            hi, hd = ix.knn(Q, k, _lib.EXCLUDE_NONE)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            hi, hd = ix.knn(Q, k, _lib.EXCLUDE_NONE)
+        t1 = time.perf_counter()
+        checks = verify(oracle, X, Q, hi, hd, k, args.verify)
+        print(json.dumps({"host_path_queries_per_s": round(m * args.steps / (t1 - t0), 1),
+                          "ms_per_step": round((t1 - t0) / args.steps * 1e3, 3), "mode": args.mode,
+                          "verify": checks}), flush=True)
+        ix.close()
+        return
     main_run = run_mode(_lib, Xd, Qd, n, m, d, k, args.mode, args.widen, args.steps, args.warmup, stream, out_i,
                         out_d)
     print(f"# {args.mode}: {main_run['ms_per_step']:.2f} ms/step, {main_run['qps']:.0f} q/s", file=sys.stderr,

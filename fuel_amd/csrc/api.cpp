@@ -83,6 +83,19 @@ int device_cu_count(int dev) {
 
 bool cancelled() { return g_cancel.load(std::memory_order_relaxed); }
 
+hipError_t upload_rows(void* dst, int64_t ld, const void* host, int64_t rows, int d, size_t es, DevBuf& stage,
+                       hipStream_t st) {
+  const size_t row_bytes = size_t(d) * es;
+  if (rows <= 0 || d <= 0) return hipSuccess;
+  if (ld == d) return hipMemcpyAsync(dst, host, size_t(rows) * row_bytes, hipMemcpyHostToDevice, st);
+  hipError_t e = stage.ensure(size_t(rows) * row_bytes);
+  if (e == hipSuccess) e = hipMemcpyAsync(stage.p, host, size_t(rows) * row_bytes, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpy2DAsync(dst, size_t(ld) * es, stage.p, row_bytes, row_bytes, (size_t)rows, hipMemcpyDeviceToDevice,
+                         st);
+  return e;
+}
+
 // Choose the DB split count so that (query blocks x splits) fills the GPU.
 int choose_splits(int64_t qblocks, int64_t n, int cus, int64_t min_rows) {
   int64_t target = int64_t(cus) * 8;
@@ -165,9 +178,16 @@ static int32_t index_create_common(const void* data, bool on_device, int64_t n, 
   }
   e = ix->X.ensure(size_t(std::max<int64_t>(n, 1)) * ix->ld * es);
   if (e == hipSuccess) e = hipMemsetAsync(ix->X.p, 0, size_t(std::max<int64_t>(n, 1)) * ix->ld * es, ix->stream);
-  if (e == hipSuccess && n > 0 && d > 0)
-    e = hipMemcpy2DAsync(ix->X.p, ix->ld * es, data, (on_device ? ld : d) * es, size_t(d) * es, (size_t)n,
-                         on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ix->stream);
+  if (e == hipSuccess && n > 0 && d > 0) {
+    if (on_device) {
+      e = hipMemcpy2DAsync(ix->X.p, ix->ld * es, data, ld * es, size_t(d) * es, (size_t)n, hipMemcpyDeviceToDevice,
+                           ix->stream);
+    } else {
+      fa::DevBuf stage;
+      e = fa::upload_rows(ix->X.p, ix->ld, data, n, d, es, stage, ix->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(ix->stream);  // before `stage` is released
+    }
+  }
   if (e == hipSuccess) e = hipStreamSynchronize(ix->stream);
   if (e != hipSuccess) {
     int32_t r = hip_fail(e, "index upload");
@@ -284,8 +304,7 @@ int32_t fuel_amd_knn(fuel_amd_index* ix, const void* query, int64_t m, int32_t k
   if (e == hipSuccess) e = ix->hout_dist.ensure(size_t(m) * k * es);
   if (e != hipSuccess) return hip_fail(e, "query buffers");
   if (ix->d > 0) {
-    e = hipMemcpy2DAsync(ix->hq.p, ix->ld * es, query, size_t(ix->d) * es, size_t(ix->d) * es, (size_t)m,
-                         hipMemcpyHostToDevice, st);
+    e = fa::upload_rows(ix->hq.p, ix->ld, query, m, ix->d, es, ix->upload, st);
     if (e != hipSuccess) return hip_fail(e, "query upload");
   }
   rc = knn_run(ix, ix->hq.p, m, ix->ld, k, exclude_self, (int64_t*)ix->hout_idx.p, ix->hout_dist.p, st);

@@ -33,6 +33,12 @@ int32_t check_device(int dev);
 int device_cu_count(int dev);
 bool cancelled();
 int choose_splits(int64_t qblocks, int64_t n, int cus, int64_t min_rows);
+// Host rows of d elements (es bytes each) -> device rows of pitch ld elements:
+// one contiguous host-to-device copy (through `stage` when the pitches differ),
+// then the padding is applied by a device-side 2D copy.  A 2D copy straight from
+// pageable host memory moves one row at a time.
+hipError_t upload_rows(void* dst, int64_t ld, const void* host, int64_t rows, int d, size_t es, DevBuf& stage,
+                       hipStream_t st);
 
 }  // namespace fa
 
@@ -47,6 +53,7 @@ struct fuel_amd_index {
   hipStream_t stream = nullptr;
   std::mutex mu;
   fa::DevBuf X;                          // resident rows (n x ld)
+  fa::DevBuf upload;                     // staging of host query rows whose pitch differs from ld
   fa::DevBuf hq, hout_idx, hout_dist;    // staging for host-pointer calls
   fa::DevBuf slots, lcnts, fail_list;    // scan workspaces
   fa::DevBuf ws_a, ws_b, ws_c, ws_d, ws_e, ws_tmp;  // range / pairwise / outlier workspaces

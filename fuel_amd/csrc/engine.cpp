@@ -111,7 +111,7 @@ int32_t range_impl(fuel_amd_index* ix, const void* query, int64_t m, double radi
   if (e == hipSuccess) e = ix->ws_c.ensure(sizeof(int64_t) * (m + 1));    // per-query offsets
   if (e != hipSuccess) return hip_fail(e, "range workspace");
   if (ix->d > 0) {
-    e = hipMemcpy2DAsync(ix->hq.p, ix->ld * es, query, ix->d * es, ix->d * es, (size_t)m, hipMemcpyHostToDevice, st);
+    e = upload_rows(ix->hq.p, ix->ld, query, m, ix->d, es, ix->upload, st);
     if (e != hipSuccess) return hip_fail(e, "query upload");
   }
   RangeArgs<F> ra{};
