@@ -39,7 +39,7 @@ EXPORTED = [
     "fuel_amd_version", "fuel_amd_device_count", "fuel_amd_last_error",
     "fuel_amd_request_cancel", "fuel_amd_clear_cancel",
     "fuel_amd_index_create", "fuel_amd_index_create_device", "fuel_amd_index_destroy",
-    "fuel_amd_knn", "fuel_amd_knn_device", "fuel_amd_range", "fuel_amd_csr_free",
+    "fuel_amd_knn", "fuel_amd_knn_device", "fuel_amd_range", "fuel_amd_knn_ties", "fuel_amd_csr_free",
     "fuel_amd_pairwise", "fuel_amd_lof", "fuel_amd_knn_outlier", "fuel_amd_last_stats",
     "fuel_amd_timing_enable", "fuel_amd_timing_read",
 ]
@@ -107,6 +107,8 @@ def load():
     lib.fuel_amd_knn_device.restype = I32
     lib.fuel_amd_range.argtypes = [P, P, I64, ctypes.c_double, I32, ctypes.POINTER(Csr)]
     lib.fuel_amd_range.restype = I32
+    lib.fuel_amd_knn_ties.argtypes = [P, P, I64, I32, ctypes.POINTER(Csr)]
+    lib.fuel_amd_knn_ties.restype = I32
     lib.fuel_amd_csr_free.argtypes = [ctypes.POINTER(Csr)]
     lib.fuel_amd_pairwise.argtypes = [P, I64, I32, I32, I32, I32, P]
     lib.fuel_amd_pairwise.restype = I32
@@ -223,14 +225,27 @@ class Index:
                                          ctypes.c_void_p(stream)))
 
     def range(self, query, radius, exclude_self=False):
-        lib = load()
+        """All points within `radius` (inclusive) of each query: CSR (offsets, idx, dist)."""
         query = np.ascontiguousarray(query, dtype=self.dtype)
         if query.ndim != 2 or query.shape[1] != self.d:
             raise ValueError("query shape mismatch")
-        m = query.shape[0]
         c = Csr()
-        check(lib.fuel_amd_range(self.handle, query.ctypes.data, m, float(radius), int(bool(exclude_self)),
-                                 ctypes.byref(c)))
+        check(load().fuel_amd_range(self.handle, query.ctypes.data, query.shape[0], float(radius),
+                                    int(bool(exclude_self)), ctypes.byref(c)))
+        return self._take_csr(c, query.shape[0])
+
+    def knn_ties(self, query, k):
+        """kNN with fuel's tie semantics (every point at or below the k-th distance):
+        CSR (offsets, idx, dist) sorted by (distance, index)."""
+        query = np.ascontiguousarray(query, dtype=self.dtype)
+        if query.ndim != 2 or query.shape[1] != self.d:
+            raise ValueError("query shape mismatch")
+        c = Csr()
+        check(load().fuel_amd_knn_ties(self.handle, query.ctypes.data, query.shape[0], int(k), ctypes.byref(c)))
+        return self._take_csr(c, query.shape[0])
+
+    def _take_csr(self, c, m):
+        lib = load()
         try:
             offs = np.ctypeslib.as_array(c.offsets, shape=(m + 1,)).copy()
             nnz = int(c.nnz)

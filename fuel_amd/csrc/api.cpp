@@ -343,6 +343,18 @@ int32_t fuel_amd_range(fuel_amd_index* ix, const void* query, int64_t m, double 
   return range_run(ix, query, m, radius, exclude_self, out);
 }
 
+int32_t fuel_amd_knn_ties(fuel_amd_index* ix, const void* query, int64_t m, int32_t k, fuel_amd_csr* out) {
+  g_last_error.clear();
+  if (!ix || !out) return fail(FUEL_AMD_ERR_INVALID, "NULL argument");
+  std::memset(out, 0, sizeof(*out));
+  if (m < 0 || k < 0) return fail(FUEL_AMD_ERR_INVALID, "m and k must be non-negative");
+  if (m > 0 && !query) return fail(FUEL_AMD_ERR_INVALID, "query must not be NULL");
+  std::lock_guard<std::mutex> lock(ix->mu);
+  int32_t rc = check_device(ix->device);
+  if (rc) return rc;
+  return knn_ties_run(ix, query, m, k, out);
+}
+
 // ---- pairwise ---------------------------------------------------------------
 
 int32_t fuel_amd_pairwise(const void* data, int64_t n, int32_t d, int32_t dtype, int32_t metric, int32_t form,

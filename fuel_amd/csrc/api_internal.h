@@ -115,6 +115,7 @@ int32_t knn_run(fuel_amd_index* ix, const void* d_query, int64_t m, int64_t ldq,
                 int64_t* d_out_idx, void* d_out_dist, hipStream_t st);
 int32_t range_run(fuel_amd_index* ix, const void* query, int64_t m, double radius, int exclude_self,
                   fuel_amd_csr* out);
+int32_t knn_ties_run(fuel_amd_index* ix, const void* query, int64_t m, int k, fuel_amd_csr* out);
 int32_t pairwise_run(fuel_amd_index* ix, int form, void* out);
 int32_t outlier_run(fuel_amd_index* ix, int k, int knn_outlier, void* out_scores);
 // Mixed-precision kNN (f32 indexes): kind 0 bf16 MFMA, 1 f32 MFMA.  Returns

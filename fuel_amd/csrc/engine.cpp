@@ -85,9 +85,11 @@ int32_t knn_exact(fuel_amd_index* ix, const F* Q, int64_t m, int64_t ldq, int k,
   return FUEL_AMD_OK;
 }
 
+// Range search of m query rows already in device memory (pitch ix->ld), with a
+// common radius or per-query radii (d_radii[i * radii_stride]), into a host CSR.
 template <typename F>
-int32_t range_impl(fuel_amd_index* ix, const void* query, int64_t m, double radius, int exclude_self,
-                   fuel_amd_csr* out) {
+int32_t range_core(fuel_amd_index* ix, const F* d_q, int64_t m, double radius, const F* d_radii,
+                   int64_t radii_stride, int exclude_self, fuel_amd_csr* out) {
   using K = typename OrdKey<F>::T;
   hipStream_t st = ix->stream;
   const size_t es = sizeof(F);
@@ -105,26 +107,23 @@ int32_t range_impl(fuel_amd_index* ix, const void* query, int64_t m, double radi
   rps = std::max<int64_t>(256, (rps + 255) & ~int64_t(255));
   nsplit = (int)std::max<int64_t>(1, (ix->n + rps - 1) / rps);
   const size_t ncs = size_t(m) * nsplit;
-  hipError_t e = ix->hq.ensure(size_t(m) * ix->ld * es);
-  if (e == hipSuccess) e = ix->ws_a.ensure(sizeof(int64_t) * (ncs + 1));  // counts
+  hipError_t e = ix->ws_a.ensure(sizeof(int64_t) * (ncs + 1));           // counts
   if (e == hipSuccess) e = ix->ws_b.ensure(sizeof(int64_t) * (ncs + 1));  // offsets
   if (e == hipSuccess) e = ix->ws_c.ensure(sizeof(int64_t) * (m + 1));    // per-query offsets
   if (e != hipSuccess) return hip_fail(e, "range workspace");
-  if (ix->d > 0) {
-    e = upload_rows(ix->hq.p, ix->ld, query, m, ix->d, es, ix->upload, st);
-    if (e != hipSuccess) return hip_fail(e, "query upload");
-  }
   RangeArgs<F> ra{};
   ra.X = ix->X.as<F>();
   ra.ldx = ix->ld;
   ra.n = ix->n;
   ra.d = ix->d;
-  ra.Q = ix->hq.as<F>();
+  ra.Q = d_q;
   ra.ldq = ix->ld;
   ra.nq = m;
   ra.qbase = 0;
   ra.metric = ix->metric;
   ra.radius = (F)radius;  // python/search.rs:785 (`radius as f32`)
+  ra.radii = d_radii;
+  ra.radii_stride = radii_stride;
   ra.exclude_self = exclude_self ? 1 : 0;
   ra.nsplit = nsplit;
   ra.rows_per_split = rps;
@@ -183,6 +182,49 @@ int32_t range_impl(fuel_amd_index* ix, const void* query, int64_t m, double radi
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return hip_fail(e, "range download");
   return FUEL_AMD_OK;
+}
+
+template <typename F>
+int32_t range_impl(fuel_amd_index* ix, const void* query, int64_t m, double radius, int exclude_self,
+                   fuel_amd_csr* out) {
+  hipError_t e = ix->hq.ensure(size_t(std::max<int64_t>(m, 1)) * ix->ld * sizeof(F));
+  if (e != hipSuccess) return hip_fail(e, "query buffer");
+  if (m > 0 && ix->d > 0) {
+    e = upload_rows(ix->hq.p, ix->ld, query, m, ix->d, sizeof(F), ix->upload, ix->stream);
+    if (e != hipSuccess) return hip_fail(e, "query upload");
+  }
+  return range_core<F>(ix, ix->hq.as<F>(), m, radius, nullptr, 0, exclude_self, out);
+}
+
+// KNNHeap's result (src/api/search.rs:80-151): every point whose key is <= the
+// k-th key, sorted by (key, index) -- the exact k-th key of each query from the
+// kNN path, then an inclusive range search at that radius.
+template <typename F>
+int32_t knn_ties_impl(fuel_amd_index* ix, const void* query, int64_t m, int k, fuel_amd_csr* out) {
+  const size_t es = sizeof(F);
+  hipStream_t st = ix->stream;
+  hipError_t e = ix->hq.ensure(size_t(std::max<int64_t>(m, 1)) * ix->ld * es);
+  if (e != hipSuccess) return hip_fail(e, "query buffer");
+  if (k == 0 || m == 0) {  // k = 0 -> empty lists (src/api/search.rs:106-108)
+    out->m = m;
+    out->nnz = 0;
+    out->offsets = (int64_t*)std::calloc(size_t(m + 1), sizeof(int64_t));
+    out->index = (int64_t*)std::malloc(8);
+    out->dist = std::malloc(es);
+    if (!out->offsets || !out->index || !out->dist) return fail(FUEL_AMD_ERR_NOMEM, "host allocation failed");
+    return FUEL_AMD_OK;
+  }
+  if (ix->d > 0) {
+    e = upload_rows(ix->hq.p, ix->ld, query, m, ix->d, es, ix->upload, st);
+    if (e != hipSuccess) return hip_fail(e, "query upload");
+  }
+  e = ix->hout_idx.ensure(size_t(m) * k * 8);
+  if (e == hipSuccess) e = ix->hout_dist.ensure(size_t(m) * k * es);
+  if (e != hipSuccess) return hip_fail(e, "kNN buffers");
+  int32_t rc = knn_run(ix, ix->hq.p, m, ix->ld, k, EXCL_NONE, ix->hout_idx.as<int64_t>(), ix->hout_dist.p, st);
+  if (rc) return rc;
+  // the k-th key of every query (+inf when fewer than k points exist: then all of them)
+  return range_core<F>(ix, ix->hq.as<F>(), m, 0.0, ix->hout_dist.as<F>() + (k - 1), k, 0, out);
 }
 
 template <typename F>
@@ -280,6 +322,13 @@ int32_t range_run(fuel_amd_index* ix, const void* query, int64_t m, double radiu
                   fuel_amd_csr* out) {
   int32_t rc = ix->dtype == FUEL_AMD_F32 ? range_impl<float>(ix, query, m, radius, exclude_self, out)
                                          : range_impl<double>(ix, query, m, radius, exclude_self, out);
+  if (rc) fuel_amd_csr_free(out);
+  return rc;
+}
+
+int32_t knn_ties_run(fuel_amd_index* ix, const void* query, int64_t m, int k, fuel_amd_csr* out) {
+  int32_t rc = ix->dtype == FUEL_AMD_F32 ? knn_ties_impl<float>(ix, query, m, k, out)
+                                         : knn_ties_impl<double>(ix, query, m, k, out);
   if (rc) fuel_amd_csr_free(out);
   return rc;
 }

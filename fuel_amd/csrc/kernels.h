@@ -62,6 +62,8 @@ struct RangeArgs {
   int64_t qbase;  // data row of query slot 0 (self-exclusion)
   int metric;
   F radius;
+  const F* radii;        // optional per-query radius (query slot i: radii[i * radii_stride])
+  int64_t radii_stride;
   int exclude_self;
   int nsplit;
   int64_t rows_per_split;

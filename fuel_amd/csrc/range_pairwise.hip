@@ -2,7 +2,8 @@
 // pairwise distance matrices, both on the restated fuel distance.
 //
 // Range: fuel's linear_scan_range keeps every point with distance <= radius and
-// sorts by (distance, index) (src/api/search.rs:154-166).  For the Euclidean
+// sorts by (distance, index) (src/api/search.rs:154-166).  A per-query radius
+// (the k-th distance of each query) gives KNNHeap's all-ties kNN result.  For the Euclidean
 // metric the test sqrt(s) <= r is done as s <= S(r), S(r) the largest float whose
 // correctly rounded sqrt is <= r, which is the same predicate.
 // Pairwise: out[i][j] = restated(p_i, p_j); the restated distance is symmetric
@@ -35,7 +36,12 @@ __global__ __launch_bounds__(256) void range_kernel(RangeArgs<F> a) {
   const bool single = d <= DC;
   if (single) stage_queries<F, QW, DC>(qbuf, qrow, nqb, 0, d);
   __syncthreads();
-  const F rs = a.metric == 0 ? sq_upper(a.radius) : a.radius;
+  F rad[QW], rs[QW];
+#pragma unroll
+  for (int qq = 0; qq < QW; ++qq) {
+    rad[qq] = (a.radii && qq < nqb) ? a.radii[(qs0 + qq) * a.radii_stride] : a.radius;
+    rs[qq] = a.metric == 0 ? sq_upper(rad[qq]) : rad[qq];
+  }
   for (int64_t rb = r0; rb < r1; rb += 256) {
     const int64_t row = rb + tid;
     const bool valid = row < r1;
@@ -44,8 +50,8 @@ __global__ __launch_bounds__(256) void range_kernel(RangeArgs<F> a) {
     uint64_t hits[QW];
 #pragma unroll
     for (int qq = 0; qq < QW; ++qq) {
-      bool hit = qq < nqb && valid && s[qq] <= rs && !(a.exclude_self && row == a.qbase + qs0 + qq);
-      if (hit && a.metric == 0) hit = sqrt_rn(s[qq]) <= a.radius;  // exact predicate
+      bool hit = qq < nqb && valid && s[qq] <= rs[qq] && !(a.exclude_self && row == a.qbase + qs0 + qq);
+      if (hit && a.metric == 0) hit = sqrt_rn(s[qq]) <= rad[qq];  // exact predicate
       hits[qq] = __ballot(hit);
       if (__lane_id() == 0) wcnt[qq][wave] = __popcll(hits[qq]);
     }

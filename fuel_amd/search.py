@@ -123,6 +123,12 @@ class SearchIndex:
     The rows are copied to HBM once; ``knn`` and ``radius_search`` then stream
     only the queries.  ``mode`` selects how candidates are generated ('auto',
     'exact', 'fp32', 'bf16'); every mode returns the exact fuel answer.
+
+    ``precompute=max_k`` fills fuel's precomputed kNN table (every point's
+    neighbours with ties up to max_k) with one GPU call; self-queries with
+    k <= max_k are then served from it.  fuel's precomputed linear index answers
+    queries from another array with point 0's table row (its query index is
+    never set); this index answers them with a scan instead.
     """
 
     def __init__(self, data, *, distance='euclidean', tree=None, seed=None, precompute=None, mode='auto',
@@ -135,7 +141,9 @@ class SearchIndex:
         self.seed = seed
         self.precompute = precompute
         if precompute is not None:
-            raise NotImplementedError("precompute= is not supported by the GPU linear-scan engine")
+            precompute = int(precompute)
+            if precompute < 0:
+                raise OverflowError("can't convert negative int to unsigned")
         self.dtype = data.dtype
         self.shape = data.shape
         self.tree = _resolve_tree(tree)
@@ -143,6 +151,41 @@ class SearchIndex:
         self._source_strides = data.strides
         self._source_ptr = data.__array_interface__['data'][0]
         self.index = _lib.Index(_np.ascontiguousarray(data), distance, mode=mode, widen=widen, device=device)
+        # PrecomputedKnnSearcher::new (src/search/precomputed.rs:36-56): every data
+        # point's kNN with ties up to max_k = min(precompute, n), self included,
+        # filled by one GPU call.
+        self._table = None
+        self._max_k = 0
+        if precompute is not None:
+            n = data.shape[0]
+            self._max_k = min(precompute, n)
+            if n > 0 and self._max_k > 0:
+                self._table = self.index.knn_ties(_np.ascontiguousarray(data), self._max_k)
+
+    def _table_knn(self, m, k, exclude_self):
+        """SearchIndex.knn on the precomputed arm for self-queries
+        (python/search.rs:571-589 with batch_knn_with_query :1154-1173): search
+        k (+1 with exclude_self) through subset_with_ties (precomputed.rs:60-67),
+        drop the query's own index, pack to (m, k) with -1 / +inf."""
+        offs, idx, dist = self._table
+        query_k = k + 1 if exclude_self else k
+        out_i = _np.full((m, k), -1, _np.int64)
+        out_d = _np.full((m, k), _np.inf, self.dtype)
+        for i in range(m):
+            row_i = idx[offs[i]:offs[i + 1]]
+            row_d = dist[offs[i]:offs[i + 1]]
+            if query_k == 0 or row_i.size == 0:
+                continue
+            kth = row_d[min(query_k, row_i.size) - 1]
+            cut = int(_np.searchsorted(row_d, kth, side='right'))
+            ri, rd = row_i[:cut], row_d[:cut]
+            if exclude_self:
+                keep = ri != i
+                ri, rd = ri[keep], rd[keep]
+            c = min(k, ri.size)
+            out_i[i, :c] = ri[:c]
+            out_d[i, :c] = rd[:c]
+        return out_i, out_d
 
     def _same_source(self, query):
         return (query.shape == self._source_shape and query.strides == self._source_strides
@@ -173,7 +216,15 @@ class SearchIndex:
                 "exclude_self=True is only supported when query refers to the same underlying array as data")
         if k < 0:
             raise OverflowError("can't convert negative int to unsigned")
-        return self.index.knn(query, int(k), _lib.EXCLUDE_PY_LINEAR if exclude_self else _lib.EXCLUDE_NONE)
+        k = int(k)
+        if self.precompute is not None and self._same_source(query_in if isinstance(query_in, _np.ndarray) else query):
+            # precomputed arm: k (+1 with exclude_self) with ties, then drop self and
+            # truncate -- the search_knn + retain + pack of the reference; within
+            # max_k served from the table, beyond it by the scan (same result)
+            if (k + 1 if exclude_self else k) <= self._max_k and self._table is not None:
+                return self._table_knn(query.shape[0], k, exclude_self)
+            return self.index.knn(query, k, _lib.EXCLUDE_FOR_EACH_KNN if exclude_self else _lib.EXCLUDE_NONE)
+        return self.index.knn(query, k, _lib.EXCLUDE_PY_LINEAR if exclude_self else _lib.EXCLUDE_NONE)
 
     def radius_search(self, query, radius, *, exclude_self=None):
         """All neighbours within radius as (list of int64 arrays, list of float

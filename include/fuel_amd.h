@@ -147,6 +147,15 @@ int32_t fuel_amd_range(fuel_amd_index* index, const void* query, int64_t m, doub
                        int32_t exclude_self, fuel_amd_csr* out);
 void fuel_amd_csr_free(fuel_amd_csr* csr);
 
+/* kNN with fuel's tie semantics: for each query, every point whose key is <=
+ * the k-th smallest key (so possibly more than k), sorted by (distance,
+ * index), as CSR (library-allocated, fuel_amd_csr_free).  k = 0 gives empty
+ * lists; k >= n gives all points.  Replaces KNNHeap-based search_knn
+ * (src/api/search.rs:80-151, :169-184), the source of the table of
+ * PrecomputedKnnSearcher::new (src/search/precomputed.rs:36-56), which
+ * subset_with_ties (:60-67) then cuts per request. */
+int32_t fuel_amd_knn_ties(fuel_amd_index* index, const void* query, int64_t m, int32_t k, fuel_amd_csr* out);
+
 /* Pairwise distance matrix of n rows (square n x n, or condensed n(n-1)/2 in
  * fuel's internal lower-triangular order), caller-allocated host output.
  * Replaces _compute_pairwise_distances_{f32,f64} and the _condensed_ variants
