@@ -25,6 +25,7 @@ SOURCES = [
     "api.cpp",
     "engine.cpp",
     "engine_mfma.cpp",
+    "cluster.cpp",
     "scan_exact.hip",
     "scan_mfma.hip",
     "finalize_rerank.hip",

@@ -41,6 +41,7 @@ EXPORTED = [
     "fuel_amd_index_create", "fuel_amd_index_create_device", "fuel_amd_index_destroy",
     "fuel_amd_knn", "fuel_amd_knn_device", "fuel_amd_range", "fuel_amd_knn_ties", "fuel_amd_csr_free",
     "fuel_amd_pairwise", "fuel_amd_lof", "fuel_amd_knn_outlier", "fuel_amd_last_stats",
+    "fuel_amd_dbscan", "fuel_amd_optics",
     "fuel_amd_timing_enable", "fuel_amd_timing_read",
 ]
 
@@ -107,6 +108,10 @@ def load():
     lib.fuel_amd_knn_device.restype = I32
     lib.fuel_amd_range.argtypes = [P, P, I64, ctypes.c_double, I32, ctypes.POINTER(Csr)]
     lib.fuel_amd_range.restype = I32
+    lib.fuel_amd_dbscan.argtypes = [P, ctypes.c_double, I64, I32, P]
+    lib.fuel_amd_dbscan.restype = I32
+    lib.fuel_amd_optics.argtypes = [P, ctypes.c_double, I64, P, P, P, P, P]
+    lib.fuel_amd_optics.restype = I32
     lib.fuel_amd_knn_ties.argtypes = [P, P, I64, I32, ctypes.POINTER(Csr)]
     lib.fuel_amd_knn_ties.restype = I32
     lib.fuel_amd_csr_free.argtypes = [ctypes.POINTER(Csr)]
@@ -243,6 +248,24 @@ class Index:
         c = Csr()
         check(load().fuel_amd_knn_ties(self.handle, query.ctypes.data, query.shape[0], int(k), ctypes.byref(c)))
         return self._take_csr(c, query.shape[0])
+
+    def dbscan(self, eps, min_points, variant=0):
+        """DBSCAN labels of the index rows (variant 0 sequential, 1 parallel)."""
+        labels = np.empty(self.n, np.int64)
+        check(load().fuel_amd_dbscan(self.handle, float(eps), int(min_points), int(variant), labels.ctypes.data))
+        return labels
+
+    def optics(self, max_eps, min_points):
+        """(ordering, reachability, core_distance, predecessor, labels) of the index rows."""
+        n = self.n
+        ordering = np.empty(n, np.int64)
+        reach = np.empty(n, self.dtype)
+        core = np.empty(n, self.dtype)
+        pred = np.empty(n, np.int64)
+        labels = np.empty(n, np.int64)
+        check(load().fuel_amd_optics(self.handle, float(max_eps), int(min_points), ordering.ctypes.data,
+                                     reach.ctypes.data, core.ctypes.data, pred.ctypes.data, labels.ctypes.data))
+        return ordering, reach, core, pred, labels
 
     def _take_csr(self, c, m):
         lib = load()

@@ -355,6 +355,37 @@ int32_t fuel_amd_knn_ties(fuel_amd_index* ix, const void* query, int64_t m, int3
   return knn_ties_run(ix, query, m, k, out);
 }
 
+// ---- density clustering -------------------------------------------------------
+
+int32_t fuel_amd_dbscan(fuel_amd_index* ix, double eps, int64_t min_points, int32_t variant, int64_t* out_labels) {
+  g_last_error.clear();
+  if (!ix || (!out_labels && ix->n > 0)) return fail(FUEL_AMD_ERR_INVALID, "NULL argument");
+  if (!(eps >= 0.0)) return fail(FUEL_AMD_ERR_INVALID, "eps must be non-negative");
+  if (variant != 0 && variant != 1) return fail(FUEL_AMD_ERR_INVALID, "variant must be 0 (dbscan) or 1 (parallel)");
+  if (min_points < 0) return fail(FUEL_AMD_ERR_INVALID, "min_points must be non-negative");
+  if (variant == 1 && min_points == 0) return fail(FUEL_AMD_ERR_INVALID, "min_points must be greater than 0");
+  std::lock_guard<std::mutex> lock(ix->mu);
+  int32_t rc = check_device(ix->device);
+  if (rc) return rc;
+  return dbscan_run(ix, eps, min_points, variant, out_labels);
+}
+
+int32_t fuel_amd_optics(fuel_amd_index* ix, double max_eps, int64_t min_points, int64_t* out_ordering,
+                        void* out_reachability, void* out_core_distance, int64_t* out_predecessor,
+                        int64_t* out_labels) {
+  g_last_error.clear();
+  if (!ix) return fail(FUEL_AMD_ERR_INVALID, "NULL argument");
+  if (ix->n > 0 && (!out_ordering || !out_reachability || !out_core_distance || !out_predecessor || !out_labels))
+    return fail(FUEL_AMD_ERR_INVALID, "NULL output");
+  if (!(max_eps >= 0.0)) return fail(FUEL_AMD_ERR_INVALID, "eps must be non-negative");
+  if (min_points <= 0) return fail(FUEL_AMD_ERR_INVALID, "min_points must be greater than 0");
+  std::lock_guard<std::mutex> lock(ix->mu);
+  int32_t rc = check_device(ix->device);
+  if (rc) return rc;
+  return optics_run(ix, max_eps, min_points, out_ordering, out_reachability, out_core_distance, out_predecessor,
+                    out_labels);
+}
+
 // ---- pairwise ---------------------------------------------------------------
 
 int32_t fuel_amd_pairwise(const void* data, int64_t n, int32_t d, int32_t dtype, int32_t metric, int32_t form,

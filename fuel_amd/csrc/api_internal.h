@@ -116,6 +116,13 @@ int32_t knn_run(fuel_amd_index* ix, const void* d_query, int64_t m, int64_t ldq,
 int32_t range_run(fuel_amd_index* ix, const void* query, int64_t m, double radius, int exclude_self,
                   fuel_amd_csr* out);
 int32_t knn_ties_run(fuel_amd_index* ix, const void* query, int64_t m, int k, fuel_amd_csr* out);
+// Exact range of every indexed row against the index (self included), host CSR.
+int32_t range_self_run(fuel_amd_index* ix, double radius, fuel_amd_csr* out);
+// Density clustering over range_self_run (cluster.cpp).  variant 0: dbscan,
+// 1: parallel_dbscan.
+int32_t dbscan_run(fuel_amd_index* ix, double eps, int64_t min_points, int variant, int64_t* labels);
+int32_t optics_run(fuel_amd_index* ix, double max_eps, int64_t min_points, int64_t* ordering, void* reach,
+                   void* core, int64_t* pred, int64_t* labels);
 int32_t pairwise_run(fuel_amd_index* ix, int form, void* out);
 int32_t outlier_run(fuel_amd_index* ix, int k, int knn_outlier, void* out_scores);
 // Mixed-precision kNN (f32 indexes): kind 0 bf16 MFMA, 1 f32 MFMA.  Returns

@@ -326,6 +326,14 @@ int32_t range_run(fuel_amd_index* ix, const void* query, int64_t m, double radiu
   return rc;
 }
 
+int32_t range_self_run(fuel_amd_index* ix, double radius, fuel_amd_csr* out) {
+  int32_t rc = ix->dtype == FUEL_AMD_F32
+                   ? range_core<float>(ix, ix->X.as<float>(), ix->n, radius, nullptr, 0, 0, out)
+                   : range_core<double>(ix, ix->X.as<double>(), ix->n, radius, nullptr, 0, 0, out);
+  if (rc) fuel_amd_csr_free(out);
+  return rc;
+}
+
 int32_t knn_ties_run(fuel_amd_index* ix, const void* query, int64_t m, int k, fuel_amd_csr* out) {
   int32_t rc = ix->dtype == FUEL_AMD_F32 ? knn_ties_impl<float>(ix, query, m, k, out)
                                          : knn_ties_impl<double>(ix, query, m, k, out);

@@ -156,6 +156,23 @@ void fuel_amd_csr_free(fuel_amd_csr* csr);
  * subset_with_ties (:60-67) then cuts per request. */
 int32_t fuel_amd_knn_ties(fuel_amd_index* index, const void* query, int64_t m, int32_t k, fuel_amd_csr* out);
 
+/* DBSCAN over the index rows with the exact range search (eps inclusive, cast
+ * to the index dtype like fuel, python/dbscan.rs:45).  variant 0 = dbscan
+ * (src/cluster/dbscan.rs:14-83), 1 = parallel_dbscan
+ * (src/cluster/parallel_dbscan.rs:16-113); labels (n int64): -1 noise, else
+ * cluster ids from 0.  Replaces python/dbscan.rs:35-75 with a linear index. */
+int32_t fuel_amd_dbscan(fuel_amd_index* index, double eps, int64_t min_points, int32_t variant,
+                        int64_t* out_labels);
+
+/* OPTICS over the index rows (src/cluster/optics.rs:56-133): processing order,
+ * reachability and core distance (index dtype; core NaN when not a core
+ * point), predecessor (-1 when none) and the DBSCAN-style labels at max_eps
+ * (:345-377).  All outputs have n entries, caller-allocated.  Replaces
+ * python/dbscan.rs optics_f32/_f64 with a linear index. */
+int32_t fuel_amd_optics(fuel_amd_index* index, double max_eps, int64_t min_points, int64_t* out_ordering,
+                        void* out_reachability, void* out_core_distance, int64_t* out_predecessor,
+                        int64_t* out_labels);
+
 /* Pairwise distance matrix of n rows (square n x n, or condensed n(n-1)/2 in
  * fuel's internal lower-triangular order), caller-allocated host output.
  * Replaces _compute_pairwise_distances_{f32,f64} and the _condensed_ variants

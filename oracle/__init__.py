@@ -65,6 +65,9 @@ def _load():
         getattr(lib, f"oracle_knn_outlier_{t}").argtypes = [P, I64, I64, I64, I, I, P]
         getattr(lib, f"oracle_subset_with_ties_{t}").argtypes = [P, I64, I64]
         getattr(lib, f"oracle_subset_with_ties_{t}").restype = I64
+        getattr(lib, f"oracle_dbscan_{t}").argtypes = [P, I64, I64, ctypes.c_double, I64, I, I, I, P]
+        getattr(lib, f"oracle_optics_{t}").argtypes = [P, I64, I64, ctypes.c_double, I64, I, P, P, P, P, P]
+        getattr(lib, f"oracle_extract_xi_{t}").argtypes = [P, P, P, I64, ctypes.c_double, I64, P]
     lib.oracle_csr_free.argtypes = [ctypes.POINTER(_Csr)]
     _lib = lib
     return lib
@@ -201,3 +204,49 @@ def subset_with_ties(dist_row, k):
     dt = np.float64 if np.asarray(dist_row).dtype == np.float64 else np.float32
     d = _c(dist_row, dt)
     return int(getattr(lib, f"oracle_subset_with_ties_{_tag(dt)}")(d.ctypes.data, d.shape[0], k))
+
+
+def dbscan(X, eps, min_points, metric="euclidean", variant="dbscan", threads=None):
+    """Labels of src/cluster/dbscan.rs:14-83 ('dbscan') or
+    src/cluster/parallel_dbscan.rs:16-113 ('parallel') over the linear scan."""
+    lib = _load()
+    dt = np.float64 if np.asarray(X).dtype == np.float64 else np.float32
+    X = _c(X, dt)
+    labels = np.empty(X.shape[0], np.int64)
+    getattr(lib, f"oracle_dbscan_{_tag(dt)}")(
+        X.ctypes.data, X.shape[0], X.shape[1], float(eps), int(min_points), METRICS[metric],
+        0 if variant == "dbscan" else 1, threads or default_threads(), labels.ctypes.data)
+    return labels
+
+
+def optics(X, eps, min_points, metric="euclidean"):
+    """(ordering, reachability, core_distance (NaN = none), predecessor (-1 = none),
+    labels) of src/cluster/optics.rs:56-133 over the linear scan."""
+    lib = _load()
+    dt = np.float64 if np.asarray(X).dtype == np.float64 else np.float32
+    X = _c(X, dt)
+    n = X.shape[0]
+    ordering = np.empty(n, np.int64)
+    reach = np.empty(n, dt)
+    core = np.empty(n, dt)
+    pred = np.empty(n, np.int64)
+    labels = np.empty(n, np.int64)
+    getattr(lib, f"oracle_optics_{_tag(dt)}")(
+        X.ctypes.data, n, X.shape[1], float(eps), int(min_points), METRICS[metric], ordering.ctypes.data,
+        reach.ctypes.data, core.ctypes.data, pred.ctypes.data, labels.ctypes.data)
+    return ordering, reach, core, pred, labels
+
+
+def extract_xi(ordering, reachability, predecessor, xi, min_points):
+    """extract_xi_labels, src/cluster/optics.rs:148-304."""
+    lib = _load()
+    reach = np.asarray(reachability)
+    dt = np.float64 if reach.dtype == np.float64 else np.float32
+    ordering = np.ascontiguousarray(ordering, np.int64)
+    reach = np.ascontiguousarray(reach, dt)
+    pred = np.ascontiguousarray(predecessor, np.int64)
+    labels = np.empty(ordering.shape[0], np.int64)
+    getattr(lib, f"oracle_extract_xi_{_tag(dt)}")(
+        ordering.ctypes.data, reach.ctypes.data, pred.ctypes.data, ordering.shape[0], float(xi), int(min_points),
+        labels.ctypes.data)
+    return labels

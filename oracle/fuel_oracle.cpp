@@ -26,6 +26,16 @@
 //                                     condensed order i(i-1)/2+j of
 //                                     src/api/condensed_distance_matrix.rs:94-101.
 //   precomputed subset_with_ties .... pinned: src/search/precomputed.rs:136-165.
+//   dbscan / parallel_dbscan ........ pinned: the sklearn toy labels of
+//                                     src/cluster/dbscan.rs tests (eps 1, min_points
+//                                     1-4) and the two-cluster case; the two variants
+//                                     agree where border points are unambiguous.
+//   optics .......................... pinned: the ELKI ordering and reachability of
+//                                     src/cluster/optics.rs:587-743 (60 points,
+//                                     eps = inf, min_points 5) and the sklearn
+//                                     processing order of :554-585.
+//   extract_xi ...................... pinned: the two-valley plot of
+//                                     src/cluster/optics.rs:512-552.
 #include <immintrin.h>
 
 #include <algorithm>
@@ -34,7 +44,10 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
+#include <queue>
 #include <thread>
+#include <unordered_set>
 #include <vector>
 
 namespace {
@@ -437,6 +450,309 @@ int64_t subset_with_ties(const F* dist, int64_t len, int64_t k) {
   return c;
 }
 
+// ---------------------------------------------------------------------------
+// Density clustering over the linear-scan range search.
+//
+// dbscan, src/cluster/dbscan.rs:14-83: a hash-set frontier, drained by taking an
+// arbitrary element (the reference's frontier.iter().next()).
+template <typename F>
+void dbscan(const F* X, int64_t n, int64_t d, double eps, int64_t min_points, int metric, int64_t* labels) {
+  const int64_t UNVISITED = -2, NOISE = -1;
+  F e = (F)eps;
+  for (int64_t i = 0; i < n; ++i) labels[i] = UNVISITED;
+  int64_t cluster_id = 0;
+  std::unordered_set<int64_t> frontier;
+  std::vector<int64_t> neighbors;
+  for (int64_t p = 0; p < n; ++p) {
+    if (labels[p] != UNVISITED) continue;
+    frontier.clear();
+    for (auto& pr : linear_scan_range(X, n, d, X + p * d, e, metric)) frontier.insert(pr.index);
+    if ((int64_t)frontier.size() < min_points) {
+      labels[p] = NOISE;
+      continue;
+    }
+    labels[p] = cluster_id;
+    while (!frontier.empty()) {
+      int64_t cur = *frontier.begin();
+      frontier.erase(frontier.begin());
+      if (labels[cur] == NOISE) labels[cur] = cluster_id;
+      if (labels[cur] != UNVISITED) continue;
+      labels[cur] = cluster_id;
+      neighbors.clear();
+      for (auto& pr : linear_scan_range(X, n, d, X + cur * d, e, metric)) neighbors.push_back(pr.index);
+      if ((int64_t)neighbors.size() >= min_points) {
+        for (int64_t nb : neighbors) {
+          if (labels[nb] == NOISE) labels[nb] = cluster_id;
+          if (labels[nb] == UNVISITED) frontier.insert(nb);
+        }
+      }
+    }
+    ++cluster_id;
+  }
+}
+
+// parallel_dbscan, src/cluster/parallel_dbscan.rs:16-113 (sequentially): core
+// flags, union of neighbouring cores, labels in order of each component's first
+// core, border points take the first core of their range result.
+template <typename F>
+void parallel_dbscan(const F* X, int64_t n, int64_t d, double eps, int64_t min_points, int metric,
+                     int nthreads, int64_t* labels) {
+  F e = (F)eps;
+  std::vector<std::vector<DistPair<F>>> nb((size_t)n);
+  parallel_for(n, nthreads, [&](int64_t i) { nb[i] = linear_scan_range(X, n, d, X + i * d, e, metric); });
+  std::vector<int64_t> core_ids(n, -1), cores;
+  for (int64_t i = 0; i < n; ++i)
+    if ((int64_t)nb[i].size() >= min_points) {
+      core_ids[i] = (int64_t)cores.size();
+      cores.push_back(i);
+    }
+  if (cores.empty()) {
+    for (int64_t i = 0; i < n; ++i) labels[i] = -1;
+    return;
+  }
+  std::vector<int64_t> parent(cores.size()), rank(cores.size(), 0);
+  for (size_t i = 0; i < parent.size(); ++i) parent[i] = (int64_t)i;
+  auto find = [&](int64_t x) {
+    while (parent[x] != x) x = parent[x] = parent[parent[x]];
+    return x;
+  };
+  auto unite = [&](int64_t a, int64_t b) {
+    a = find(a);
+    b = find(b);
+    if (a == b) return;
+    if (rank[a] < rank[b]) std::swap(a, b);
+    parent[b] = a;
+    if (rank[a] == rank[b]) ++rank[a];
+  };
+  for (size_t c = 0; c < cores.size(); ++c)
+    for (auto& pr : nb[cores[c]])
+      if (core_ids[pr.index] >= 0) unite((int64_t)c, core_ids[pr.index]);
+  std::vector<int64_t> root_label(cores.size(), -1), label_of(cores.size());
+  int64_t next = 0;
+  for (size_t c = 0; c < cores.size(); ++c) {
+    int64_t r = find((int64_t)c);
+    if (root_label[r] < 0) root_label[r] = next++;
+    label_of[c] = root_label[r];
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    if (core_ids[i] >= 0) {
+      labels[i] = label_of[core_ids[i]];
+      continue;
+    }
+    int64_t a = -1;
+    for (auto& pr : nb[i]) {
+      if (a != -1) break;
+      if (core_ids[pr.index] >= 0) a = label_of[core_ids[pr.index]];
+    }
+    labels[i] = a;
+  }
+}
+
+// optics, src/cluster/optics.rs:56-133, :306-377: the seeds heap orders by
+// reachability then index (ReachCandidate's reversed Ord, :28-35).
+template <typename F>
+void optics(const F* X, int64_t n, int64_t d, double eps, int64_t min_points, int metric, int64_t* ordering,
+            F* reach, F* core, int64_t* pred, int64_t* labels) {
+  F e = (F)eps;
+  const F inf = std::numeric_limits<F>::infinity();
+  std::vector<char> processed(n, 0), is_core(n, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    reach[i] = inf;
+    core[i] = std::numeric_limits<F>::quiet_NaN();
+    pred[i] = -1;
+  }
+  struct RC {
+    F r;
+    int64_t i;
+    bool operator<(const RC& o) const {  // max-heap on the reversed order
+      if (o.r < r) return true;
+      if (r < o.r) return false;
+      return o.i < i;
+    }
+  };
+  std::priority_queue<RC> seeds;
+  int64_t no = 0;
+  auto region = [&](int64_t p) {
+    std::vector<std::pair<int64_t, F>> v;
+    for (auto& pr : linear_scan_range(X, n, d, X + p * d, e, metric)) v.push_back({pr.index, pr.distance});
+    return v;
+  };
+  auto core_distance = [&](std::vector<std::pair<int64_t, F>>& v, F& out) {
+    if ((int64_t)v.size() < min_points) return false;
+    std::nth_element(v.begin(), v.begin() + (min_points - 1), v.end(),
+                     [](const std::pair<int64_t, F>& a, const std::pair<int64_t, F>& b) { return a.second < b.second; });
+    out = v[min_points - 1].second;
+    return true;
+  };
+  auto update = [&](const std::vector<std::pair<int64_t, F>>& v, int64_t p, F cd) {
+    for (auto& pr : v) {
+      if (pr.first == p || processed[pr.first]) continue;
+      F nr = std::max(cd, pr.second);
+      if (nr < reach[pr.first]) {
+        reach[pr.first] = nr;
+        pred[pr.first] = p;
+        seeds.push(RC{nr, pr.first});
+      }
+    }
+  };
+  for (int64_t p = 0; p < n; ++p) {
+    if (processed[p]) continue;
+    auto nbs = region(p);
+    processed[p] = 1;
+    ordering[no++] = p;
+    F cd;
+    if (core_distance(nbs, cd)) {
+      core[p] = cd;
+      is_core[p] = 1;
+      while (!seeds.empty()) seeds.pop();
+      update(nbs, p, cd);
+      while (!seeds.empty()) {
+        RC c = seeds.top();
+        seeds.pop();
+        if (processed[c.i]) continue;
+        if (c.r > reach[c.i]) continue;
+        auto cn = region(c.i);
+        processed[c.i] = 1;
+        ordering[no++] = c.i;
+        F cc;
+        if (core_distance(cn, cc)) {
+          core[c.i] = cc;
+          is_core[c.i] = 1;
+          update(cn, c.i, cc);
+        }
+      }
+    }
+  }
+  int64_t cid = -1;
+  for (int64_t o = 0; o < n; ++o) {
+    int64_t p = ordering[o];
+    F r = reach[p], c = is_core[p] ? core[p] : inf;
+    if (r > e)
+      labels[p] = c <= e ? ++cid : -1;
+    else
+      labels[p] = cid >= 0 ? cid : -1;
+  }
+}
+
+// extract_xi_labels, src/cluster/optics.rs:148-304, :379-453: steep-down areas
+// kept in a list, cluster intervals from each steep-up area, the predecessor
+// correction, then the smallest intervals claim their points first.
+template <typename F>
+void extract_xi(const int64_t* ordering, const F* reach, const int64_t* pred, int64_t n, double xi_in,
+                int64_t min_points, int64_t* labels) {
+  for (int64_t i = 0; i < n; ++i) labels[i] = -1;
+  if (n == 0) return;
+  const F xi = (F)xi_in, ixi = (F)1 - xi;
+  const F inf = std::numeric_limits<F>::infinity();
+  std::vector<F> r(n);
+  std::vector<int64_t> pos(n, -1);
+  for (int64_t o = 0; o < n; ++o) {
+    r[o] = reach[ordering[o]];
+    pos[ordering[o]] = o;
+  }
+  auto steep_up = [&](int64_t i, F f) { return std::isfinite(r[i]) && (i + 1 >= n || r[i] <= r[i + 1] * f); };
+  auto steep_down = [&](int64_t i, F f) { return i + 1 < n && std::isfinite(r[i + 1]) && r[i + 1] <= r[i] * f; };
+  auto next_r = [&](int64_t i) { return i + 1 < n ? r[i + 1] : inf; };
+  struct SDA {
+    int64_t start;
+    F maximum, mib;
+  };
+  auto filter = [&](F mib, std::vector<SDA>& set) {
+    std::vector<SDA> keep;
+    for (auto s : set) {
+      if (s.maximum * ixi <= mib) continue;
+      if (mib > s.mib) s.mib = mib;
+      keep.push_back(s);
+    }
+    set.swap(keep);
+  };
+  auto pred_filter = [&](int64_t cstart, int64_t cend) {
+    if (cend >= n) return n - 1;
+    F sv = r[cstart];
+    while (cend > cstart) {
+      int64_t p = ordering[cend];
+      if (r[cend] < sv) break;
+      int64_t pp = pred[p];
+      if (pp >= 0 && pos[pp] >= cstart && pos[pp] < cend) break;
+      --cend;
+    }
+    return cend;
+  };
+  F mib = 0;
+  std::vector<SDA> sdaset;
+  std::vector<std::pair<int64_t, int64_t>> intervals;
+  int64_t scan = 0;
+  while (scan < n) {
+    mib = std::max(mib, r[scan]);
+    if (steep_down(scan, ixi)) {
+      filter(mib, sdaset);
+      F startval = r[scan];
+      mib = 0;
+      int64_t startsteep = scan, endsteep = scan;
+      ++scan;
+      while (scan < n) {
+        if (steep_down(scan, ixi))
+          endsteep = scan;
+        else if (!steep_down(scan, (F)1) || scan - endsteep > min_points)
+          break;
+        ++scan;
+      }
+      sdaset.push_back(SDA{startsteep, startval, (F)0});
+      continue;
+    }
+    if (steep_up(scan, ixi)) {
+      filter(mib, sdaset);
+      int64_t endsteep = scan;
+      mib = r[scan];
+      F esuccr = next_r(scan);
+      while (std::isfinite(esuccr) && scan < n) {
+        ++scan;
+        if (scan >= n) break;
+        if (steep_up(scan, ixi)) {
+          endsteep = scan;
+          mib = r[scan];
+          esuccr = next_r(scan);
+        } else if (!steep_up(scan, (F)1) || scan - endsteep > min_points) {
+          break;
+        }
+      }
+      if (std::isinf(esuccr) && scan < n) ++scan;
+      for (auto it = sdaset.rbegin(); it != sdaset.rend(); ++it) {
+        int64_t cstart = it->start, cend = pred_filter(it->start, endsteep);
+        F eu = cend + 1 < n ? r[cend + 1] : inf;
+        if (it->mib > std::min(it->maximum, eu) * ixi) continue;
+        if (it->maximum * ixi >= eu) {
+          while (cstart < cend && r[cstart + 1] * ixi > eu) ++cstart;
+        } else if (eu * ixi >= it->maximum) {
+          while (cend > cstart && r[cend] * ixi > it->maximum) --cend;
+        }
+        cend = pred_filter(cstart, cend);
+        if (cend + 1 - cstart < min_points) continue;
+        intervals.push_back({cstart, cend});
+      }
+      continue;
+    }
+    ++scan;
+  }
+  std::stable_sort(intervals.begin(), intervals.end(), [](const auto& a, const auto& b) {
+    return std::make_pair(a.second - a.first, a.first) < std::make_pair(b.second - b.first, b.first);
+  });
+  std::vector<char> claimed(n, 0);
+  int64_t cid = 0;
+  for (auto& iv : intervals) {
+    bool any = false;
+    for (int64_t o = iv.first; o <= iv.second; ++o) {
+      int64_t p = ordering[o];
+      if (!claimed[p]) {
+        claimed[p] = 1;
+        labels[p] = cid;
+        any = true;
+      }
+    }
+    if (any) ++cid;
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -547,6 +863,37 @@ int64_t oracle_subset_with_ties_f64(const double* dist, int64_t len, int64_t k) 
 }
 int64_t oracle_subset_with_ties_f32(const float* dist, int64_t len, int64_t k) {
   return subset_with_ties<float>(dist, len, k);
+}
+
+void oracle_dbscan_f32(const float* X, int64_t n, int64_t d, double eps, int64_t min_points, int metric,
+                       int variant, int nthreads, int64_t* labels) {
+  if (variant == 0)
+    dbscan<float>(X, n, d, eps, min_points, metric, labels);
+  else
+    parallel_dbscan<float>(X, n, d, eps, min_points, metric, nthreads, labels);
+}
+void oracle_dbscan_f64(const double* X, int64_t n, int64_t d, double eps, int64_t min_points, int metric,
+                       int variant, int nthreads, int64_t* labels) {
+  if (variant == 0)
+    dbscan<double>(X, n, d, eps, min_points, metric, labels);
+  else
+    parallel_dbscan<double>(X, n, d, eps, min_points, metric, nthreads, labels);
+}
+void oracle_optics_f32(const float* X, int64_t n, int64_t d, double eps, int64_t min_points, int metric,
+                       int64_t* ordering, float* reach, float* core, int64_t* pred, int64_t* labels) {
+  optics<float>(X, n, d, eps, min_points, metric, ordering, reach, core, pred, labels);
+}
+void oracle_optics_f64(const double* X, int64_t n, int64_t d, double eps, int64_t min_points, int metric,
+                       int64_t* ordering, double* reach, double* core, int64_t* pred, int64_t* labels) {
+  optics<double>(X, n, d, eps, min_points, metric, ordering, reach, core, pred, labels);
+}
+void oracle_extract_xi_f32(const int64_t* ordering, const float* reach, const int64_t* pred, int64_t n,
+                           double xi, int64_t min_points, int64_t* labels) {
+  extract_xi<float>(ordering, reach, pred, n, xi, min_points, labels);
+}
+void oracle_extract_xi_f64(const int64_t* ordering, const double* reach, const int64_t* pred, int64_t n,
+                           double xi, int64_t min_points, int64_t* labels) {
+  extract_xi<double>(ordering, reach, pred, n, xi, min_points, labels);
 }
 
 }  // extern "C"

@@ -224,3 +224,81 @@ def test_lof_degenerate_cases(oracle):
     assert oracle.lof(np.zeros((1, 3)), 5).tolist() == [1.0]
     # all duplicates: reachability sums 0 -> lrd inf -> score 1
     assert oracle.lof(np.zeros((6, 2)), 3).tolist() == [1.0] * 6
+
+
+# ---- density clustering (oracle restatements and the host Xi extraction) ----
+
+SKLEARN_TOY = [(1, [0, 1, 1, 1, 2, 3, 4]), (2, [-1, 0, 0, 0, -1, -1, -1]), (3, [-1, 0, 0, 0, -1, -1, -1]),
+               (4, [-1] * 7)]
+
+
+@pytest.mark.parametrize("variant", ["dbscan", "parallel"])
+def test_dbscan_sklearn_toy_labels(oracle, variant):
+    # src/cluster/dbscan.rs and parallel_dbscan.rs tests: eps 1.0 on 1-d points
+    X = np.array([[0.0], [2.0], [3.0], [4.0], [6.0], [8.0], [10.0]])
+    for min_points, expected in SKLEARN_TOY:
+        assert list(oracle.dbscan(X, 1.0, min_points, variant=variant)) == expected
+
+
+@pytest.mark.parametrize("variant", ["dbscan", "parallel"])
+def test_dbscan_two_clusters_and_noise(oracle, variant):
+    P = np.array([[0, 0], [0.1, 0], [0, 0.1], [10, 10], [10.1, 10], [10, 10.1], [5, 5]], np.float64)
+    labels = oracle.dbscan(P, 0.25, 3, variant=variant)
+    assert labels[6] == -1 and labels[0] >= 0 and labels[3] >= 0 and labels[0] != labels[3]
+    assert labels[1] == labels[2] == labels[0] and labels[4] == labels[5] == labels[3]
+
+
+def test_optics_sklearn_processing_order(oracle):
+    # src/cluster/optics.rs:554-585
+    o, r, c, p, l = oracle.optics(np.array([[0.0], [10.0], [-10.0], [25.0]]), 15.0, 3)
+    assert list(o) == [0, 1, 2, 3]
+    assert np.isinf(r[0]) and list(r[1:]) == [10.0, 10.0, 15.0]
+    assert list(c[:2]) == [10.0, 15.0] and np.isnan(c[2]) and np.isnan(c[3])
+
+
+def test_optics_elki_reference(oracle):
+    # src/cluster/optics.rs:587-743: ELKI ordering and reachability (fixture)
+    import json
+    import os
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "optics_elki_60.json")))
+    X = np.array(g["points"])
+    o, r, c, p, l = oracle.optics(X, float("inf"), g["min_points"])
+    assert list(o) == g["expected_ordering"]
+    exp = np.array([np.inf if v is None else v for v in g["expected_reachability_by_order"]])
+    got = r[o]
+    assert np.array_equal(np.isinf(got), np.isinf(exp))
+    fin = ~np.isinf(exp)
+    assert np.max(np.abs(got[fin] - exp[fin])) < 1e-9
+
+
+def _two_valleys():
+    n = 16
+    reach = np.array([np.inf, 10, 8, 6, 4, 6, 8, 10, np.inf, 9, 7, 5, 3, 5, 7, 9], np.float64)
+    ordering = np.arange(n, dtype=np.int64)
+    pred = np.array([-1] + list(range(n - 1)), np.int64)
+    return ordering, reach, pred
+
+
+def test_extract_xi_two_valleys(oracle):
+    # src/cluster/optics.rs:512-552, for the oracle and for the product's host code
+    from fuel_amd.cluster import extract_xi_labels
+    ordering, reach, pred = _two_valleys()
+    for labels in (oracle.extract_xi(ordering, reach, pred, 0.2, 3), extract_xi_labels(ordering, reach, pred, 0.2, 3)):
+        assert labels[4] >= 0 and labels[12] >= 0 and labels[4] != labels[12]
+        assert len({int(v) for v in labels if v >= 0}) >= 2
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_extract_xi_host_matches_oracle(oracle, seed, dtype):
+    # the product's Python Xi extraction against the oracle's C++ one on OPTICS
+    # outputs of clustered data (several xi and min_points)
+    from fuel_amd.cluster import extract_xi_labels
+    rng = np.random.default_rng(seed)
+    centers = rng.uniform(-8, 8, (4, 2))
+    X = np.concatenate([c + rng.standard_normal((60, 2)) * s for c, s in zip(centers, (0.3, 0.6, 1.0, 0.2))])
+    X = np.concatenate([X, rng.uniform(-10, 10, (20, 2))]).astype(dtype)
+    o, r, c, p, l = oracle.optics(X, float("inf"), 5)
+    for xi in (0.05, 0.1, 0.3):
+        for mp in (3, 5, 10):
+            assert np.array_equal(extract_xi_labels(o, r, p, xi, mp), oracle.extract_xi(o, r, p, xi, mp))
