@@ -31,6 +31,7 @@ SOURCES = [
     "finalize_rerank.hip",
     "prep.hip",
     "range_pairwise.hip",
+    "kmeans_pairwise.hip",
     "lof.hip",
     "sort.hip",
 ]

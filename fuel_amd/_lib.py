@@ -41,7 +41,7 @@ EXPORTED = [
     "fuel_amd_index_create", "fuel_amd_index_create_device", "fuel_amd_index_destroy",
     "fuel_amd_knn", "fuel_amd_knn_device", "fuel_amd_range", "fuel_amd_knn_ties", "fuel_amd_csr_free",
     "fuel_amd_pairwise", "fuel_amd_lof", "fuel_amd_knn_outlier", "fuel_amd_last_stats",
-    "fuel_amd_dbscan", "fuel_amd_optics",
+    "fuel_amd_dbscan", "fuel_amd_optics", "fuel_amd_pairwise_sqdist", "fuel_amd_pairwise_sqdist_device",
     "fuel_amd_timing_enable", "fuel_amd_timing_read",
 ]
 
@@ -108,6 +108,10 @@ def load():
     lib.fuel_amd_knn_device.restype = I32
     lib.fuel_amd_range.argtypes = [P, P, I64, ctypes.c_double, I32, ctypes.POINTER(Csr)]
     lib.fuel_amd_range.restype = I32
+    lib.fuel_amd_pairwise_sqdist.argtypes = [P, I64, P, I64, I32, I32, P]
+    lib.fuel_amd_pairwise_sqdist.restype = I32
+    lib.fuel_amd_pairwise_sqdist_device.argtypes = [P, I64, I64, P, I64, I64, I32, I32, P, I64, P]
+    lib.fuel_amd_pairwise_sqdist_device.restype = I32
     lib.fuel_amd_dbscan.argtypes = [P, ctypes.c_double, I64, I32, P]
     lib.fuel_amd_dbscan.restype = I32
     lib.fuel_amd_optics.argtypes = [P, ctypes.c_double, I64, P, P, P, P, P]
@@ -325,3 +329,18 @@ def request_cancel():
 
 def clear_cancel():
     load().fuel_amd_clear_cancel()
+
+
+def pairwise_sqdist(points1, points2):
+    """fuel's math::pairwise_sqdist on the GPU: (nrows, ncols) squared distances
+    in the reference's k-means rounding order."""
+    p1 = np.asarray(points1)
+    dt = np.float64 if p1.dtype == np.float64 else np.float32
+    p1 = np.ascontiguousarray(p1, dt)
+    p2 = np.ascontiguousarray(points2, dt)
+    if p1.ndim != 2 or p2.ndim != 2 or p1.shape[1] != p2.shape[1]:
+        raise ValueError("point dimensionality mismatch")
+    out = np.empty((p1.shape[0], p2.shape[0]), dt)
+    check(load().fuel_amd_pairwise_sqdist(p1.ctypes.data, p1.shape[0], p2.ctypes.data, p2.shape[0], p1.shape[1],
+                                          dtype_code(dt), out.ctypes.data))
+    return out

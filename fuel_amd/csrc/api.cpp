@@ -120,6 +120,32 @@ void timing_mark(fuel_amd_index* ix, hipStream_t st, bool begin, const char* nam
 
 }  // namespace fa
 
+namespace fa {
+template <typename F>
+int32_t kmeans_pairwise_device(const F* p1, int64_t m, int64_t ld1, const F* p2, int64_t n, int64_t ld2, int d,
+                               F* out, int64_t ldo, hipStream_t st) {
+  KmeansPairwiseArgs<F> a{};
+  a.P1 = p1;
+  a.ld1 = ld1;
+  a.m = m;
+  a.P2 = p2;
+  a.ld2 = ld2;
+  a.n = n;
+  a.d = d;
+  a.out = out;
+  a.ldo = ldo;
+  hipError_t e = launch_kmeans_pairwise<F>(a, st);
+  if (e != hipSuccess) return hip_fail(e, "pairwise_sqdist launch");
+  return FUEL_AMD_OK;
+}
+
+int32_t check_sqdist_args(int64_t m, int64_t n, int32_t d, int32_t dtype) {
+  if (m < 0 || n < 0 || d < 0) return fail(FUEL_AMD_ERR_INVALID, "sizes must be non-negative");
+  if (dtype != FUEL_AMD_F32 && dtype != FUEL_AMD_F64) return fail(FUEL_AMD_ERR_INVALID, "dtype must be F32 or F64");
+  return FUEL_AMD_OK;
+}
+}  // namespace fa
+
 // ---------------------------------------------------------------------------
 extern "C" {
 
@@ -384,6 +410,70 @@ int32_t fuel_amd_optics(fuel_amd_index* ix, double max_eps, int64_t min_points, 
   if (rc) return rc;
   return optics_run(ix, max_eps, min_points, out_ordering, out_reachability, out_core_distance, out_predecessor,
                     out_labels);
+}
+
+// ---- k-means distance matrix (math::pairwise_sqdist) -----------------------
+
+
+int32_t fuel_amd_pairwise_sqdist_device(const void* d_points1, int64_t nrows, int64_t ld1, const void* d_points2,
+                                        int64_t ncols, int64_t ld2, int32_t d, int32_t dtype, void* d_out,
+                                        int64_t ldo, void* stream) {
+  g_last_error.clear();
+  int32_t rc = check_sqdist_args(nrows, ncols, d, dtype);
+  if (rc) return rc;
+  if (nrows == 0 || ncols == 0) return FUEL_AMD_OK;
+  if (!d_points1 || !d_points2 || !d_out) return fail(FUEL_AMD_ERR_INVALID, "NULL buffer");
+  if (ld1 < d || ld2 < d || ldo < ncols) return fail(FUEL_AMD_ERR_INVALID, "row pitch smaller than the row");
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) (void)hipGetLastError();
+  rc = check_device(dev);  // the caller's current device
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  return dtype == FUEL_AMD_F32
+             ? kmeans_pairwise_device<float>((const float*)d_points1, nrows, ld1, (const float*)d_points2, ncols,
+                                             ld2, d, (float*)d_out, ldo, st)
+             : kmeans_pairwise_device<double>((const double*)d_points1, nrows, ld1, (const double*)d_points2, ncols,
+                                              ld2, d, (double*)d_out, ldo, st);
+}
+
+int32_t fuel_amd_pairwise_sqdist(const void* points1, int64_t nrows, const void* points2, int64_t ncols, int32_t d,
+                                 int32_t dtype, void* out) {
+  g_last_error.clear();
+  int32_t rc = check_sqdist_args(nrows, ncols, d, dtype);
+  if (rc) return rc;
+  if (nrows == 0 || ncols == 0) return FUEL_AMD_OK;
+  if (!out || (d > 0 && (!points1 || !points2))) return fail(FUEL_AMD_ERR_INVALID, "NULL buffer");
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) (void)hipGetLastError();
+  rc = check_device(dev);
+  if (rc) return rc;
+  const size_t es = dtype == FUEL_AMD_F32 ? 4 : 8;
+  const int64_t dd = d > 0 ? d : 1;
+  DevBuf b1, b2, bo;
+  hipStream_t st = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+  e = b1.ensure(size_t(nrows) * dd * es);
+  if (e == hipSuccess) e = b2.ensure(size_t(ncols) * dd * es);
+  if (e == hipSuccess) e = bo.ensure(size_t(nrows) * ncols * es);
+  if (e == hipSuccess && d > 0) e = hipMemcpyAsync(b1.p, points1, size_t(nrows) * d * es, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && d > 0) e = hipMemcpyAsync(b2.p, points2, size_t(ncols) * d * es, hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) {
+    hipStreamDestroy(st);
+    return hip_fail(e, "pairwise_sqdist buffers");
+  }
+  rc = dtype == FUEL_AMD_F32
+           ? kmeans_pairwise_device<float>(b1.as<float>(), nrows, dd, b2.as<float>(), ncols, dd, d, bo.as<float>(),
+                                           ncols, st)
+           : kmeans_pairwise_device<double>(b1.as<double>(), nrows, dd, b2.as<double>(), ncols, dd, d,
+                                            bo.as<double>(), ncols, st);
+  if (rc == FUEL_AMD_OK) {
+    e = hipMemcpyAsync(out, bo.p, size_t(nrows) * ncols * es, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) rc = hip_fail(e, "pairwise_sqdist download");
+  }
+  hipStreamDestroy(st);
+  return rc;
 }
 
 // ---- pairwise ---------------------------------------------------------------

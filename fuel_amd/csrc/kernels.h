@@ -195,6 +195,23 @@ struct PrepQArgs {
   const float* theta;   // optional: slot i's initial threshold (key units); +inf without
 };
 
+// fuel's math::pairwise_sqdist (k-means distance matrix): out[i * ldo + j] for
+// rows i of P1 (m x d, pitch ld1) and rows j of P2 (n x d, pitch ld2).
+template <typename F>
+struct KmeansPairwiseArgs {
+  const F* P1;
+  int64_t ld1;
+  int64_t m;
+  const F* P2;
+  int64_t ld2;
+  int64_t n;
+  int d;
+  F* out;
+  int64_t ldo;
+};
+template <typename F>
+hipError_t launch_kmeans_pairwise(const KmeansPairwiseArgs<F>& a, hipStream_t st);
+
 // Finite sentinels of the MFMA scan.  Thresholds start at KEY_LIMIT instead of
 // +inf and padding rows carry PAD_HXN as their C input, so a padding row's
 // result (PAD_HXN) never beats a threshold and every value stays finite, which

@@ -173,6 +173,22 @@ int32_t fuel_amd_optics(fuel_amd_index* index, double max_eps, int64_t min_point
                         void* out_reachability, void* out_core_distance, int64_t* out_predecessor,
                         int64_t* out_labels);
 
+/* k-means distance matrix: out[i * ncols + j] = squared distance of row i of
+ * points1 (nrows x d) and row j of points2 (ncols x d), in the rounding order
+ * of fuel's math::pairwise_sqdist (src/math/mod.rs:562-579): for d >= 4 one
+ * sequential fused multiply-add chain per output (src/math/avx2.rs:888-963,
+ * :1036-1082), for d < 4 the rounded squares summed left to right
+ * (src/math/scalar.rs:10-16).  Replaces N::vec_pairwise_sqdist in the k-means
+ * family (src/cluster/kmeans/lloyd.rs:76, elkan.rs:83, hamerly.rs:89,
+ * selkan.rs:73, shamerly.rs:74).  Host arrays, synchronous. */
+int32_t fuel_amd_pairwise_sqdist(const void* points1, int64_t nrows, const void* points2, int64_t ncols,
+                                 int32_t d, int32_t dtype, void* out);
+/* Same on device buffers (row pitches in elements), asynchronous on `stream`,
+ * for a k-means loop that keeps points and centres resident. */
+int32_t fuel_amd_pairwise_sqdist_device(const void* d_points1, int64_t nrows, int64_t ld1,
+                                        const void* d_points2, int64_t ncols, int64_t ld2, int32_t d,
+                                        int32_t dtype, void* d_out, int64_t ldo, void* stream);
+
 /* Pairwise distance matrix of n rows (square n x n, or condensed n(n-1)/2 in
  * fuel's internal lower-triangular order), caller-allocated host output.
  * Replaces _compute_pairwise_distances_{f32,f64} and the _condensed_ variants

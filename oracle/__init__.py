@@ -65,6 +65,7 @@ def _load():
         getattr(lib, f"oracle_knn_outlier_{t}").argtypes = [P, I64, I64, I64, I, I, P]
         getattr(lib, f"oracle_subset_with_ties_{t}").argtypes = [P, I64, I64]
         getattr(lib, f"oracle_subset_with_ties_{t}").restype = I64
+        getattr(lib, f"oracle_kmeans_pairwise_{t}").argtypes = [P, I64, P, I64, I64, I, P]
         getattr(lib, f"oracle_dbscan_{t}").argtypes = [P, I64, I64, ctypes.c_double, I64, I, I, I, P]
         getattr(lib, f"oracle_optics_{t}").argtypes = [P, I64, I64, ctypes.c_double, I64, I, P, P, P, P, P]
         getattr(lib, f"oracle_extract_xi_{t}").argtypes = [P, P, P, I64, ctypes.c_double, I64, P]
@@ -250,3 +251,15 @@ def extract_xi(ordering, reachability, predecessor, xi, min_points):
         ordering.ctypes.data, reach.ctypes.data, pred.ctypes.data, ordering.shape[0], float(xi), int(min_points),
         labels.ctypes.data)
     return labels
+
+
+def kmeans_pairwise(A, B, threads=None):
+    """math::pairwise_sqdist (src/math/mod.rs:562-579): (m, n) in the k-means
+    rounding order."""
+    lib = _load()
+    dt = np.float64 if np.asarray(A).dtype == np.float64 else np.float32
+    A, B = _c(A, dt), _c(B, dt)
+    out = np.empty((A.shape[0], B.shape[0]), dt)
+    getattr(lib, f"oracle_kmeans_pairwise_{_tag(dt)}")(A.ctypes.data, A.shape[0], B.ctypes.data, B.shape[0],
+                                                       A.shape[1], threads or default_threads(), out.ctypes.data)
+    return out

@@ -34,6 +34,9 @@
 //                                     src/cluster/optics.rs:587-743 (60 points,
 //                                     eps = inf, min_points 5) and the sklearn
 //                                     processing order of :554-585.
+//   kmeans_pairwise ................. pinned: the exact matrices of
+//                                     src/math/mod.rs:672-690 and an exact-rational
+//                                     emulation of the sequential fused chain.
 //   extract_xi ...................... pinned: the two-valley plot of
 //                                     src/cluster/optics.rs:512-552.
 #include <immintrin.h>
@@ -451,6 +454,29 @@ int64_t subset_with_ties(const F* dist, int64_t len, int64_t k) {
 }
 
 // ---------------------------------------------------------------------------
+// a17: math::pairwise_sqdist, src/math/mod.rs:562-579 (the k-means distance
+// matrix).  d >= 4: the AVX2 micro-kernel (src/math/avx2.rs:888-963,
+// :1036-1082) gives every output one sequential fused chain
+// acc = fma(a_k - b_k, a_k - b_k, acc) from +0; d < 4: scalar::sqdist
+// (src/math/scalar.rs:10-16), the rounded squares summed left to right.
+template <typename F>
+void kmeans_pairwise(const F* A, int64_t m, const F* B, int64_t n, int64_t d, int nthreads, F* out) {
+  parallel_for(m, nthreads, [&](int64_t i) {
+    for (int64_t j = 0; j < n; ++j) {
+      F acc = 0;
+      for (int64_t k = 0; k < d; ++k) {
+        F t = A[i * d + k] - B[j * d + k];
+        if (d >= 4)
+          acc = std::fma(t, t, acc);
+        else
+          acc = acc + t * t;
+      }
+      out[i * n + j] = acc;
+    }
+  });
+}
+
+// ---------------------------------------------------------------------------
 // Density clustering over the linear-scan range search.
 //
 // dbscan, src/cluster/dbscan.rs:14-83: a hash-set frontier, drained by taking an
@@ -863,6 +889,15 @@ int64_t oracle_subset_with_ties_f64(const double* dist, int64_t len, int64_t k) 
 }
 int64_t oracle_subset_with_ties_f32(const float* dist, int64_t len, int64_t k) {
   return subset_with_ties<float>(dist, len, k);
+}
+
+void oracle_kmeans_pairwise_f32(const float* A, int64_t m, const float* B, int64_t n, int64_t d, int nthreads,
+                                float* out) {
+  kmeans_pairwise<float>(A, m, B, n, d, nthreads, out);
+}
+void oracle_kmeans_pairwise_f64(const double* A, int64_t m, const double* B, int64_t n, int64_t d, int nthreads,
+                                double* out) {
+  kmeans_pairwise<double>(A, m, B, n, d, nthreads, out);
 }
 
 void oracle_dbscan_f32(const float* X, int64_t n, int64_t d, double eps, int64_t min_points, int metric,

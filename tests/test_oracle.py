@@ -302,3 +302,54 @@ def test_extract_xi_host_matches_oracle(oracle, seed, dtype):
     for xi in (0.05, 0.1, 0.3):
         for mp in (3, 5, 10):
             assert np.array_equal(extract_xi_labels(o, r, p, xi, mp), oracle.extract_xi(o, r, p, xi, mp))
+
+
+# ---- math::pairwise_sqdist (k-means distance matrix) ----
+
+def test_kmeans_pairwise_reference_matrices(oracle):
+    # src/math/mod.rs:672-690
+    p = np.array([[0.0, 1.0], [2.0, 3.0], [1.0, 4.0]], np.float32)
+    assert oracle.kmeans_pairwise(p, p).ravel().tolist() == [0, 8, 10, 8, 0, 2, 10, 2, 0]
+    q = np.array([[1, 2, 3, 4], [4, 3, 2, 1], [0, 0, 0, 0]], np.float64)
+    assert oracle.kmeans_pairwise(q, q).ravel().tolist() == [0, 20, 30, 20, 0, 30, 30, 30, 0]
+
+
+def _round_f32(fr):
+    """Exact round-to-nearest-even of a Fraction to float32."""
+    from fractions import Fraction
+    f = np.float32(float(fr))
+    best = None
+    for c in (np.nextafter(f, np.float32(-np.inf)), f, np.nextafter(f, np.float32(np.inf))):
+        err = abs(Fraction(float(c)) - fr)
+        if best is None or err < best[0] or (err == best[0] and (int(c.view(np.uint32)) & 1) == 0):
+            best = (err, c)
+    return best[1]
+
+
+def _chain_exact(a, b, fused):
+    from fractions import Fraction
+    acc = np.float32(0)
+    for x, y in zip(a, b):
+        t = np.float32(x) - np.float32(y)  # one rounding (numpy f32 subtraction)
+        if fused:
+            acc = _round_f32(Fraction(float(t)) ** 2 + Fraction(float(acc)))
+        else:
+            acc = _round_f32(Fraction(float(_round_f32(Fraction(float(t)) ** 2))) + Fraction(float(acc)))
+    return acc
+
+
+@pytest.mark.parametrize("d", [1, 2, 3, 4, 5, 8, 13])
+def test_kmeans_pairwise_rounding_order(oracle, d):
+    # a sequential fused chain for d >= 4, rounded squares added in order below;
+    # the values make other orders (8 lanes, unfused) round differently
+    rng = np.random.default_rng(40 + d)
+    A = (rng.standard_normal((6, d)) * rng.choice([1e-3, 1.0, 3e3], (6, d))).astype(np.float32)
+    B = (rng.standard_normal((5, d)) * rng.choice([1e-3, 1.0, 3e3], (5, d))).astype(np.float32)
+    got = oracle.kmeans_pairwise(A, B)
+    for i in range(6):
+        for j in range(5):
+            exp = _chain_exact(A[i], B[j], fused=d >= 4)
+            assert got[i, j].view(np.uint32) == exp.view(np.uint32), (i, j)
+    if d >= 8:  # the search kernel's 8-lane order differs on these inputs
+        lanes = np.array([[oracle.sqdist(A[i], B[j]) for j in range(5)] for i in range(6)], np.float32)
+        assert np.any(lanes.view(np.uint32) != got.view(np.uint32))
