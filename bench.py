@@ -47,6 +47,10 @@ def parse():
     p.add_argument("--cpu-sample", type=int, default=1000, help="queries timed on the CPU baseline (0: skip)")
     p.add_argument("--fp32-steps", type=int, default=2,
                    help="timed steps of the plain FP32 MFMA form run beside the mixed one (0: skip)")
+    p.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "dbscan", "kmeans"],
+                   help="c2: the headline kNN line; the others print their own line (not the headline): "
+                        "c1 self-kNN 10k x 16 + pairwise 2000^2, c3 LOF 70k x 784 k=20, dbscan on c1, "
+                        "kmeans distance matrix 1M x 64 vs 100 centres")
     p.add_argument("--host-path", action="store_true",
                    help="only time the host-pointer entry point (queries and results cross PCIe each call)")
     p.add_argument("--copy-probe", type=int, default=0,
@@ -139,6 +143,133 @@ def verify(oracle, X, Q, gi, gd, k, nverify):
             "max_rel_err_vs_fp64": max_rel_err}
 
 
+VALU_F32_PEAK = 256 * 4 * 16 * 2 * 2.4e9  # FMA lanes/s x 2 flop, no packed FP32 (78.6 TFLOP/s)
+
+
+def timed(fn, reps):
+    """Median wall time of `reps` calls (each synchronous)."""
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def secondary(args):
+    """Lines for the other configurations and the widened rows: GPU time beside
+    the oracle's CPU time on the same inputs, parity checked on the way."""
+    from fuel_amd import _lib, workloads
+    import fuel_amd
+    import oracle
+    threads = oracle.default_threads()
+    line = {"workload": args.workload, "n_gpus": 1, "data": "synthetic", "cpu_threads": threads}
+    if args.workload == "c1":
+        X = workloads.c1()
+        ix = _lib.Index(X, "euclidean")
+        knn = lambda: ix.knn(X, 10, _lib.EXCLUDE_PY_LINEAR)  # noqa: E731
+        knn()
+        t_g = timed(knn, 5)
+        gi, gd = knn()
+        t0 = time.perf_counter()
+        oi, od = oracle.knn(X, X, 10, "euclidean", "python_linear", threads=threads)
+        t_c = time.perf_counter() - t0
+        P = X[:2000]
+        pw = lambda: fuel_amd.pairwise_distances(P, "sqeuclidean", form="square")  # noqa: E731
+        pw()
+        t_pg = timed(pw, 5)
+        t0 = time.perf_counter()
+        po = oracle.pairwise(P, "sqeuclidean", "square", threads=threads)
+        t_pc = time.perf_counter() - t0
+        line.update({"config": "self-kNN n=10000 d=16 k=10 exclude_self (host arrays) + pairwise 2000^2 sqeuclidean",
+                     "knn_gpu_ms": round(t_g * 1e3, 3), "knn_cpu_ms": round(t_c * 1e3, 3),
+                     "knn_identical": bool(np.array_equal(gi, oi) and gd.tobytes() == od.tobytes()),
+                     "pairwise_gpu_ms": round(t_pg * 1e3, 3), "pairwise_cpu_ms": round(t_pc * 1e3, 3),
+                     "pairwise_identical": bool(pw().tobytes() == po.tobytes())})
+    elif args.workload == "c3":
+        X = workloads.c3()
+        ix = _lib.Index(X, "euclidean")
+        lof = lambda: ix.lof(20)  # noqa: E731
+        lof()
+        t_g = timed(lof, 3)
+        st = ix.stats()
+        S = 500
+        t0 = time.perf_counter()
+        oracle.knn(X, X[:S], 21, "euclidean", "for_each_knn", threads=threads)
+        t_s = time.perf_counter() - t0
+        line.update({"config": "LOF n=70000 d=784 integers 0..255 (80% zero) k=20 (self-kNN + LOF, host arrays)",
+                     "lof_gpu_ms": round(t_g * 1e3, 3), "knn_stats": st,
+                     "cpu_knn_ms_extrapolated": round(t_s / S * X.shape[0] * 1e3, 1),
+                     "cpu_sample": f"{S} self-kNN queries of the oracle in {t_s:.2f}s, scaled to 70000",
+                     "algorithmic_flop": 2.0 * X.shape[0] ** 2 * X.shape[1]})
+        # parity on a sub-problem the oracle finishes in seconds
+        Xs = X[:6000]
+        g = _lib.Index(Xs, "euclidean").lof(20)
+        line["lof_identical_6000_subset"] = bool(g.tobytes() == oracle.lof(Xs, 20, threads=threads).tobytes())
+    elif args.workload == "dbscan":
+        from fuel_amd.cluster import dbscan, optics
+        X = workloads.c1()
+        for variant in ("dbscan", "parallel"):
+            f = lambda: dbscan(X, 4.0, 10, variant=variant)  # noqa: E731
+            f()
+            t_g = timed(f, 3)
+            t0 = time.perf_counter()
+            o = oracle.dbscan(X, 4.0, 10, variant=variant, threads=threads)
+            t_c = time.perf_counter() - t0
+            line[variant] = {"gpu_ms": round(t_g * 1e3, 3), "cpu_ms": round(t_c * 1e3, 3),
+                             "identical": bool(np.array_equal(f(), o)), "clusters": int(o.max() + 1)}
+        f = lambda: optics(X, 4.0, 10)  # noqa: E731
+        f()
+        t_g = timed(f, 3)
+        t0 = time.perf_counter()
+        o = oracle.optics(X, 4.0, 10)
+        t_c = time.perf_counter() - t0
+        line["optics"] = {"gpu_ms": round(t_g * 1e3, 3), "cpu_ms": round(t_c * 1e3, 3),
+                          "identical": bool(np.array_equal(f().ordering(), o[0]))}
+        line["config"] = "c1 data (10000 x 16, 6 Gaussians), eps=4.0, min_points=10; range on the GPU, walk on the host"
+    elif args.workload == "kmeans":
+        import ctypes
+
+        import torch
+        rng = np.random.default_rng(8)
+        n, k, d = 1_000_000, 100, 64
+        pts = rng.standard_normal((n, d)).astype(np.float32)
+        cen = pts[rng.choice(n, k, replace=False)].copy()
+        dev = torch.device("cuda:0")
+        P = torch.from_numpy(pts).to(dev)
+        C = torch.from_numpy(cen).to(dev)
+        out = torch.empty((k, n), dtype=torch.float32, device=dev)
+        st = torch.cuda.current_stream().cuda_stream
+        lib = _lib.load()
+        call = lambda: _lib.check(lib.fuel_amd_pairwise_sqdist_device(  # noqa: E731
+            C.data_ptr(), k, d, P.data_ptr(), n, d, d, _lib.F32, out.data_ptr(), n, ctypes.c_void_p(st)))
+        call()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            call()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        terms = float(n) * k * d
+        S = 20_000
+        t0 = time.perf_counter()
+        ref = oracle.kmeans_pairwise(cen, pts[:S], threads=threads)
+        t_c = time.perf_counter() - t0
+        same = bool(out[:, :S].cpu().numpy().tobytes() == ref.tobytes())
+        line.update({"config": "k-means distance matrix: 100 centres x 1,000,000 points, d=64 (device buffers)",
+                     "gpu_ms": round(ms, 3), "identical_first_20000_columns": same,
+                     "roofline": {"bound": "valu", "achieved": round(2 * terms / (ms * 1e-3) / 1e12, 3),
+                                  "peak": round(VALU_F32_PEAK / 1e12, 1), "unit": "TFLOP/s",
+                                  "frac": round(2 * terms / (ms * 1e-3) / VALU_F32_PEAK, 4),
+                                  "note": "one sub + one fma per term (2 flop counted), non-packed FP32 FMA peak"},
+                     "output_bytes": 4 * n * k,
+                     "cpu_ms_extrapolated": round(t_c * n / S * 1e3, 1),
+                     "cpu_sample": f"oracle on {S} of the points, scaled"})
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -148,6 +279,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported",
                           "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
+    if args.workload != "c2":
+        secondary(args)
+        return
     import torch
 
     from fuel_amd import _lib, workloads
