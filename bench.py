@@ -47,10 +47,14 @@ def parse():
     p.add_argument("--cpu-sample", type=int, default=1000, help="queries timed on the CPU baseline (0: skip)")
     p.add_argument("--fp32-steps", type=int, default=2,
                    help="timed steps of the plain FP32 MFMA form run beside the mixed one (0: skip)")
-    p.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "dbscan", "kmeans"],
+    p.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c5", "dbscan", "kmeans"],
                    help="c2: the headline kNN line; the others print their own line (not the headline): "
-                        "c1 self-kNN 10k x 16 + pairwise 2000^2, c3 LOF 70k x 784 k=20, dbscan on c1, "
-                        "kmeans distance matrix 1M x 64 vs 100 centres")
+                        "c1 self-kNN 10k x 16 + pairwise 2000^2, c3 LOF 70k x 784 k=20, c5 serving 10M x 96 "
+                        "under Poisson arrivals, dbscan on c1, kmeans distance matrix 1M x 64 vs 100 centres")
+    p.add_argument("--serve-mode", default="fp32", choices=["bf16", "fp32", "exact"],
+                   help="candidate mode of the served index (c5)")
+    p.add_argument("--serve-seconds", type=float, default=4.0, help="seconds of arrivals per rate (c5)")
+    p.add_argument("--serve-rows", type=int, default=10_000_000, help="rows of the served index (c5)")
     p.add_argument("--host-path", action="store_true",
                    help="only time the host-pointer entry point (queries and results cross PCIe each call)")
     p.add_argument("--copy-probe", type=int, default=0,
@@ -156,6 +160,55 @@ def timed(fn, reps):
     return float(np.median(ts))
 
 
+def serve_run(srv, pool, lam, seconds, rng, k=10, mean_batch=64, cap=4096, keep=0):
+    """Open-loop arrivals: requests arrive as a Poisson process of rate
+    lam / mean_batch per second with geometric sizes (mean mean_batch, capped),
+    so queries arrive at lam per second on average.  Returns the server's
+    latency statistics over the run and the first `keep` answers."""
+    import queue
+    import threading
+    srv.stats(reset=True)
+    tickets = queue.Queue()
+    kept = []
+
+    def waiter():
+        while True:
+            item = tickets.get()
+            if item is None:
+                return
+            t, off, m = item
+            idx, dist, _, _ = srv.wait(t)
+            if len(kept) < keep:
+                kept.append((off, m, idx, dist))
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t_start = time.perf_counter()
+    t_next, off, nq, nreq = t_start, 0, 0, 0
+    while True:
+        t_next += rng.exponential(mean_batch / lam)
+        if t_next - t_start > seconds:
+            break
+        m = int(min(cap, rng.geometric(1.0 / mean_batch)))
+        now = time.perf_counter()
+        if t_next > now:
+            time.sleep(t_next - now)
+        if off + m > pool.shape[0]:
+            off = 0
+        tickets.put((srv.submit(pool[off:off + m], k), off, m))
+        off += m
+        nq += m
+        nreq += 1
+    tickets.put(None)
+    th.join()
+    elapsed = time.perf_counter() - t_start
+    st = srv.stats(reset=True)
+    return {"offered_qps": lam, "achieved_qps": round(st["queries"] / elapsed, 1), "requests": nreq,
+            "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3), "max_ms": round(st["max_ms"], 3),
+            "launches": st["batches"], "mean_launch_queries": round(st["queries"] / max(1, st["batches"]), 1),
+            "max_launch_queries": st["max_batch_queries"], "gpu_busy_frac": round(st["busy_s"] / elapsed, 3)}, kept
+
+
 def secondary(args):
     """Lines for the other configurations and the widened rows: GPU time beside
     the oracle's CPU time on the same inputs, parity checked on the way."""
@@ -206,6 +259,47 @@ def secondary(args):
         Xs = X[:6000]
         g = _lib.Index(Xs, "euclidean").lof(20)
         line["lof_identical_6000_subset"] = bool(g.tobytes() == oracle.lof(Xs, 20, threads=threads).tobytes())
+    elif args.workload == "c5":
+        from fuel_amd import Server
+        t0 = time.time()
+        X, means = workloads.c5(n=args.serve_rows)
+        rng = np.random.default_rng(66)
+        P = 65536
+        pool = (means[rng.integers(0, means.shape[0], P)] + rng.normal(0.0, 1.0, (P, X.shape[1]))).astype(np.float32)
+        ix = _lib.Index(X, "euclidean", mode=args.serve_mode)
+        print(f"# c5 data + index in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+        rates, kept = [], []
+        with Server(ix, workers=2, max_batch=4096) as srv:
+            serve_run(srv, pool, 2000, 0.5, rng)  # warm-up (workspaces, operand copy)
+            for lam in (2000, 5000, 10000, 20000):
+                r, kp = serve_run(srv, pool, lam, args.serve_seconds, rng, keep=2)
+                rates.append(r)
+                kept += kp
+                print(f"# rate {lam}: {r}", file=sys.stderr, flush=True)
+            # saturation: 64 requests of 64 queries submitted at once
+            srv.stats(reset=True)
+            t1 = time.perf_counter()
+            ts = [srv.submit(pool[i * 64:(i + 1) * 64], 10) for i in range(64)]
+            for t in ts:
+                srv.wait(t)
+            sat = 64 * 64 / (time.perf_counter() - t1)
+            sst = srv.stats(reset=True)
+        # exactness of served answers: a sample of rows against the oracle
+        rows = [(off, min(m, 12), idx[:12], dist[:12]) for off, m, idx, dist in kept]
+        t0 = time.perf_counter()
+        same, nchk = True, 0
+        for off, m, idx, dist in rows:
+            oi, od = oracle.knn(X, pool[off:off + m], 10, "euclidean", "none", threads=threads)
+            same = same and bool(np.array_equal(oi, idx) and od.tobytes() == dist.tobytes())
+            nchk += m
+        t_c = time.perf_counter() - t0
+        line.update({"config": f"serving: resident index {X.shape[0]:,} x {X.shape[1]} f32 (50-component mixture), "
+                               "k=10, Poisson arrivals, geometric batch sizes (mean 64, cap 4096)",
+                     "mode": args.serve_mode, "workers": 2, "rates": rates,
+                     "saturated_qps": round(sat, 1), "saturated_p99_ms": round(sst["p99_ms"], 3),
+                     "identical_sample": same, "checked_queries": nchk,
+                     "cpu_baseline": {"value": round(nchk / t_c, 2), "unit": "queries/s", "cores": threads,
+                                      "kind": "port", "sample": f"{nchk} served queries re-answered by the oracle"}})
     elif args.workload == "dbscan":
         from fuel_amd.cluster import dbscan, optics
         X = workloads.c1()

@@ -26,6 +26,7 @@ SOURCES = [
     "engine.cpp",
     "engine_mfma.cpp",
     "cluster.cpp",
+    "serve.cpp",
     "scan_exact.hip",
     "scan_mfma.hip",
     "finalize_rerank.hip",

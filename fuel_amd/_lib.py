@@ -43,6 +43,8 @@ EXPORTED = [
     "fuel_amd_pairwise", "fuel_amd_lof", "fuel_amd_knn_outlier", "fuel_amd_last_stats",
     "fuel_amd_dbscan", "fuel_amd_optics", "fuel_amd_pairwise_sqdist", "fuel_amd_pairwise_sqdist_device",
     "fuel_amd_timing_enable", "fuel_amd_timing_read",
+    "fuel_amd_server_create", "fuel_amd_server_submit", "fuel_amd_server_wait", "fuel_amd_server_stats",
+    "fuel_amd_server_destroy",
 ]
 
 
@@ -60,6 +62,18 @@ class Stats(ctypes.Structure):
     _fields_ = [("queries", ctypes.c_int64), ("fallback_queries", ctypes.c_int64),
                 ("mode_used", ctypes.c_int32), ("splits", ctypes.c_int32), ("kp", ctypes.c_int32),
                 ("exact_queries", ctypes.c_int32), ("ms_total", ctypes.c_double)]
+
+
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("workers", ctypes.c_int32), ("reserved0", ctypes.c_int32), ("max_batch", ctypes.c_int64),
+                ("hbm_budget", ctypes.c_int64), ("reserved", ctypes.c_int64 * 5)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("requests", ctypes.c_int64), ("queries", ctypes.c_int64), ("batches", ctypes.c_int64),
+                ("max_batch_queries", ctypes.c_int64), ("failed", ctypes.c_int64), ("p50_ms", ctypes.c_double),
+                ("p99_ms", ctypes.c_double), ("max_ms", ctypes.c_double), ("mean_ms", ctypes.c_double),
+                ("busy_s", ctypes.c_double)]
 
 
 class FuelAmdError(RuntimeError):
@@ -132,6 +146,17 @@ def load():
     lib.fuel_amd_timing_read.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(I64),
                                          ctypes.c_char_p, I32]
     lib.fuel_amd_timing_read.restype = I32
+    lib.fuel_amd_server_create.argtypes = [P, ctypes.POINTER(ServerOpts), ctypes.POINTER(P)]
+    lib.fuel_amd_server_create.restype = I32
+    lib.fuel_amd_server_submit.argtypes = [P, P, I64, I32, ctypes.POINTER(ctypes.c_uint64)]
+    lib.fuel_amd_server_submit.restype = I32
+    lib.fuel_amd_server_wait.argtypes = [P, ctypes.c_uint64, P, P, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(I64)]
+    lib.fuel_amd_server_wait.restype = I32
+    lib.fuel_amd_server_stats.argtypes = [P, I32, ctypes.POINTER(ServeStats)]
+    lib.fuel_amd_server_stats.restype = I32
+    lib.fuel_amd_server_destroy.argtypes = [P]
+    lib.fuel_amd_server_destroy.restype = I32
     _lib = lib
     return lib
 

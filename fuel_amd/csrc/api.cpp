@@ -193,6 +193,7 @@ static int32_t index_create_common(const void* data, bool on_device, int64_t n, 
   ix->n = n;
   ix->d = d;
   ix->opts = o;
+  ix->ctx.timed = true;
   const size_t es = dtype == FUEL_AMD_F32 ? 4 : 8;
   // rows padded to a 32-byte multiple so the exact kernels can use 16-byte loads
   ix->ld = dtype == FUEL_AMD_F32 ? ((int64_t(d) + 7) & ~int64_t(7)) : ((int64_t(d) + 3) & ~int64_t(3));
@@ -292,7 +293,7 @@ int32_t fuel_amd_timing_read(fuel_amd_index* ix, double* kernel_ms, int64_t* lau
 
 int32_t fuel_amd_last_stats(const fuel_amd_index* ix, fuel_amd_stats* out) {
   if (!ix || !out) return fail(FUEL_AMD_ERR_INVALID, "NULL argument");
-  *out = ix->stats;
+  *out = ix->ctx.stats;
   return FUEL_AMD_OK;
 }
 
@@ -309,7 +310,7 @@ int32_t fuel_amd_knn_device(fuel_amd_index* ix, const void* d_query, int64_t m, 
   int32_t rc = check_device(ix->device);
   if (rc) return rc;
   hipStream_t st = stream ? (hipStream_t)stream : ix->stream;
-  return knn_run(ix, d_query, m, ldq, k, exclude_self, d_out_idx, d_out_dist, st);
+  return knn_run(ix, &ix->ctx, d_query, m, ldq, k, exclude_self, d_out_idx, d_out_dist, st);
 }
 
 int32_t fuel_amd_knn(fuel_amd_index* ix, const void* query, int64_t m, int32_t k, int32_t exclude_self,
@@ -333,7 +334,7 @@ int32_t fuel_amd_knn(fuel_amd_index* ix, const void* query, int64_t m, int32_t k
     e = fa::upload_rows(ix->hq.p, ix->ld, query, m, ix->d, es, ix->upload, st);
     if (e != hipSuccess) return hip_fail(e, "query upload");
   }
-  rc = knn_run(ix, ix->hq.p, m, ix->ld, k, exclude_self, (int64_t*)ix->hout_idx.p, ix->hout_dist.p, st);
+  rc = knn_run(ix, &ix->ctx, ix->hq.p, m, ix->ld, k, exclude_self, (int64_t*)ix->hout_idx.p, ix->hout_dist.p, st);
   if (rc) return rc;
   e = hipMemcpyAsync(out_idx, ix->hout_idx.p, size_t(m) * k * 8, hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipMemcpyAsync(out_dist, ix->hout_dist.p, size_t(m) * k * es, hipMemcpyDeviceToHost, st);

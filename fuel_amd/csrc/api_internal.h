@@ -40,6 +40,35 @@ int choose_splits(int64_t qblocks, int64_t n, int cus, int64_t min_rows);
 hipError_t upload_rows(void* dst, int64_t ld, const void* host, int64_t rows, int d, size_t es, DevBuf& stage,
                        hipStream_t st);
 
+// Query-side workspaces of one caller of the kNN path.  The index owns one,
+// used by the synchronous entry points under the index mutex; each serving
+// worker owns another, so batches on different streams run concurrently
+// against one resident index (whose rows and operand copies are read-only).
+struct QueryCtx {
+  struct LevelWs {  // one per level of the certification chain
+    DevBuf Qop, Q32, qnorm, qint, slots, lcnts, fail_count, fail_list, fail_theta, gthr;
+  } lv[3];
+  DevBuf slots, lcnts;  // exact scan
+  fuel_amd_stats stats{};
+  bool timed = false;  // record kernel timing events (the index's own context)
+  void release() {
+    for (auto& w : lv) {
+      w.Qop.release();
+      w.Q32.release();
+      w.qnorm.release();
+      w.qint.release();
+      w.slots.release();
+      w.lcnts.release();
+      w.fail_count.release();
+      w.fail_list.release();
+      w.fail_theta.release();
+      w.gthr.release();
+    }
+    slots.release();
+    lcnts.release();
+  }
+};
+
 }  // namespace fa
 
 struct fuel_amd_index {
@@ -55,22 +84,19 @@ struct fuel_amd_index {
   fa::DevBuf X;                          // resident rows (n x ld)
   fa::DevBuf upload;                     // staging of host query rows whose pitch differs from ld
   fa::DevBuf hq, hout_idx, hout_dist;    // staging for host-pointer calls
-  fa::DevBuf slots, lcnts, fail_list;    // scan workspaces
   fa::DevBuf ws_a, ws_b, ws_c, ws_d, ws_e, ws_tmp;  // range / pairwise / outlier workspaces
   // mixed-precision candidate path: one operand copy per kind (0 bf16, 1 f32),
-  // prepared on first use, and one query workspace per fallback level
+  // prepared once (under prep_mu) on first use, read-only afterwards
   struct MfmaOperand {
-    bool ready = false;
+    std::atomic<bool> ready{false};
     bool usable = true;  // false when some squared norm reaches MFMA_NORM2_LIMIT
     int ks = 0;  // K steps
     int kp = 0;  // padded K (elements)
     fa::DevBuf Xop, hx, consts;  // operand copy, ||x||^2/2, bound constants + flags
   } mop[2];
+  std::mutex prep_mu;
   int64_t n_pad = 0;
-  struct QueryWs {
-    fa::DevBuf Qop, Q32, qnorm, qint, slots, lcnts, fail_count, fail_list, fail_theta, gthr;
-  } qws[3];  // one per level of the certification chain
-  fuel_amd_stats stats{};
+  fa::QueryCtx ctx;  // the synchronous entry points' query workspaces and stats
   // profiling: event pairs around the dominant kernel of each call
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;
@@ -81,9 +107,7 @@ struct fuel_amd_index {
     hq.release();
     hout_idx.release();
     hout_dist.release();
-    slots.release();
-    lcnts.release();
-    fail_list.release();
+    ctx.release();
     ws_a.release();
     ws_b.release();
     ws_c.release();
@@ -96,23 +120,17 @@ struct fuel_amd_index {
       m.consts.release();
       m.ready = false;
     }
-    for (auto& w : qws) {
-      w.Qop.release();
-      w.Q32.release();
-      w.qnorm.release();
-      w.qint.release();
-      w.slots.release();
-      w.lcnts.release();
-      w.fail_count.release();
-      w.fail_list.release();
-      w.gthr.release();
-    }
   }
 };
 
 namespace fa {
-int32_t knn_run(fuel_amd_index* ix, const void* d_query, int64_t m, int64_t ldq, int k, int exclude,
-                int64_t* d_out_idx, void* d_out_dist, hipStream_t st);
+// kNN of m device query rows on stream st with the workspaces of ctx (the
+// index's own unless a serving worker passes its own).
+int32_t knn_run(fuel_amd_index* ix, QueryCtx* ctx, const void* d_query, int64_t m, int64_t ldq, int k,
+                int exclude, int64_t* d_out_idx, void* d_out_dist, hipStream_t st);
+// Prepare the operand copy of the mode the index's options select (a no-op for
+// the exact mode); the serving workers call the kNN path only after this.
+int32_t prepare_mode(fuel_amd_index* ix, hipStream_t st);
 int32_t range_run(fuel_amd_index* ix, const void* query, int64_t m, double radius, int exclude_self,
                   fuel_amd_csr* out);
 int32_t knn_ties_run(fuel_amd_index* ix, const void* query, int64_t m, int k, fuel_amd_csr* out);
@@ -132,13 +150,14 @@ int32_t outlier_run(fuel_amd_index* ix, int k, int knn_outlier, void* out_scores
 // position in the certification chain (which also selects the query
 // workspace); d_theta = optional per-slot starting thresholds (key units), the
 // certified second pass of queries that failed a previous level.
-int32_t knn_mfma(fuel_amd_index* ix, const float* Q, int64_t m, int64_t ldq, int k, int exclude, int kind,
-                 int64_t* out_idx, float* out_dist, hipStream_t st, const int64_t* d_rows = nullptr,
+int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, int64_t ldq, int k, int exclude,
+                 int kind, int64_t* out_idx, float* out_dist, hipStream_t st, const int64_t* d_rows = nullptr,
                  int64_t qbase = 0, int level = 0, int kp = 0, const float* d_theta = nullptr);
 bool mfma_supported(const fuel_amd_index* ix, int kind, int k, int exclude);
 // Exact path on an explicit list of query rows (the certification fallback).
-int32_t knn_exact_rows(fuel_amd_index* ix, const float* Q, int64_t ldq, const int64_t* d_rows, int64_t nrows,
-                       int64_t qbase, int k, int exclude, int64_t* out_idx, float* out_dist, hipStream_t st);
+int32_t knn_exact_rows(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t ldq, const int64_t* d_rows,
+                       int64_t nrows, int64_t qbase, int k, int exclude, int64_t* out_idx, float* out_dist,
+                       hipStream_t st);
 // Record the start (begin=true) or end of a timed kernel on `st` (no-op unless enabled).
 void timing_mark(fuel_amd_index* ix, hipStream_t st, bool begin, const char* name);
 }  // namespace fa

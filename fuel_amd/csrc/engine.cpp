@@ -24,8 +24,8 @@ int exact_qw(int64_t nq) { return nq >= 64 ? 16 : (nq >= 8 ? 8 : 4); }
 // Exact kNN of m query slots.  With qrows, slot i is query row qrows[i] of Q
 // (a device array) and its result goes to output row qrows[i].
 template <typename F>
-int32_t knn_exact(fuel_amd_index* ix, const F* Q, int64_t m, int64_t ldq, int k, int mode, int64_t* out_idx,
-                  F* out_dist, hipStream_t st, const int64_t* qrows = nullptr, int64_t qbase = 0) {
+int32_t knn_exact(fuel_amd_index* ix, QueryCtx* ctx, const F* Q, int64_t m, int64_t ldq, int k, int mode,
+                  int64_t* out_idx, F* out_dist, hipStream_t st, const int64_t* qrows = nullptr, int64_t qbase = 0) {
   using T = typename Comp<F>::T;
   const int KK = k + (mode != EXCL_NONE ? 1 : 0);
   if (KK > EXACT_NS / 2)
@@ -39,8 +39,8 @@ int32_t knn_exact(fuel_amd_index* ix, const F* Q, int64_t m, int64_t ldq, int k,
     int nsplit = choose_splits((chunk + qw - 1) / qw, ix->n, cus, 512);
     const size_t per_q = size_t(nsplit) * EXACT_NS * sizeof(T);
     chunk = std::min<int64_t>(chunk, std::max<int64_t>(qw, int64_t(SLOT_BUDGET / per_q)));
-    hipError_t e = ix->slots.ensure(per_q * chunk);
-    if (e == hipSuccess) e = ix->lcnts.ensure(sizeof(int) * size_t(nsplit) * chunk);
+    hipError_t e = ctx->slots.ensure(per_q * chunk);
+    if (e == hipSuccess) e = ctx->lcnts.ensure(sizeof(int) * size_t(nsplit) * chunk);
     if (e != hipSuccess) return hip_fail(e, "slot workspace");
     ScanExactArgs<F> sa{};
     sa.X = ix->X.as<F>();
@@ -57,11 +57,11 @@ int32_t knn_exact(fuel_amd_index* ix, const F* Q, int64_t m, int64_t ldq, int k,
     rps = std::max<int64_t>(256, (rps + 255) & ~int64_t(255));
     sa.rows_per_split = rps;
     nsplit = (int)std::max<int64_t>(1, (ix->n + rps - 1) / rps);
-    sa.slots = ix->slots.as<T>();
-    sa.lcnts = ix->lcnts.as<int>();
-    timing_mark(ix, st, true, "scan_exact_kernel");
+    sa.slots = ctx->slots.as<T>();
+    sa.lcnts = ctx->lcnts.as<int>();
+    if (ctx->timed) timing_mark(ix, st, true, "scan_exact_kernel");
     e = launch_scan_exact<F>(sa, qw, nsplit, st);
-    timing_mark(ix, st, false, nullptr);
+    if (ctx->timed) timing_mark(ix, st, false, nullptr);
     if (e != hipSuccess) return hip_fail(e, "scan_exact launch");
     FinalizeArgs<F> fa_{};
     fa_.slots = sa.slots;
@@ -78,8 +78,8 @@ int32_t knn_exact(fuel_amd_index* ix, const F* Q, int64_t m, int64_t ldq, int k,
     fa_.out_dist = qrows ? out_dist : out_dist + done * k;
     e = launch_finalize_exact<F>(fa_, st);
     if (e != hipSuccess) return hip_fail(e, "finalize launch");
-    ix->stats.splits = nsplit;
-    ix->stats.kp = KK;
+    ctx->stats.splits = nsplit;
+    ctx->stats.kp = KK;
     done += chunk;
   }
   return FUEL_AMD_OK;
@@ -221,7 +221,7 @@ int32_t knn_ties_impl(fuel_amd_index* ix, const void* query, int64_t m, int k, f
   e = ix->hout_idx.ensure(size_t(m) * k * 8);
   if (e == hipSuccess) e = ix->hout_dist.ensure(size_t(m) * k * es);
   if (e != hipSuccess) return hip_fail(e, "kNN buffers");
-  int32_t rc = knn_run(ix, ix->hq.p, m, ix->ld, k, EXCL_NONE, ix->hout_idx.as<int64_t>(), ix->hout_dist.p, st);
+  int32_t rc = knn_run(ix, &ix->ctx, ix->hq.p, m, ix->ld, k, EXCL_NONE, ix->hout_idx.as<int64_t>(), ix->hout_dist.p, st);
   if (rc) return rc;
   // the k-th key of every query (+inf when fewer than k points exist: then all of them)
   return range_core<F>(ix, ix->hq.as<F>(), m, 0.0, ix->hout_dist.as<F>() + (k - 1), k, 0, out);
@@ -266,7 +266,7 @@ int32_t outlier_impl(fuel_amd_index* ix, int k, int knn_outlier, void* out_score
   if (e == hipSuccess) e = ix->ws_c.ensure(sizeof(F) * size_t(n) * 3);
   if (e != hipSuccess) return hip_fail(e, "outlier workspace");
   if (k_eff > 0) {
-    int32_t rc = knn_run(ix, ix->X.p, n, ix->ld, k_eff, EXCL_FOR_EACH_KNN, ix->ws_a.as<int64_t>(), ix->ws_b.p, st);
+    int32_t rc = knn_run(ix, &ix->ctx, ix->X.p, n, ix->ld, k_eff, EXCL_FOR_EACH_KNN, ix->ws_a.as<int64_t>(), ix->ws_b.p, st);
     if (rc) return rc;
   }
   LofArgs<F> la{};
@@ -286,34 +286,36 @@ int32_t outlier_impl(fuel_amd_index* ix, int k, int knn_outlier, void* out_score
 
 }  // namespace
 
-int32_t knn_exact_rows(fuel_amd_index* ix, const float* Q, int64_t ldq, const int64_t* d_rows, int64_t nrows,
-                       int64_t qbase, int k, int exclude, int64_t* out_idx, float* out_dist, hipStream_t st) {
-  return knn_exact<float>(ix, Q, nrows, ldq, k, exclude, out_idx, out_dist, st, d_rows, qbase);
+int32_t knn_exact_rows(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t ldq, const int64_t* d_rows,
+                       int64_t nrows, int64_t qbase, int k, int exclude, int64_t* out_idx, float* out_dist,
+                       hipStream_t st) {
+  return knn_exact<float>(ix, ctx, Q, nrows, ldq, k, exclude, out_idx, out_dist, st, d_rows, qbase);
 }
 
-int32_t knn_run(fuel_amd_index* ix, const void* d_query, int64_t m, int64_t ldq, int k, int exclude,
-                int64_t* d_out_idx, void* d_out_dist, hipStream_t st) {
+int32_t knn_run(fuel_amd_index* ix, QueryCtx* ctx, const void* d_query, int64_t m, int64_t ldq, int k,
+                int exclude, int64_t* d_out_idx, void* d_out_dist, hipStream_t st) {
   if (m == 0 || k == 0) return FUEL_AMD_OK;
   if (!d_query || !d_out_idx || !d_out_dist) return fail(FUEL_AMD_ERR_INVALID, "NULL buffer");
   auto t0 = std::chrono::steady_clock::now();
-  ix->stats = fuel_amd_stats{};
-  ix->stats.queries = m;
+  ctx->stats = fuel_amd_stats{};
+  ctx->stats.queries = m;
   int mode = ix->opts.mode;
   if (mode == FUEL_AMD_MODE_AUTO) mode = FUEL_AMD_MODE_BF16;
   int kind = mode == FUEL_AMD_MODE_BF16 ? 0 : (mode == FUEL_AMD_MODE_FP32 ? 1 : -1);
   if (kind >= 0 && !mfma_supported(ix, kind, k, exclude)) kind = -1;
   int32_t rc;
   if (kind >= 0) {
-    ix->stats.mode_used = kind == 0 ? FUEL_AMD_MODE_BF16 : FUEL_AMD_MODE_FP32;
-    rc = knn_mfma(ix, (const float*)d_query, m, ldq, k, exclude, kind, d_out_idx, (float*)d_out_dist, st);
+    ctx->stats.mode_used = kind == 0 ? FUEL_AMD_MODE_BF16 : FUEL_AMD_MODE_FP32;
+    rc = knn_mfma(ix, ctx, (const float*)d_query, m, ldq, k, exclude, kind, d_out_idx, (float*)d_out_dist, st);
   } else if (ix->dtype == FUEL_AMD_F32) {
-    ix->stats.mode_used = FUEL_AMD_MODE_EXACT;
-    rc = knn_exact<float>(ix, (const float*)d_query, m, ldq, k, exclude, d_out_idx, (float*)d_out_dist, st);
+    ctx->stats.mode_used = FUEL_AMD_MODE_EXACT;
+    rc = knn_exact<float>(ix, ctx, (const float*)d_query, m, ldq, k, exclude, d_out_idx, (float*)d_out_dist, st);
   } else {
-    ix->stats.mode_used = FUEL_AMD_MODE_EXACT;
-    rc = knn_exact<double>(ix, (const double*)d_query, m, ldq, k, exclude, d_out_idx, (double*)d_out_dist, st);
+    ctx->stats.mode_used = FUEL_AMD_MODE_EXACT;
+    rc = knn_exact<double>(ix, ctx, (const double*)d_query, m, ldq, k, exclude, d_out_idx, (double*)d_out_dist,
+                           st);
   }
-  ix->stats.ms_total =
+  ctx->stats.ms_total =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return rc;
 }

@@ -33,7 +33,9 @@ int kp_for(const fuel_amd_index* ix, int k, int exclude) {
 
 int32_t ensure_prepared(fuel_amd_index* ix, int kind, hipStream_t st) {
   auto& op = ix->mop[kind];
-  if (op.ready) return FUEL_AMD_OK;
+  if (op.ready.load(std::memory_order_acquire)) return FUEL_AMD_OK;
+  std::lock_guard<std::mutex> lock(ix->prep_mu);
+  if (op.ready.load(std::memory_order_relaxed)) return FUEL_AMD_OK;
   const int ks = mfma_ksteps_bucket(kind, ix->d);
   if (ks < 0) return fail(FUEL_AMD_ERR_UNSUPPORTED, "dimension too large for the MFMA candidate scan");
   const int kp = ks * (kind == 0 ? 16 : 8);
@@ -70,19 +72,21 @@ int32_t ensure_prepared(fuel_amd_index* ix, int kind, hipStream_t st) {
   double x2;
   std::memcpy(&x2, &x2bits, sizeof(x2));
   op.usable = x2 < MFMA_NORM2_LIMIT;
-  op.ready = true;
   op.ks = ks;
   op.kp = kp;
   ix->n_pad = n_pad;
+  op.ready.store(true, std::memory_order_release);
   return FUEL_AMD_OK;
 }
 
 // Number of DB splits: fill the resident workgroup slots with as little tail as
 // possible (the splits of a query share its threshold, so extra splits cost
-// only the per-split warm-up).
+// only the per-split warm-up and the rerank's merge of one more list).  Small
+// batches (serving) need many splits to fill the chip: one query block of 64
+// queries against 10M rows runs as ~500 splits of ~20k rows.
 int mfma_splits(int64_t nqb, int64_t n_pad, int cus, int64_t rows_tile, int wg_per_cu) {
   const int64_t slots = int64_t(cus) * wg_per_cu;
-  const int64_t max_s = std::max<int64_t>(1, std::min<int64_t>(128, n_pad / (rows_tile * 4)));
+  const int64_t max_s = std::max<int64_t>(1, std::min<int64_t>(1024, n_pad / (rows_tile * 4)));
   int best = 1;
   double best_eff = -1.0;
   for (int64_t s = 1; s <= max_s; ++s) {
@@ -107,18 +111,30 @@ bool mfma_supported(const fuel_amd_index* ix, int kind, int k, int exclude) {
   return KK <= KP_MAX;
 }
 
-int32_t knn_mfma(fuel_amd_index* ix, const float* Q, int64_t m, int64_t ldq, int k, int exclude, int kind,
-                 int64_t* out_idx, float* out_dist, hipStream_t st, const int64_t* d_rows, int64_t qbase,
+int32_t prepare_mode(fuel_amd_index* ix, hipStream_t st) {
+  int mode = ix->opts.mode;
+  if (mode == FUEL_AMD_MODE_AUTO) mode = FUEL_AMD_MODE_BF16;
+  if (mode == FUEL_AMD_MODE_EXACT || ix->dtype != FUEL_AMD_F32) return FUEL_AMD_OK;
+  const int kind = mode == FUEL_AMD_MODE_BF16 ? 0 : 1;
+  if (mfma_ksteps_bucket(kind, ix->d) < 0) return FUEL_AMD_OK;
+  int32_t rc = ensure_prepared(ix, kind, st);
+  // the certification chain continues bf16 with f32
+  if (rc == FUEL_AMD_OK && kind == 0 && mfma_ksteps_bucket(1, ix->d) >= 0) rc = ensure_prepared(ix, 1, st);
+  return rc;
+}
+
+int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, int64_t ldq, int k, int exclude,
+                 int kind, int64_t* out_idx, float* out_dist, hipStream_t st, const int64_t* d_rows, int64_t qbase,
                  int level, int kp, const float* d_theta) {
   int32_t rc = ensure_prepared(ix, kind, st);
   if (rc) return rc;
   auto& op = ix->mop[kind];
   if (!op.usable) {
     // norms beyond the scan's finite sentinels: every query takes the exact scan
-    ix->stats.exact_queries += m;
-    return knn_exact_rows(ix, Q, ldq, d_rows, m, qbase, k, exclude, out_idx, out_dist, st);
+    ctx->stats.exact_queries += m;
+    return knn_exact_rows(ix, ctx, Q, ldq, d_rows, m, qbase, k, exclude, out_idx, out_dist, st);
   }
-  auto& ws = ix->qws[level];
+  auto& ws = ctx->lv[level];
   const int KK = k + (exclude != EXCL_NONE ? 1 : 0);
   const int KP = kp > 0 ? std::max(kp, k + (exclude != EXCL_NONE ? 1 : 0)) : kp_for(ix, k, exclude);
   const int V = KP <= 64 ? 4 : 8;
@@ -132,7 +148,7 @@ int32_t knn_mfma(fuel_amd_index* ix, const float* Q, int64_t m, int64_t ldq, int
   const int cus = device_cu_count(ix->device);
   const size_t es = kind == 0 ? 2 : 4;
   const int64_t ld32 = ix->ld;
-  const bool timed = level == 0;
+  const bool timed = level == 0 && ctx->timed;
 
   int64_t done = 0;
   while (done < m) {
@@ -255,18 +271,19 @@ int32_t knn_mfma(fuel_amd_index* ix, const float* Q, int64_t m, int64_t ldq, int
       else if (level == 1 && kind == 0 && mfma_supported(ix, 1, k, exclude))
         next_kind = 1;
       if (next_kind >= 0) {
-        rc = knn_mfma(ix, Qb, nfail, ldq, k, exclude, next_kind, ra.out_idx, ra.out_dist, st, ra.fail_list,
+        rc = knn_mfma(ix, ctx, Qb, nfail, ldq, k, exclude, next_kind, ra.out_idx, ra.out_dist, st, ra.fail_list,
                       ra.qbase, level + 1, KP_MAX, ra.fail_theta);
       } else {
-        ix->stats.exact_queries += nfail;  // queries answered by the exact scan
-        rc = knn_exact_rows(ix, Qb, ldq, ra.fail_list, nfail, ra.qbase, k, exclude, ra.out_idx, ra.out_dist, st);
+        ctx->stats.exact_queries += nfail;  // queries answered by the exact scan
+        rc = knn_exact_rows(ix, ctx, Qb, ldq, ra.fail_list, nfail, ra.qbase, k, exclude, ra.out_idx, ra.out_dist,
+                            st);
       }
       if (rc) return rc;
     }
     if (level == 0) {
-      ix->stats.fallback_queries += nfail;
-      ix->stats.splits = nsplit;
-      ix->stats.kp = KP;
+      ctx->stats.fallback_queries += nfail;
+      ctx->stats.splits = nsplit;
+      ctx->stats.kp = KP;
     }
     done += chunk;
   }

@@ -164,36 +164,46 @@ __global__ __launch_bounds__(64) void rerank_kernel(RerankArgs a) {
   const int lane = __lane_id();
   const int KP = a.KP;
 
-  // 1. union of split lists -> KP smallest approximate keys
+  // 1. union of split lists -> KP smallest approximate keys.  Entries are
+  //    compacted into an LDS buffer; whenever it holds NS, it is sorted and cut
+  //    back to its KP smallest, and `cut` (the KP-th of those) drops every later
+  //    entry at or above it: such an entry cannot be among the union's KP
+  //    smallest (composites are distinct: each row belongs to one split).
+  __shared__ T buf[NS + 64];
   T x[V];
-#pragma unroll
-  for (int v = 0; v < V; ++v) x[v] = C::MAX;
+  T cut = C::MAX;
   int fill = 0;
   int64_t total = 0;
   for (int s = 0; s < a.nsplit; ++s) {
     const T* sl = a.slots + (size_t(s) * a.nq_pad + qs) * a.slot_ns;
     const int cnt = a.lcnts[size_t(s) * a.nq_pad + qs];
     total += cnt;
-    int off = 0;
-    while (off < cnt) {
-      const int room = NS - fill;
-      const int take = (cnt - off) < room ? (cnt - off) : room;
+    for (int off = 0; off < cnt; off += 64) {
+      const T val = off + lane < cnt ? sl[off + lane] : C::MAX;
+      const bool keep = val < cut;
+      const uint64_t mask = __ballot(keep);
+      if (keep) buf[fill + __popcll(mask & ((uint64_t(1) << lane) - 1))] = val;
+      fill += __popcll(mask);
+      if (fill >= NS) {
+        __syncthreads();
 #pragma unroll
-      for (int v = 0; v < V; ++v) {
-        const int e = v * 64 + lane;
-        if (e >= fill && e < fill + take) x[v] = sl[off + e - fill];
-      }
-      fill += take;
-      off += take;
-      if (fill == NS) {
+        for (int v = 0; v < V; ++v) x[v] = buf[v * 64 + lane];
+        const T over = lane < fill - NS ? buf[NS + lane] : C::MAX;
         wave_bitonic_sort<T, V>(x);
+        cut = wave_get<T, V>(x, KP - 1);
+        __syncthreads();
 #pragma unroll
         for (int v = 0; v < V; ++v)
-          if (v * 64 + lane >= KP) x[v] = C::MAX;
-        fill = KP;
+          if (v * 64 + lane < KP) buf[v * 64 + lane] = x[v];
+        if (lane < fill - NS) buf[KP + lane] = over;
+        fill = KP + (fill - NS);
+        __syncthreads();
       }
     }
   }
+  __syncthreads();
+#pragma unroll
+  for (int v = 0; v < V; ++v) x[v] = v * 64 + lane < fill ? buf[v * 64 + lane] : C::MAX;
   wave_bitonic_sort<T, V>(x);
   const int ncand = total < KP ? (int)total : KP;
   // Bound on the excluded rows: with a cut here, the KP-th kept key; otherwise

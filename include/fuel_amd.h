@@ -218,6 +218,55 @@ int32_t fuel_amd_timing_enable(fuel_amd_index* index, int32_t enable);
 int32_t fuel_amd_timing_read(fuel_amd_index* index, double* kernel_ms, int64_t* launches, char* name,
                              int32_t name_len);
 
+/* ---- serving ----------------------------------------------------------------
+ * A request loop over one resident index (no counterpart in fuel, whose
+ * SearchIndex.knn, python/search.rs:560-569, answers one batch per call).
+ * Requests are submitted from any thread and answered by worker threads, each
+ * with its own stream and query workspaces.  A free worker admits every
+ * request waiting at that moment (arrival order, same k) up to max_batch
+ * queries and while the query-side device memory of all in-flight batches
+ * stays within hbm_budget, so requests arriving while other batches run are
+ * coalesced into the next launch and overlap with them on the device
+ * (continuous batching).  Each query's answer is exactly what fuel_amd_knn
+ * (exclude_self 0) returns for it alone.  Destroy every server of an index
+ * before the index. */
+typedef struct fuel_amd_server fuel_amd_server;
+
+typedef struct fuel_amd_server_opts {
+  int32_t workers;    /* batches in flight at once, one stream each (0: 2) */
+  int32_t reserved0;
+  int64_t max_batch;  /* queries per launch (0: 4096) */
+  int64_t hbm_budget; /* bytes of query-side device memory over all in-flight batches (0: 4 GiB) */
+  int64_t reserved[5];
+} fuel_amd_server_opts;
+
+typedef struct fuel_amd_serve_stats {
+  int64_t requests;  /* completed since the last reset */
+  int64_t queries;
+  int64_t batches;   /* launches */
+  int64_t max_batch_queries;
+  int64_t failed;    /* requests answered with an error */
+  double p50_ms;     /* request latency, submit -> results ready on the host */
+  double p99_ms;
+  double max_ms;
+  double mean_ms;
+  double busy_s;     /* seconds with at least one batch in flight */
+} fuel_amd_serve_stats;
+
+int32_t fuel_amd_server_create(fuel_amd_index* index, const fuel_amd_server_opts* opts, fuel_amd_server** out);
+/* Copies m query rows (index dtype, row-major, d per row); never blocks on the
+ * GPU.  m must not exceed max_batch. */
+int32_t fuel_amd_server_submit(fuel_amd_server* server, const void* query, int64_t m, int32_t k, uint64_t* ticket);
+/* Blocks until the request is answered, writes int64 [m,k] indices and
+ * [m,k] distances (index dtype; -1 / +inf padding as fuel_amd_knn), and
+ * releases the ticket.  latency_ms and batch_queries (queries of the launch
+ * that answered it) may be NULL. */
+int32_t fuel_amd_server_wait(fuel_amd_server* server, uint64_t ticket, int64_t* out_idx, void* out_dist,
+                             double* latency_ms, int64_t* batch_queries);
+int32_t fuel_amd_server_stats(fuel_amd_server* server, int32_t reset, fuel_amd_serve_stats* out);
+/* Answers every submitted request, then stops the workers. */
+int32_t fuel_amd_server_destroy(fuel_amd_server* server);
+
 #ifdef __cplusplus
 }
 #endif

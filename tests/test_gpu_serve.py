@@ -1,0 +1,119 @@
+"""Serving loop (continuous batching): every request's answer is the same
+alone and in the mix, and equals the oracle's linear scan on the same rows;
+admission by batch size and HBM budget; requests from several threads."""
+import threading
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _mixture(n, d, comps, seed):
+    from fuel_amd import workloads
+    rng = np.random.default_rng(seed)
+    return workloads.gaussian_mixture(rng, n, d, comps, -10.0, 10.0)
+
+
+@pytest.fixture(scope="module")
+def served(gpu):
+    # clustered rows: the serving configuration's shape (50-component mixture,
+    # d = 96) at 120k rows, where the bf16 bound sends queries down the chain
+    X = _mixture(120_000, 96, 50, 61)
+    Q = _mixture(3000, 96, 50, 62)
+    return X, Q
+
+
+def _requests(Q, rng, ks=(10,)):
+    reqs, off = [], 0
+    while off < Q.shape[0]:
+        m = int(min(Q.shape[0] - off, rng.geometric(1.0 / 40)))
+        reqs.append((off, m, int(rng.choice(ks))))
+        off += m
+    return reqs
+
+
+@pytest.mark.parametrize("mode", ["bf16", "fp32", "exact"])
+def test_mix_equals_alone_and_oracle(gpu, oracle, served, mode):
+    from fuel_amd import Server
+    X, Q = served
+    ix = gpu.Index(X, "euclidean", mode=mode)
+    rng = np.random.default_rng(7)
+    reqs = _requests(Q, rng, ks=(10, 10, 10, 3))
+    results = {}
+    with Server(ix, workers=2, max_batch=512) as srv:
+        lock = threading.Lock()
+
+        def client(part):
+            tickets = [(r, srv.submit(Q[r[0]:r[0] + r[1]], r[2])) for r in part]
+            for r, t in tickets:
+                idx, dist, lat, bq = srv.wait(t)
+                assert lat >= 0.0 and r[1] <= bq <= 512
+                with lock:
+                    results[r] = (idx, dist)
+
+        threads = [threading.Thread(target=client, args=(reqs[i::3],)) for i in range(3)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        st = srv.stats()
+    assert len(results) == len(reqs)
+    assert st["requests"] == len(reqs) and st["failed"] == 0
+    assert st["queries"] == Q.shape[0] and st["max_batch_queries"] <= 512
+    assert st["batches"] < len(reqs)  # requests were coalesced
+    assert 0.0 < st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
+    # alone: each sampled request by itself through the synchronous entry point
+    for r in reqs[::7]:
+        ai, ad = ix.knn(Q[r[0]:r[0] + r[1]], r[2])
+        gi, gd = results[r]
+        assert np.array_equal(ai, gi) and ai.dtype == gi.dtype
+        assert ad.tobytes() == gd.tobytes()
+    # and against the oracle on a sample of the rows
+    for r in reqs[:: max(1, len(reqs) // 6)]:
+        q = Q[r[0]:r[0] + r[1]]
+        oi, od = oracle.knn(X, q, r[2], "euclidean", "none")
+        gi, gd = results[r]
+        assert np.array_equal(oi, gi) and od.tobytes() == gd.tobytes()
+
+
+def test_budget_and_edge_requests(gpu):
+    from fuel_amd import Server
+    from fuel_amd._lib import FuelAmdError
+    rng = np.random.default_rng(3)
+    X = rng.random((5000, 16), dtype=np.float32)
+    ix = gpu.Index(X, "sqeuclidean")
+    # a budget below the smallest batch's estimate rejects the request
+    with Server(ix, workers=1, max_batch=64, hbm_budget=1 << 20) as srv:
+        t = srv.submit(X[:4], 5)
+        with pytest.raises(FuelAmdError):
+            srv.wait(t)
+        with pytest.raises(ValueError):
+            srv.submit(X[:65], 5)  # above max_batch
+    with Server(ix, workers=2, max_batch=64) as srv:
+        t0 = srv.submit(X[:0], 5)
+        t1 = srv.submit(X[:3], 0)
+        i0, d0, _, _ = srv.wait(t0)
+        i1, d1, _, _ = srv.wait(t1)
+        assert i0.shape == (0, 5) and i1.shape == (3, 0)
+        with pytest.raises(FuelAmdError):
+            srv.wait(123456789)  # unknown ticket (checked before the C call)
+    # k beyond n pads with -1 / +inf like the synchronous call
+    small = gpu.Index(X[:8], "sqeuclidean")
+    with Server(small, workers=1) as srv:
+        gi, gd, _, _ = srv.wait(srv.submit(X[:2], 10))
+    ai, ad = small.knn(X[:2], 10)
+    assert np.array_equal(ai, gi) and ad.tobytes() == gd.tobytes() and gi[0, 9] == -1
+
+
+def test_unwaited_tickets_and_f64(gpu, oracle):
+    from fuel_amd import Server
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((3000, 12))
+    ix = gpu.Index(X, "euclidean")
+    srv = Server(ix, workers=2, max_batch=256)
+    tickets = [srv.submit(X[i:i + 17], 6) for i in range(0, 300, 17)]
+    gi, gd, _, _ = srv.wait(tickets[3])
+    oi, od = oracle.knn(X, X[51:68], 6, "euclidean", "none")
+    assert np.array_equal(gi, oi) and gd.tobytes() == od.tobytes()
+    srv.close()  # answers and releases the tickets never waited for
