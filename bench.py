@@ -47,10 +47,13 @@ def parse():
     p.add_argument("--cpu-sample", type=int, default=1000, help="queries timed on the CPU baseline (0: skip)")
     p.add_argument("--fp32-steps", type=int, default=2,
                    help="timed steps of the plain FP32 MFMA form run beside the mixed one (0: skip)")
-    p.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c5", "dbscan", "kmeans"],
+    p.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "c5", "dbscan", "kmeans"],
                    help="c2: the headline kNN line; the others print their own line (not the headline): "
-                        "c1 self-kNN 10k x 16 + pairwise 2000^2, c3 LOF 70k x 784 k=20, c5 serving 10M x 96 "
+                        "c1 self-kNN 10k x 16 + pairwise 2000^2, c3 LOF 70k x 784 k=20, c4 512 uneven self-kNN "
+                        "problems (batched), c5 serving 10M x 96 "
                         "under Poisson arrivals, dbscan on c1, kmeans distance matrix 1M x 64 vs 100 centres")
+    p.add_argument("--batch-mode", default="auto", choices=["auto", "bf16", "fp32", "exact"],
+                   help="configuration of the batched problems (c4): autotuned or fixed")
     p.add_argument("--serve-mode", default="fp32", choices=["bf16", "fp32", "exact"],
                    help="candidate mode of the served index (c5)")
     p.add_argument("--serve-seconds", type=float, default=4.0, help="seconds of arrivals per rate (c5)")
@@ -259,6 +262,58 @@ def secondary(args):
         Xs = X[:6000]
         g = _lib.Index(Xs, "euclidean").lof(20)
         line["lof_identical_6000_subset"] = bool(g.tobytes() == oracle.lof(Xs, 20, threads=threads).tobytes())
+    elif args.workload == "c4":
+        from fuel_amd import BatchKnn
+        t0 = time.time()
+        probs = workloads.c4()
+        print(f"# c4: {len(probs)} problems generated in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+        t0 = time.time()
+        b = BatchKnn(probs, 15, mode=args.batch_mode)
+        tune_s = time.time() - t0
+        print(f"# c4: planned (autotune) in {tune_s:.1f}s", file=sys.stderr, flush=True)
+        for _ in range(args.warmup):
+            b.run()
+        import torch
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.run()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / args.steps * 1e3
+        info = b.describe()
+        flop = float(sum(2.0 * X.shape[0] ** 2 * X.shape[1] for X in probs))
+        achieved = flop / (ms * 1e-3)
+        # exactness: the smallest problems in full against the oracle
+        small = sorted(range(len(probs)), key=lambda i: probs[i].shape[0])[:6]
+        res = b.results()
+        same = True
+        for i in small:
+            oi, od = oracle.knn(probs[i], probs[i], 15, "euclidean", "for_each_knn", threads=threads)
+            same = same and bool(np.array_equal(oi, res[i][0]) and od.tobytes() == res[i][1].tobytes())
+        # CPU baseline: the oracle's for_each_knn on two mid-size problems, rate in flop/s
+        mid = sorted(range(len(probs)), key=lambda i: probs[i].shape[0])[len(probs) // 2:len(probs) // 2 + 2]
+        t0 = time.perf_counter()
+        f_s = 0.0
+        for i in mid:
+            oracle.knn(probs[i], probs[i], 15, "euclidean", "for_each_knn", threads=threads)
+            f_s += 2.0 * probs[i].shape[0] ** 2 * probs[i].shape[1]
+        t_c = time.perf_counter() - t0
+        line.update({"config": "512 self-kNN problems, n log-uniform in [1e3, 2e5], d in {8,32,64,256}, "
+                               "3-20 Gaussian components (means U[-5,5]^d), k=15, for_each_knn exclusion",
+                     "rows": int(sum(X.shape[0] for X in probs)), "algorithmic_flop": flop,
+                     "ms_per_pass": round(ms, 2), "steps": args.steps, "warmup": args.warmup,
+                     "plan_seconds": round(tune_s, 1), "plan": info["groups"], "graph": info["graph"],
+                     "fallback_queries": info["last_fallback_queries"], "exact_queries": info["last_exact_queries"],
+                     "roofline": {"bound": "mfma", "achieved": round(achieved / 1e12, 2),
+                                  "peak": PEAKS["bf16"][0] / 1e12, "unit": "TFLOP/s",
+                                  "frac": round(achieved / PEAKS["bf16"][0], 4),
+                                  "note": "2 n^2 d per problem over the whole pass; bf16 dense peak"},
+                     "identical_smallest_6": same,
+                     "cpu_baseline": {"value": round(flop / (f_s / t_c) * 1e3, 1), "unit": "ms per pass (extrapolated)",
+                                      "cores": threads, "kind": "port",
+                                      "sample": f"oracle for_each_knn on 2 mid-size problems ({t_c:.1f}s), "
+                                                "scaled by 2 n^2 d"}})
+        b.close()
     elif args.workload == "c5":
         from fuel_amd import Server
         t0 = time.time()

@@ -11,7 +11,8 @@ from .search import SearchIndex, knn_search, pairwise_distances, range_search
 from .outlier import OutlierResult, k_nearest_neighbors_outlier, local_outlier_factor
 from .cluster import OpticsResult, dbscan, extract_xi_labels, optics
 from .serve import Server
+from .batch import BatchKnn
 
 __all__ = ["SearchIndex", "knn_search", "range_search", "pairwise_distances", "local_outlier_factor",
            "k_nearest_neighbors_outlier", "OutlierResult", "dbscan", "optics", "OpticsResult",
-           "extract_xi_labels", "pairwise_sqdist", "Server"]
+           "extract_xi_labels", "pairwise_sqdist", "Server", "BatchKnn"]

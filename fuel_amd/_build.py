@@ -27,6 +27,7 @@ SOURCES = [
     "engine_mfma.cpp",
     "cluster.cpp",
     "serve.cpp",
+    "batch.cpp",
     "scan_exact.hip",
     "scan_mfma.hip",
     "finalize_rerank.hip",

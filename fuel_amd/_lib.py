@@ -45,6 +45,7 @@ EXPORTED = [
     "fuel_amd_timing_enable", "fuel_amd_timing_read",
     "fuel_amd_server_create", "fuel_amd_server_submit", "fuel_amd_server_wait", "fuel_amd_server_stats",
     "fuel_amd_server_destroy",
+    "fuel_amd_batch_create", "fuel_amd_batch_run", "fuel_amd_batch_describe", "fuel_amd_batch_destroy",
 ]
 
 
@@ -74,6 +75,16 @@ class ServeStats(ctypes.Structure):
                 ("max_batch_queries", ctypes.c_int64), ("failed", ctypes.c_int64), ("p50_ms", ctypes.c_double),
                 ("p99_ms", ctypes.c_double), ("max_ms", ctypes.c_double), ("mean_ms", ctypes.c_double),
                 ("busy_s", ctypes.c_double)]
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("n", ctypes.c_int64), ("ld", ctypes.c_int64), ("d", ctypes.c_int32),
+                ("k", ctypes.c_int32), ("out_idx", ctypes.c_void_p), ("out_dist", ctypes.c_void_p)]
+
+
+class BatchOpts(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int32), ("widen", ctypes.c_int32), ("device", ctypes.c_int32),
+                ("no_graph", ctypes.c_int32), ("reserved", ctypes.c_int64 * 4)]
 
 
 class FuelAmdError(RuntimeError):
@@ -157,6 +168,15 @@ def load():
     lib.fuel_amd_server_stats.restype = I32
     lib.fuel_amd_server_destroy.argtypes = [P]
     lib.fuel_amd_server_destroy.restype = I32
+    lib.fuel_amd_batch_create.argtypes = [ctypes.POINTER(Problem), I32, I32, I32, ctypes.POINTER(BatchOpts),
+                                          ctypes.POINTER(P)]
+    lib.fuel_amd_batch_create.restype = I32
+    lib.fuel_amd_batch_run.argtypes = [P]
+    lib.fuel_amd_batch_run.restype = I32
+    lib.fuel_amd_batch_describe.argtypes = [P, ctypes.c_char_p, I32]
+    lib.fuel_amd_batch_describe.restype = I32
+    lib.fuel_amd_batch_destroy.argtypes = [P]
+    lib.fuel_amd_batch_destroy.restype = I32
     _lib = lib
     return lib
 

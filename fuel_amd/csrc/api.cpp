@@ -38,9 +38,10 @@ int32_t hip_fail(hipError_t e, const char* what) {
 
 DevBuf::~DevBuf() { release(); }
 void DevBuf::release() {
-  if (p) hipFree(p);
+  if (p && owned) hipFree(p);
   p = nullptr;
   bytes = 0;
+  owned = true;
 }
 hipError_t DevBuf::ensure(size_t want) {
   if (want <= bytes && p) return hipSuccess;

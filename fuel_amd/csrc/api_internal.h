@@ -15,12 +15,19 @@ namespace fa {
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool owned = true;  // false: p is borrowed from the caller and never freed here
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf();
   void release();
   hipError_t ensure(size_t want);
+  void borrow(const void* ptr, size_t nbytes) {
+    release();
+    p = const_cast<void*>(ptr);
+    bytes = nbytes;
+    owned = false;
+  }
   template <typename T>
   T* as() const {
     return reinterpret_cast<T*>(p);

@@ -1,0 +1,643 @@
+// Batched self-kNN over many independent problems of uneven size and
+// dimension (each one the search loop of fuel's for_each_knn,
+// src/outlier/common.rs:12-49, over its own dataset).
+//
+// Plan: problems are grouped by dimension.  Every group runs with one
+// configuration (MFMA kind, candidate widening, split size), chosen by the
+// autotuner or fixed by the caller.  A group's problems keep their own operand
+// copies and workspaces (carved from one device pool that the groups reuse in
+// turn) and share one persistent MFMA scan launch whose workgroups take
+// (problem, query block, DB split) items from a counter, so the small problems
+// fill the gaps the large ones leave.  One pass =
+//   per group: per problem {bound constants reset, operand copy, query
+//   preparation, certification counter reset}, persistent scan, per problem
+//   exact rerank + certificate;
+// captured once into a hipGraph (fill kernels, never memset nodes) and
+// replayed.  Then the certification counters come back in one copy, and the
+// queries that did not certify take the chain of the single-index path (same
+// kind from the certified threshold, f32, exact) through a shell index that
+// borrows the problem's rows.
+//
+// The autotuner times every configuration (bf16 / f32 MFMA x widening 2, 4, 8
+// x split size 1x, 4x the persistent slots) on a sample of each group's
+// problems, without the graph, and keeps the fastest.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "api_internal.h"
+#include "kernels.h"
+
+namespace fa {
+namespace {
+
+constexpr int KP_LIMIT = 128;  // widest kept list of the MFMA path
+using Clock = std::chrono::steady_clock;
+
+struct Cfg {
+  int kind = 0;  // 0 bf16 MFMA, 1 f32 MFMA, -1 exact scan
+  int widen = 4;
+  int fill = 1;  // work items per persistent workgroup slot, / 4
+};
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct ProbPlan {
+  int bucket = -1;  // -1: outside the MFMA groups
+  bool exact = false;
+  int kind = 0, ks = 0, Kp = 0, KP = 0, KK = 0;
+  int64_t n_pad = 0, nqb = 0, nq_pad = 0, rps = 0;
+  int nsplit = 0;
+  size_t o_xop = 0, o_hx = 0, o_consts = 0, o_qop = 0, o_q32 = 0, o_qn = 0, o_qint = 0, o_gthr = 0, o_slots = 0,
+         o_lcnts = 0;      // level-0 pool (shared by the groups in turn)
+  size_t o_fl = 0, o_ft = 0;  // kept pool: certification failures of every problem
+};
+
+struct Bucket {
+  int d = 0;
+  Cfg cfg;
+  int ks = 0, V = 4, grid = 0;
+  std::vector<int> members;  // largest first
+  int64_t nitems = 0;
+  size_t pool_bytes = 0;
+  DevBuf probs_dev, starts_dev;
+};
+
+}  // namespace
+}  // namespace fa
+
+struct fuel_amd_batch {
+  int device = 0, metric = 0, exclude = 0;
+  bool use_graph = true;
+  std::vector<fuel_amd_problem> probs;
+  std::vector<char> force_exact;  // norms beyond the MFMA scan's sentinels
+  std::vector<fa::ProbPlan> plan;
+  std::deque<fa::Bucket> buckets;  // in place: a group owns device buffers
+  hipStream_t stream = nullptr;
+  fa::DevBuf pool, keep, fail_counts, counter;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  fuel_amd_index shell;  // borrows one problem's rows at a time (certification chain, exact path)
+  std::vector<int> h_fail;
+  double last_ms = 0.0;
+  int64_t last_fallback = 0, last_exact = 0, runs = 0;
+  std::string tuning;  // JSON array of the autotuner's measurements
+  ~fuel_amd_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipStreamDestroy(stream);
+    }
+    shell.stream = nullptr;
+  }
+};
+
+namespace fa {
+namespace {
+
+template <typename T>
+T* at(const DevBuf& b, size_t off) {
+  return reinterpret_cast<T*>(static_cast<char*>(b.p) + off);
+}
+
+int32_t plan_batch(fuel_amd_batch* b, const std::map<int, Cfg>& cfg_by_d) {
+  const int cus = device_cu_count(b->device);
+  const int np = (int)b->probs.size();
+  b->plan.assign(np, ProbPlan{});
+  b->buckets.clear();
+  std::map<int, std::vector<int>> by_d;
+  for (int i = 0; i < np; ++i) {
+    const auto& pr = b->probs[i];
+    auto& pp = b->plan[i];
+    if (pr.n == 0 || pr.k == 0) continue;
+    pp.KK = pr.k + (b->exclude != EXCL_NONE ? 1 : 0);
+    auto it = cfg_by_d.find(pr.d);
+    const Cfg c = it != cfg_by_d.end() ? it->second : Cfg{};
+    if (c.kind < 0 || b->force_exact[i] || pp.KK > KP_LIMIT || mfma_ksteps_bucket(c.kind, pr.d) < 0 || pr.d == 0) {
+      pp.exact = true;
+      continue;
+    }
+    by_d[pr.d].push_back(i);
+  }
+  size_t keep_off = 0;
+  for (int i = 0; i < np; ++i) {
+    const auto& pr = b->probs[i];
+    b->plan[i].o_fl = keep_off;
+    keep_off += align256(size_t(pr.n) * 8);
+    b->plan[i].o_ft = keep_off;
+    keep_off += align256(size_t(pr.n) * 4);
+  }
+  hipError_t e = b->keep.ensure(std::max<size_t>(keep_off, 256));
+  if (e == hipSuccess) e = b->fail_counts.ensure(sizeof(int) * std::max(1, np));
+  if (e == hipSuccess) e = b->counter.ensure(256);
+  if (e != hipSuccess) return hip_fail(e, "batch workspaces");
+  size_t pool_max = 0;
+  for (auto& kv : by_d) {
+    const int bucket_id = (int)b->buckets.size();
+    b->buckets.emplace_back();
+    Bucket& bk = b->buckets.back();
+    bk.d = kv.first;
+    bk.cfg = cfg_by_d.at(kv.first);
+    const int kind = bk.cfg.kind;
+    bk.ks = mfma_ksteps_bucket(kind, bk.d);
+    bk.members = kv.second;
+    std::sort(bk.members.begin(), bk.members.end(),
+              [&](int x, int y) { return b->probs[x].n > b->probs[y].n || (b->probs[x].n == b->probs[y].n && x < y); });
+    const int qpb = mfma_queries_per_block(kind, bk.ks);
+    const int rows_tile = mfma_rows_per_tile(kind, bk.ks);
+    const int64_t slots = int64_t(cus) * mfma_workgroups_per_cu(kind, bk.ks);
+    const int Kp = bk.ks * (kind == 0 ? 16 : 8);
+    const size_t es = kind == 0 ? 2 : 4;
+    double work = 0.0;
+    bk.V = 4;
+    for (int i : bk.members) {
+      auto& pp = b->plan[i];
+      const auto& pr = b->probs[i];
+      pp.bucket = bucket_id;
+      pp.kind = kind;
+      pp.ks = bk.ks;
+      pp.Kp = Kp;
+      pp.KP = std::max(pp.KK, std::min(KP_LIMIT, bk.cfg.widen * pr.k));
+      if (pp.KP > 64) bk.V = 8;
+      pp.n_pad = std::max<int64_t>(256, (pr.n + 255) & ~int64_t(255));
+      pp.nqb = (pr.n + qpb - 1) / qpb;
+      pp.nq_pad = pp.nqb * qpb;
+      work += double(pp.nqb) * double(pp.n_pad);
+    }
+    const int NS = 64 * bk.V;
+    // split size: about 4 * fill items per persistent workgroup over the group
+    const double target = std::max<double>(4.0 * rows_tile, work / (4.0 * bk.cfg.fill * double(slots)));
+    size_t off = 0;
+    std::vector<int64_t> starts{0};
+    for (int i : bk.members) {
+      auto& pp = b->plan[i];
+      const auto& pr = b->probs[i];
+      int64_t ns = std::max<int64_t>(1, std::min<int64_t>(1024, (int64_t)std::ceil(double(pp.n_pad) / target)));
+      int64_t rps = (pp.n_pad + ns - 1) / ns;
+      rps = (rps + rows_tile - 1) / rows_tile * rows_tile;
+      pp.rps = rps;
+      pp.nsplit = (int)((pp.n_pad + rps - 1) / rps);
+      pp.o_xop = off;
+      off += align256(size_t(pp.n_pad) * Kp * es);
+      pp.o_hx = off;
+      off += align256(size_t(pp.n_pad) * 4);
+      pp.o_consts = off;
+      off += 256;
+      pp.o_qop = off;
+      off += align256(size_t(pp.nq_pad) * Kp * es);
+      pp.o_q32 = off;
+      off += align256(size_t(pr.n) * pr.ld * 4);
+      pp.o_qn = off;
+      off += align256(size_t(pr.n) * 3 * 8);
+      pp.o_qint = off;
+      off += align256(size_t(pr.n) * 4);
+      pp.o_gthr = off;
+      off += align256(size_t(pp.nq_pad) * 4);
+      pp.o_slots = off;
+      off += align256(size_t(pp.nsplit) * pp.nq_pad * NS * 8);
+      pp.o_lcnts = off;
+      off += align256(size_t(pp.nsplit) * pp.nq_pad * 4);
+      starts.push_back(starts.back() + pp.nqb * pp.nsplit);
+    }
+    bk.nitems = starts.back();
+    bk.pool_bytes = off;
+    bk.grid = (int)slots;
+    pool_max = std::max(pool_max, off);
+    Bucket& B = bk;
+    e = B.starts_dev.ensure(sizeof(int64_t) * starts.size());
+    if (e == hipSuccess) e = B.probs_dev.ensure(sizeof(ScanMfmaArgs) * B.members.size());
+    if (e != hipSuccess) return hip_fail(e, "batch tables");
+    e = hipMemcpyAsync(B.starts_dev.p, starts.data(), sizeof(int64_t) * starts.size(), hipMemcpyHostToDevice,
+                       b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return hip_fail(e, "batch tables");
+  }
+  e = b->pool.ensure(std::max<size_t>(pool_max, 256));
+  if (e != hipSuccess) return hip_fail(e, "batch pool");
+  // argument tables of the persistent scans
+  for (auto& B : b->buckets) {
+    std::vector<ScanMfmaArgs> tab;
+    for (int i : B.members) {
+      const auto& pp = b->plan[i];
+      ScanMfmaArgs sa{};
+      sa.Xop = at<void>(b->pool, pp.o_xop);
+      sa.hx = at<float>(b->pool, pp.o_hx);
+      sa.Kp = pp.Kp;
+      sa.n_pad = pp.n_pad;
+      sa.Qop = at<void>(b->pool, pp.o_qop);
+      sa.nq_pad = pp.nq_pad;
+      sa.nqb = (int)pp.nqb;
+      sa.rows_per_split = pp.rps;
+      sa.KP = pp.KP;
+      sa.Csub = (64 * B.V - pp.KP) / 2;
+      sa.slots = at<uint64_t>(b->pool, pp.o_slots);
+      sa.lcnts = at<int>(b->pool, pp.o_lcnts);
+      sa.gthr = at<uint32_t>(b->pool, pp.o_gthr);
+      tab.push_back(sa);
+    }
+    e = hipMemcpyAsync(B.probs_dev.p, tab.data(), sizeof(ScanMfmaArgs) * tab.size(), hipMemcpyHostToDevice,
+                       b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return hip_fail(e, "batch tables");
+  }
+  b->h_fail.assign(np, 0);
+  return FUEL_AMD_OK;
+}
+
+PrepDbArgs prep_db_args(fuel_amd_batch* b, int i) {
+  const auto& pp = b->plan[i];
+  const auto& pr = b->probs[i];
+  PrepDbArgs pa{};
+  pa.X = static_cast<const float*>(pr.data);
+  pa.ldx = pr.ld;
+  pa.n = pr.n;
+  pa.n_pad = pp.n_pad;
+  pa.d = pr.d;
+  pa.Kp = pp.Kp;
+  pa.bf16 = pp.kind == 0;
+  pa.Xop = at<void>(b->pool, pp.o_xop);
+  pa.hx = at<float>(b->pool, pp.o_hx);
+  pa.consts = at<unsigned long long>(b->pool, pp.o_consts);
+  pa.flags = at<int>(b->pool, pp.o_consts + 64);
+  return pa;
+}
+
+// Level 0 of one group on stream st (no host synchronisation: capturable).
+hipError_t enqueue_bucket(fuel_amd_batch* b, const Bucket& B, hipStream_t st) {
+  hipError_t e = hipSuccess;
+  for (int i : B.members) {
+    const auto& pp = b->plan[i];
+    const auto& pr = b->probs[i];
+    const PrepDbArgs pa = prep_db_args(b, i);
+    e = launch_fill_u32(reinterpret_cast<uint32_t*>(pa.consts), 16, 0u, st);
+    if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(pa.flags), 2, 1u, st);
+    if (e == hipSuccess) e = launch_prep_db(pa, st);
+    PrepQArgs qa{};
+    qa.Q = static_cast<const float*>(pr.data);
+    qa.ldq = pr.ld;
+    qa.m = pr.n;
+    qa.m_pad = pp.nq_pad;
+    qa.d = pr.d;
+    qa.Kp = pp.Kp;
+    qa.bf16 = pp.kind == 0;
+    qa.Qop = at<void>(b->pool, pp.o_qop);
+    qa.Q32 = at<float>(b->pool, pp.o_q32);
+    qa.ld32 = pr.ld;
+    qa.qn2 = at<double>(b->pool, pp.o_qn);
+    qa.eq2 = qa.qn2 + pr.n;
+    qa.qt2 = qa.eq2 + pr.n;
+    qa.qint = at<int>(b->pool, pp.o_qint);
+    qa.gthr = at<uint32_t>(b->pool, pp.o_gthr);
+    if (e == hipSuccess) e = launch_prep_queries(qa, st);
+    if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(b->fail_counts.as<int>() + i), 1, 0u, st);
+    if (e != hipSuccess) return e;
+  }
+  e = launch_fill_u32(b->counter.as<uint32_t>(), 1, 0u, st);
+  ScanBatchArgs sb{};
+  sb.probs = B.probs_dev.as<ScanMfmaArgs>();
+  sb.item_start = B.starts_dev.as<int64_t>();
+  sb.nprob = (int)B.members.size();
+  sb.nitems = B.nitems;
+  sb.counter = b->counter.as<unsigned int>();
+  if (e == hipSuccess) e = launch_scan_mfma_batch(sb, B.cfg.kind, B.ks, B.V, B.grid, st);
+  for (int i : B.members) {
+    if (e != hipSuccess) return e;
+    const auto& pp = b->plan[i];
+    const auto& pr = b->probs[i];
+    RerankArgs ra{};
+    ra.slots = at<uint64_t>(b->pool, pp.o_slots);
+    ra.lcnts = at<int>(b->pool, pp.o_lcnts);
+    ra.nsplit = pp.nsplit;
+    ra.nq = pr.n;
+    ra.nq_pad = pp.nq_pad;
+    ra.slot_ns = 64 * B.V;
+    ra.KP = pp.KP;
+    ra.X = static_cast<const float*>(pr.data);
+    ra.ldx = pr.ld;
+    ra.Q32 = at<float>(b->pool, pp.o_q32);
+    ra.ld32 = pr.ld;
+    ra.d = pr.d;
+    ra.metric = b->metric;
+    ra.bf16 = pp.kind == 0;
+    ra.qn2 = at<double>(b->pool, pp.o_qn);
+    ra.eq2 = ra.qn2 + pr.n;
+    ra.qt2 = ra.eq2 + pr.n;
+    ra.qint = at<int>(b->pool, pp.o_qint);
+    ra.consts = at<unsigned long long>(b->pool, pp.o_consts);
+    ra.flags = at<int>(b->pool, pp.o_consts + 64);
+    ra.KK = pp.KK;
+    ra.k = pr.k;
+    ra.mode = b->exclude;
+    ra.qbase = 0;
+    ra.rows = nullptr;
+    ra.out_idx = pr.out_idx;
+    ra.out_dist = static_cast<float*>(pr.out_dist);
+    ra.fail_count = b->fail_counts.as<int>() + i;
+    ra.fail_list = at<int64_t>(b->keep, pp.o_fl);
+    ra.gthr = at<uint32_t>(b->pool, pp.o_gthr);
+    ra.fail_theta = at<float>(b->keep, pp.o_ft);
+    e = launch_rerank(ra, B.V, st);
+  }
+  return e;
+}
+
+hipError_t enqueue_all(fuel_amd_batch* b, hipStream_t st) {
+  for (const auto& B : b->buckets) {
+    hipError_t e = enqueue_bucket(b, B, st);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+void point_shell(fuel_amd_batch* b, int i, int mode) {
+  auto& s = b->shell;
+  const auto& pr = b->probs[i];
+  s.device = b->device;
+  s.dtype = FUEL_AMD_F32;
+  s.metric = b->metric;
+  s.n = pr.n;
+  s.d = pr.d;
+  s.ld = pr.ld;
+  s.opts.mode = mode;
+  s.opts.widen = 4;
+  s.opts.device = b->device;
+  s.X.borrow(pr.data, size_t(pr.n) * pr.ld * 4);
+  s.mop[0].ready.store(false);
+  s.mop[1].ready.store(false);
+  s.stream = b->stream;
+}
+
+// Norms beyond the MFMA scan's finite sentinels: such problems take the exact
+// scan (found once, at planning, like the single-index path's operand check).
+int32_t drop_unusable(fuel_amd_batch* b, bool& changed) {
+  changed = false;
+  std::vector<unsigned long long> x2(b->probs.size(), 0);
+  for (const auto& B : b->buckets) {
+    for (int i : B.members) {
+      const PrepDbArgs pa = prep_db_args(b, i);
+      hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t*>(pa.consts), 16, 0u, b->stream);
+      if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(pa.flags), 2, 1u, b->stream);
+      if (e == hipSuccess) e = launch_prep_db(pa, b->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(&x2[i], pa.consts, sizeof(unsigned long long), hipMemcpyDeviceToHost, b->stream);
+      if (e != hipSuccess) return hip_fail(e, "batch operand check");
+    }
+    hipError_t e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return hip_fail(e, "batch operand check");
+  }
+  for (const auto& B : b->buckets)
+    for (int i : B.members) {
+      double v;
+      std::memcpy(&v, &x2[i], sizeof(v));
+      if (!(v < MFMA_NORM2_LIMIT)) {
+        b->force_exact[i] = 1;
+        changed = true;
+      }
+    }
+  return FUEL_AMD_OK;
+}
+
+int32_t finish(fuel_amd_batch* b) {
+  const int np = (int)b->probs.size();
+  hipError_t e = hipSuccess;
+  if (!b->buckets.empty()) {
+    e = hipMemcpyAsync(b->h_fail.data(), b->fail_counts.p, sizeof(int) * np, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return hip_fail(e, "certification counts");
+  }
+  int64_t fallback = 0, exact = 0;
+  for (int i = 0; i < np; ++i) {
+    const auto& pp = b->plan[i];
+    const auto& pr = b->probs[i];
+    if (cancelled()) return fail(FUEL_AMD_ERR_INTERRUPTED, "interrupted");
+    if (pp.bucket >= 0 && b->h_fail[i] > 0) {
+      fallback += b->h_fail[i];
+      point_shell(b, i, pp.kind == 0 ? FUEL_AMD_MODE_BF16 : FUEL_AMD_MODE_FP32);
+      const int64_t before = b->shell.ctx.stats.exact_queries;
+      int32_t rc = knn_mfma(&b->shell, &b->shell.ctx, static_cast<const float*>(pr.data), b->h_fail[i], pr.ld,
+                            pr.k, b->exclude, pp.kind, pr.out_idx, static_cast<float*>(pr.out_dist), b->stream,
+                            at<int64_t>(b->keep, pp.o_fl), 0, 1, KP_LIMIT, at<float>(b->keep, pp.o_ft));
+      if (rc) return rc;
+      exact += b->shell.ctx.stats.exact_queries - before;
+    } else if (pp.exact) {
+      point_shell(b, i, FUEL_AMD_MODE_EXACT);
+      int32_t rc = knn_run(&b->shell, &b->shell.ctx, pr.data, pr.n, pr.ld, pr.k, b->exclude, pr.out_idx, pr.out_dist,
+                           b->stream);
+      if (rc) return rc;
+      exact += pr.n;
+    }
+  }
+  e = hipStreamSynchronize(b->stream);
+  if (e != hipSuccess) return hip_fail(e, "batch fallback");
+  b->last_fallback = fallback;
+  b->last_exact = exact;
+  return FUEL_AMD_OK;
+}
+
+int32_t run_once(fuel_amd_batch* b) {
+  const auto t0 = Clock::now();
+  hipError_t e;
+  if (b->use_graph && b->exec)
+    e = hipGraphLaunch(b->exec, b->stream);
+  else
+    e = enqueue_all(b, b->stream);
+  if (e != hipSuccess) return hip_fail(e, "batch launch");
+  int32_t rc = finish(b);
+  b->last_ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+  b->runs += 1;
+  return rc;
+}
+
+int32_t make_batch(const fuel_amd_problem* probs, const std::vector<int>& which, int metric, int exclude,
+                   int device, std::map<int, Cfg> cfg_by_d, bool use_graph, std::unique_ptr<fuel_amd_batch>& out) {
+  auto b = std::make_unique<fuel_amd_batch>();
+  b->device = device;
+  b->metric = metric;
+  b->exclude = exclude;
+  b->use_graph = use_graph;
+  for (int i : which) b->probs.push_back(probs[i]);
+  b->force_exact.assign(b->probs.size(), 0);
+  hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return hip_fail(e, "batch stream");
+  b->shell.stream = b->stream;
+  int32_t rc = plan_batch(b.get(), cfg_by_d);
+  if (rc) return rc;
+  bool changed = false;
+  rc = drop_unusable(b.get(), changed);
+  if (rc) return rc;
+  if (changed) {
+    rc = plan_batch(b.get(), cfg_by_d);  // the flagged problems leave their groups
+    if (rc) return rc;
+  }
+  if (use_graph && !b->buckets.empty()) {
+    e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(e, "graph capture");
+    hipError_t le = enqueue_all(b.get(), b->stream);
+    e = hipStreamEndCapture(b->stream, &b->graph);
+    if (le != hipSuccess) return hip_fail(le, "graph capture");
+    if (e != hipSuccess) return hip_fail(e, "graph capture");
+    e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) return hip_fail(e, "graph instantiate");
+  }
+  out = std::move(b);
+  return FUEL_AMD_OK;
+}
+
+const char* kind_name(int kind) { return kind == 0 ? "bf16" : (kind == 1 ? "fp32" : "exact"); }
+
+// Times every configuration on a sample of each dimension's problems.
+int32_t autotune(const fuel_amd_problem* probs, int count, int metric, int exclude, int device,
+                 std::map<int, Cfg>& best, std::string& log) {
+  std::map<int, std::vector<int>> by_d;
+  for (int i = 0; i < count; ++i)
+    if (probs[i].n > 0 && probs[i].k > 0) by_d[probs[i].d].push_back(i);
+  log = "[";
+  bool first = true;
+  for (auto& kv : by_d) {
+    const int d = kv.first;
+    auto members = kv.second;
+    std::sort(members.begin(), members.end(), [&](int x, int y) { return probs[x].n > probs[y].n; });
+    std::vector<int> sample;
+    const size_t step = std::max<size_t>(1, members.size() / 12);
+    for (size_t j = 0; j < members.size(); j += step) sample.push_back(members[j]);
+    double best_ms = 1e300;
+    Cfg best_cfg{-1, 4, 1};
+    for (int kind = 0; kind <= 1; ++kind) {
+      if (mfma_ksteps_bucket(kind, d) < 0) continue;
+      for (int widen : {2, 4, 8}) {
+        for (int fill : {1, 4}) {
+          Cfg c{kind, widen, fill};
+          std::unique_ptr<fuel_amd_batch> sb;
+          int32_t rc = make_batch(probs, sample, metric, exclude, device, {{d, c}}, false, sb);
+          if (rc) return rc;
+          rc = run_once(sb.get());  // warm-up: first-touch allocations of the fallback path
+          if (rc) return rc;
+          rc = run_once(sb.get());
+          if (rc) return rc;
+          const double ms = sb->last_ms;
+          char buf[256];
+          std::snprintf(buf, sizeof(buf),
+                        "%s{\"d\": %d, \"mode\": \"%s\", \"widen\": %d, \"fill\": %d, \"sample\": %zu, \"ms\": %.3f}",
+                        first ? "" : ", ", d, kind_name(kind), widen, fill, sample.size(), ms);
+          log += buf;
+          first = false;
+          if (ms < best_ms) {
+            best_ms = ms;
+            best_cfg = c;
+          }
+        }
+      }
+    }
+    best[d] = best_cfg;
+  }
+  log += "]";
+  return FUEL_AMD_OK;
+}
+
+}  // namespace
+}  // namespace fa
+
+using namespace fa;
+
+extern "C" {
+
+int32_t fuel_amd_batch_create(const fuel_amd_problem* problems, int32_t count, int32_t metric, int32_t exclude_self,
+                              const fuel_amd_batch_opts* opts, fuel_amd_batch** out) {
+  if (!out) return fail(FUEL_AMD_ERR_INVALID, "out must not be NULL");
+  *out = nullptr;
+  if (count < 0 || (count > 0 && !problems)) return fail(FUEL_AMD_ERR_INVALID, "bad problem list");
+  if (metric != FUEL_AMD_EUCLIDEAN && metric != FUEL_AMD_SQEUCLIDEAN)
+    return fail(FUEL_AMD_ERR_UNSUPPORTED, "only the 'euclidean' and 'sqeuclidean' distances run on this engine");
+  if (exclude_self < 0 || exclude_self > 2) return fail(FUEL_AMD_ERR_INVALID, "bad exclude_self");
+  fuel_amd_batch_opts o{};
+  if (opts) o = *opts;
+  if (o.mode < FUEL_AMD_MODE_AUTO || o.mode > FUEL_AMD_MODE_BF16) return fail(FUEL_AMD_ERR_INVALID, "bad mode");
+  for (int i = 0; i < count; ++i) {
+    const auto& p = problems[i];
+    if (p.n < 0 || p.d < 0 || p.k < 0) return fail(FUEL_AMD_ERR_INVALID, "problem sizes must be non-negative");
+    if (p.n > 0xFFFFFFFFll - 1) return fail(FUEL_AMD_ERR_UNSUPPORTED, "more than 2^32-2 rows");
+    if (p.n > 0 && p.k > 0) {
+      if (!p.data || !p.out_idx || !p.out_dist) return fail(FUEL_AMD_ERR_INVALID, "NULL problem buffer");
+      if (p.ld < std::max(p.d, 1) || p.ld % 8 != 0) return fail(FUEL_AMD_ERR_INVALID, "ld must be >= d and a multiple of 8");
+      if (reinterpret_cast<uintptr_t>(p.data) % 32 != 0) return fail(FUEL_AMD_ERR_INVALID, "rows must be 32-byte aligned");
+      if (p.k + (exclude_self ? 1 : 0) > 255) return fail(FUEL_AMD_ERR_UNSUPPORTED, "k above 254 is not supported");
+    }
+  }
+  int32_t rc = check_device(o.device);
+  if (rc) return rc;
+  std::map<int, Cfg> cfg;
+  std::string log = "[]";
+  if (o.mode == FUEL_AMD_MODE_AUTO) {
+    rc = autotune(problems, count, metric, exclude_self, o.device, cfg, log);
+    if (rc) return rc;
+  } else {
+    const int kind = o.mode == FUEL_AMD_MODE_BF16 ? 0 : (o.mode == FUEL_AMD_MODE_FP32 ? 1 : -1);
+    for (int i = 0; i < count; ++i) cfg[problems[i].d] = Cfg{kind, o.widen > 0 ? o.widen : 4, 1};
+  }
+  std::vector<int> all(count);
+  for (int i = 0; i < count; ++i) all[i] = i;
+  std::unique_ptr<fuel_amd_batch> b;
+  rc = make_batch(problems, all, metric, exclude_self, o.device, cfg, o.no_graph == 0, b);
+  if (rc) return rc;
+  b->tuning = log;
+  *out = b.release();
+  return FUEL_AMD_OK;
+}
+
+int32_t fuel_amd_batch_run(fuel_amd_batch* b) {
+  if (!b) return fail(FUEL_AMD_ERR_INVALID, "batch must not be NULL");
+  int32_t rc = check_device(b->device);
+  if (rc) return rc;
+  return run_once(b);
+}
+
+int32_t fuel_amd_batch_describe(fuel_amd_batch* b, char* text, int32_t len) {
+  if (!b || !text || len <= 0) return fail(FUEL_AMD_ERR_INVALID, "NULL argument");
+  std::string s = "{\"graph\": ";
+  s += (b->use_graph && b->exec) ? "true" : "false";
+  s += ", \"groups\": [";
+  for (size_t g = 0; g < b->buckets.size(); ++g) {
+    const auto& B = b->buckets[g];
+    int64_t rows = 0;
+    for (int i : B.members) rows += b->probs[i].n;
+    char buf[320];
+    std::snprintf(buf, sizeof(buf),
+                  "%s{\"d\": %d, \"problems\": %zu, \"rows\": %lld, \"mode\": \"%s\", \"widen\": %d, \"fill\": %d, "
+                  "\"k_steps\": %d, \"work_items\": %lld, \"workgroups\": %d}",
+                  g ? ", " : "", B.d, B.members.size(), (long long)rows, kind_name(B.cfg.kind), B.cfg.widen,
+                  B.cfg.fill, B.ks, (long long)B.nitems, B.grid);
+    s += buf;
+  }
+  int64_t nexact = 0;
+  for (const auto& pp : b->plan) nexact += pp.exact ? 1 : 0;
+  char tail[256];
+  std::snprintf(tail, sizeof(tail),
+                "], \"exact_problems\": %lld, \"runs\": %lld, \"last_ms\": %.3f, \"last_fallback_queries\": %lld, "
+                "\"last_exact_queries\": %lld, \"tuning\": ",
+                (long long)nexact, (long long)b->runs, b->last_ms, (long long)b->last_fallback,
+                (long long)b->last_exact);
+  s += tail;
+  s += b->tuning;
+  s += "}";
+  std::strncpy(text, s.c_str(), (size_t)len - 1);
+  text[len - 1] = 0;
+  return s.size() < (size_t)len ? FUEL_AMD_OK : fail(FUEL_AMD_ERR_INVALID, "text buffer too small");
+}
+
+int32_t fuel_amd_batch_destroy(fuel_amd_batch* b) {
+  if (!b) return FUEL_AMD_OK;
+  (void)hipSetDevice(b->device);
+  delete b;
+  return FUEL_AMD_OK;
+}
+
+}  // extern "C"

@@ -273,12 +273,30 @@ struct RerankArgs {
                          // every row that can still enter the query's answer lies
 };
 
+// A kernel that stores v into count u32s (unlike hipMemsetAsync, safe to
+// record into a captured graph).
+hipError_t launch_fill_u32(uint32_t* p, int64_t count, uint32_t v, hipStream_t st);
+
+// Persistent form of the MFMA scan over many problems of one shape (same kind,
+// K steps and slot width): work item i of problem p (i in [0, nqb * nsplit))
+// is query block i % nqb of split i / nqb; workgroups take items from a shared
+// counter until none are left.
+struct ScanBatchArgs {
+  const ScanMfmaArgs* probs;   // device array, one per problem
+  const int64_t* item_start;   // nprob + 1 running totals of the problems' items
+  int nprob;
+  int64_t nitems;
+  unsigned int* counter;       // zero before the launch
+};
+
 hipError_t launch_prep_db(const PrepDbArgs& a, hipStream_t st);
 hipError_t launch_prep_queries(const PrepQArgs& a, hipStream_t st);
 // kind 0: bf16 MFMA, 1: f32 MFMA.  Returns hipErrorInvalidValue for an
 // unsupported K step count.
 hipError_t launch_scan_mfma(const ScanMfmaArgs& a, int kind, int ksteps, int V, int nsplit, hipStream_t st);
 hipError_t launch_rerank(const RerankArgs& a, int V, hipStream_t st);
+// grid = the number of persistent workgroups
+hipError_t launch_scan_mfma_batch(const ScanBatchArgs& b, int kind, int ksteps, int V, int grid, hipStream_t st);
 // Supported K step counts of the MFMA scan (bf16: 16 dims per step; f32: 8 dims
 // per step group); the padded K is steps * 16 (bf16) or steps * 8 (f32).
 int mfma_ksteps_bucket(int kind, int d);

@@ -136,6 +136,17 @@ hipError_t launch_prep_db(const PrepDbArgs& a, hipStream_t st) {
   return hipGetLastError();
 }
 
+__global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t* p, int64_t count, uint32_t v) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i < count) p[i] = v;
+}
+
+hipError_t launch_fill_u32(uint32_t* p, int64_t count, uint32_t v, hipStream_t st) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, p, count, v);
+  return hipGetLastError();
+}
+
 hipError_t launch_prep_queries(const PrepQArgs& a, hipStream_t st) {
   if (a.m_pad == 0) return hipSuccess;
   hipLaunchKernelGGL(prep_query_kernel, dim3((unsigned)((a.m_pad + 3) / 4)), dim3(256), 0, st, a);

@@ -178,7 +178,7 @@ __device__ __forceinline__ int merge_select(uint64_t* __restrict__ slot, int KP,
   } while (0)
 
 template <int KIND, int KS, int A, int R, int V>
-__device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a) {
+__device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int qb, const int split) {
   using C = Comp<float>;
   constexpr int NS = 64 * V;
   constexpr int QPW = 32 * A;
@@ -199,10 +199,6 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a) {
   const int half = lane >> 5;
   const int col = lane & 31;
   const int wave = tid >> 6;
-  // Split-major order: all query blocks of split 0 start first, so that the
-  // later splits of a query begin with the thresholds the earlier ones reached.
-  const int qb = (int)(blockIdx.x % (unsigned)a.nqb);
-  const int split = (int)(blockIdx.x / (unsigned)a.nqb);
   const int64_t q0 = int64_t(qb) * QPB + wave * QPW;
   const int64_t r0 = int64_t(split) * a.rows_per_split;
   int64_t r1 = r0 + a.rows_per_split;
@@ -358,13 +354,50 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a) {
 // Two entry points with different register budgets: shapes whose working set
 // fits 256 registers run two waves per SIMD (one wave's MFMAs overlap the other's
 // selection epilogue); wider shapes keep one wave per SIMD and no spills.
+// Split-major order: all query blocks of split 0 start first, so that the
+// later splits of a query begin with the thresholds the earlier ones reached.
 template <int KIND, int KS, int A, int R, int V>
 __global__ __launch_bounds__(256, 2) void scan_mfma_kernel(ScanMfmaArgs a) {
-  scan_mfma_body<KIND, KS, A, R, V>(a);
+  scan_mfma_body<KIND, KS, A, R, V>(a, (int)(blockIdx.x % (unsigned)a.nqb), (int)(blockIdx.x / (unsigned)a.nqb));
 }
 template <int KIND, int KS, int A, int R, int V>
 __global__ __launch_bounds__(256) void scan_mfma_kernel_wide(ScanMfmaArgs a) {
-  scan_mfma_body<KIND, KS, A, R, V>(a);
+  scan_mfma_body<KIND, KS, A, R, V>(a, (int)(blockIdx.x % (unsigned)a.nqb), (int)(blockIdx.x / (unsigned)a.nqb));
+}
+
+// Persistent workgroups over the work items of many problems.  One lane takes
+// the next item from the counter and publishes it through LDS; the barriers
+// around the exchange also separate one item's LDS tiles from the next one's
+// (the body ends with every tile read and every copy into LDS completed).  Every
+// workgroup leaves once the counter passes the last item.
+template <int KIND, int KS, int A, int R, int V>
+__device__ __forceinline__ void scan_mfma_batch_body(const ScanBatchArgs& b) {
+  __shared__ int64_t item_s;
+  for (;;) {
+    if (threadIdx.x == 0) item_s = (int64_t)atomicAdd(b.counter, 1u);
+    __syncthreads();
+    const int64_t item = item_s;
+    __syncthreads();
+    if (item >= b.nitems) return;
+    int lo = 0, hi = b.nprob - 1;  // last problem whose first item is <= item
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (b.item_start[mid] <= item) lo = mid;
+      else hi = mid - 1;
+    }
+    const int p = __builtin_amdgcn_readfirstlane(lo);
+    const ScanMfmaArgs a = b.probs[p];
+    const int64_t local = item - b.item_start[p];
+    scan_mfma_body<KIND, KS, A, R, V>(a, (int)(local % a.nqb), (int)(local / a.nqb));
+  }
+}
+template <int KIND, int KS, int A, int R, int V>
+__global__ __launch_bounds__(256, 2) void scan_mfma_batch_kernel(ScanBatchArgs b) {
+  scan_mfma_batch_body<KIND, KS, A, R, V>(b);
+}
+template <int KIND, int KS, int A, int R, int V>
+__global__ __launch_bounds__(256) void scan_mfma_batch_kernel_wide(ScanBatchArgs b) {
+  scan_mfma_batch_body<KIND, KS, A, R, V>(b);
 }
 
 // K step buckets: bf16 covers 16 coordinates per step, f32 covers 8.  Up to 16
@@ -407,6 +440,53 @@ static hipError_t launch_ks(const ScanMfmaArgs& a, int V, int nsplit, hipStream_
       hipLaunchKernelGGL((scan_mfma_kernel_wide<KIND, KS, A, R, 8>), grid, block, 0, st, a);
   }
   return hipGetLastError();
+}
+
+template <int KIND, int KS>
+static hipError_t launch_batch_ks(const ScanBatchArgs& b, int V, int grid, hipStream_t st) {
+  constexpr int A = KS <= 16 ? 2 : 1;
+  constexpr int R = KS <= 16 ? 2 : 1;
+  constexpr bool two_waves = KIND == 0 ? KS <= 12 : KS <= 8;
+  if constexpr (two_waves) {
+    if (V == 4)
+      hipLaunchKernelGGL((scan_mfma_batch_kernel<KIND, KS, A, R, 4>), dim3(grid), dim3(256), 0, st, b);
+    else
+      hipLaunchKernelGGL((scan_mfma_batch_kernel<KIND, KS, A, R, 8>), dim3(grid), dim3(256), 0, st, b);
+  } else {
+    if (V == 4)
+      hipLaunchKernelGGL((scan_mfma_batch_kernel_wide<KIND, KS, A, R, 4>), dim3(grid), dim3(256), 0, st, b);
+    else
+      hipLaunchKernelGGL((scan_mfma_batch_kernel_wide<KIND, KS, A, R, 8>), dim3(grid), dim3(256), 0, st, b);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_scan_mfma_batch(const ScanBatchArgs& b, int kind, int ks, int V, int grid, hipStream_t st) {
+  if (V != 4 && V != 8) return hipErrorInvalidValue;
+  if (b.nitems <= 0) return hipSuccess;
+  if (kind == 0) {
+    switch (ks) {
+#define KS_CASE(K) \
+  case K:          \
+    return launch_batch_ks<0, K>(b, V, grid, st);
+      KS_CASE(1) KS_CASE(2) KS_CASE(3) KS_CASE(4) KS_CASE(5) KS_CASE(6) KS_CASE(7) KS_CASE(8) KS_CASE(9)
+      KS_CASE(10) KS_CASE(12) KS_CASE(14) KS_CASE(16) KS_CASE(20) KS_CASE(24) KS_CASE(28) KS_CASE(32)
+      KS_CASE(40) KS_CASE(48) KS_CASE(56) KS_CASE(64)
+#undef KS_CASE
+      default:
+        return hipErrorInvalidValue;
+    }
+  }
+  switch (ks) {
+#define KS_CASE(K) \
+  case K:          \
+    return launch_batch_ks<1, K>(b, V, grid, st);
+    KS_CASE(1) KS_CASE(2) KS_CASE(3) KS_CASE(4) KS_CASE(6) KS_CASE(8) KS_CASE(12) KS_CASE(16) KS_CASE(20)
+    KS_CASE(24) KS_CASE(32)
+#undef KS_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_scan_mfma(const ScanMfmaArgs& a, int kind, int ks, int V, int nsplit, hipStream_t st) {

@@ -267,6 +267,50 @@ int32_t fuel_amd_server_stats(fuel_amd_server* server, int32_t reset, fuel_amd_s
 /* Answers every submitted request, then stops the workers. */
 int32_t fuel_amd_server_destroy(fuel_amd_server* server);
 
+/* ---- batched problems ---------------------------------------------------------
+ * Many independent self-kNN problems of uneven size and dimension, each the
+ * search loop of fuel's for_each_knn over its own dataset
+ * (src/outlier/common.rs:12-49; with FUEL_AMD_EXCLUDE_NONE, SearchIndex.knn of
+ * the data against itself, python/search.rs:560-569).  fuel has no batched
+ * form: it runs one dataset per call.  Problems of one dimension share one
+ * persistent MFMA scan launch fed by a work queue over all their (query block,
+ * DB split) items; the operand and query preparation, that scan and the exact
+ * rerank of every problem are captured once into a hipGraph and replayed by
+ * each run, after which the few queries whose bound did not certify take the
+ * certification chain.  Results are identical to fuel_amd_knn on each problem.
+ * f32 only.  The caller keeps every problem's rows and outputs alive and
+ * unchanged in shape while the batch exists (their contents may change). */
+typedef struct fuel_amd_batch fuel_amd_batch;
+
+typedef struct fuel_amd_problem {
+  const void* data; /* device rows: n x d f32, row stride ld elements (ld % 8 == 0, 32-byte aligned) */
+  int64_t n;
+  int64_t ld;
+  int32_t d;
+  int32_t k;
+  int64_t* out_idx; /* device, n x k (-1 padded) */
+  void* out_dist;   /* device, n x k f32 (+inf padded) */
+} fuel_amd_problem;
+
+typedef struct fuel_amd_batch_opts {
+  int32_t mode;     /* FUEL_AMD_MODE_AUTO: the autotuner picks mode, widening and split size per
+                       dimension by timing them on a sample of that dimension's problems;
+                       FUEL_AMD_MODE_BF16 / _FP32 / _EXACT: fixed for every problem */
+  int32_t widen;    /* candidate widening with a fixed mode (0: 4) */
+  int32_t device;
+  int32_t no_graph; /* 1: launch directly instead of replaying the captured graph */
+  int64_t reserved[4];
+} fuel_amd_batch_opts;
+
+int32_t fuel_amd_batch_create(const fuel_amd_problem* problems, int32_t count, int32_t metric, int32_t exclude_self,
+                              const fuel_amd_batch_opts* opts, fuel_amd_batch** out);
+/* One pass over every problem; synchronous. */
+int32_t fuel_amd_batch_run(fuel_amd_batch* batch);
+/* The plan (per dimension: mode, widening, split size, work items, tuning times)
+ * and the last run's statistics as JSON text (truncated to len - 1 bytes). */
+int32_t fuel_amd_batch_describe(fuel_amd_batch* batch, char* text, int32_t len);
+int32_t fuel_amd_batch_destroy(fuel_amd_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,112 @@
+"""Batched self-kNN over uneven problems (persistent scan per dimension, the
+level-0 pass replayed from a captured hipGraph, autotuned configuration):
+every problem's answer equals the oracle's for_each_knn on that dataset, on
+every replay, with eager work between replays."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _problems(seed, count, n_lo, n_hi, dims):
+    from fuel_amd import workloads
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(round(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi)))))
+        d = int(rng.choice(dims))
+        out.append(workloads.gaussian_mixture(rng, n, d, int(rng.integers(3, 21)), -5.0, 5.0))
+    return out
+
+
+@pytest.fixture(scope="module")
+def problems(gpu):
+    return _problems(51, 18, 200, 5000, [8, 24, 32, 64, 256])
+
+
+def _check(oracle, probs, results, k, mode="for_each_knn", metric="euclidean"):
+    for X, (gi, gd) in zip(probs, results):
+        oi, od = oracle.knn(X, X, k, metric, mode)
+        assert np.array_equal(gi, oi), X.shape
+        assert gd.tobytes() == od.tobytes(), X.shape
+
+
+@pytest.mark.parametrize("mode", ["bf16", "fp32", "exact"])
+def test_batch_fixed_modes(gpu, oracle, problems, mode):
+    from fuel_amd import BatchKnn
+    b = BatchKnn(problems, 15, mode=mode)
+    b.run()
+    _check(oracle, problems, b.results(), 15)
+    info = b.describe()
+    assert info["graph"] == (mode != "exact")
+    if mode != "exact":
+        assert {g["d"] for g in info["groups"]} == {X.shape[1] for X in problems}
+        assert all(g["mode"] == mode for g in info["groups"])
+    b.close()
+
+
+def test_graph_replays_with_eager_work_between(gpu, oracle, problems):
+    from fuel_amd import BatchKnn
+    eager = BatchKnn(problems, 15, mode="bf16", graph=False)
+    eager.run()
+    ref = eager.results()
+    _check(oracle, problems, ref, 15)
+    g = BatchKnn(problems, 15, mode="bf16", graph=True)
+    other = gpu.Index(problems[0], "euclidean")
+    for rep in range(4):
+        g.run()
+        got = g.results()
+        for (a, b_), (c, d) in zip(ref, got):
+            assert np.array_equal(a, c) and b_.tobytes() == d.tobytes(), rep
+        other.knn(problems[0][:50], 7)  # eager work on the same device between replays
+    g.close()
+    eager.close()
+
+
+def test_autotuned_plan(gpu, oracle):
+    from fuel_amd import BatchKnn
+    probs = _problems(52, 10, 300, 3000, [8, 64, 256])
+    b = BatchKnn(probs, 15, mode="auto")
+    b.run()
+    _check(oracle, probs, b.results(), 15)
+    info = b.describe()
+    dims = {X.shape[1] for X in probs}
+    assert {g["d"] for g in info["groups"]} == dims
+    tuned = {(t["d"], t["mode"], t["widen"], t["fill"]) for t in info["tuning"]}
+    for dd in dims:  # every configuration was timed for every dimension
+        assert {(m, w, f) for (d2, m, w, f) in tuned if d2 == dd} == {
+            (m, w, f) for m in ("bf16", "fp32") for w in (2, 4, 8) for f in (1, 4)}
+    for g in info["groups"]:  # the chosen one is the fastest timed
+        times = [t for t in info["tuning"] if t["d"] == g["d"]]
+        best = min(times, key=lambda t: t["ms"])
+        assert (g["mode"], g["widen"], g["fill"]) == (best["mode"], best["widen"], best["fill"])
+    b.close()
+
+
+@pytest.mark.parametrize("exclude,omode", [(0, "none"), (2, "python_linear")])
+def test_other_exclusion_and_metric(gpu, oracle, problems, exclude, omode):
+    from fuel_amd import BatchKnn
+    probs = problems[:6]
+    b = BatchKnn(probs, 9, metric="sqeuclidean", exclude=exclude, mode="bf16")
+    b.run()
+    _check(oracle, probs, b.results(), 9, omode, "sqeuclidean")
+    b.close()
+
+
+def test_edge_problems(gpu, oracle):
+    from fuel_amd import BatchKnn
+    rng = np.random.default_rng(9)
+    probs = [rng.random((1, 8), dtype=np.float32), rng.random((10, 32), dtype=np.float32),
+             rng.random((0, 16), dtype=np.float32), rng.random((700, 64), dtype=np.float32)]
+    ks = [5, 12, 3, 0]
+    b = BatchKnn(probs, ks, mode="bf16")
+    b.run()
+    res = b.results()
+    for X, kk, (gi, gd) in zip(probs, ks, res):
+        assert gi.shape == (X.shape[0], kk)
+        if X.shape[0] and kk:
+            oi, od = oracle.knn(X, X, kk, "euclidean", "for_each_knn")
+            assert np.array_equal(gi, oi) and gd.tobytes() == od.tobytes()
+    # k beyond n pads with -1 / +inf
+    assert np.all(res[0][0] == -1) and np.all(np.isinf(res[0][1]))
+    b.close()
