@@ -52,9 +52,9 @@ def parse():
                         "c1 self-kNN 10k x 16 + pairwise 2000^2, c3 LOF 70k x 784 k=20, c4 512 uneven self-kNN "
                         "problems (batched), c5 serving 10M x 96 "
                         "under Poisson arrivals, dbscan on c1, kmeans distance matrix 1M x 64 vs 100 centres")
-    p.add_argument("--batch-mode", default="auto", choices=["auto", "bf16", "fp32", "exact"],
+    p.add_argument("--batch-mode", default="auto", choices=["auto", "bf16", "bf16r", "fp32", "exact"],
                    help="configuration of the batched problems (c4): autotuned or fixed")
-    p.add_argument("--serve-mode", default="fp32", choices=["bf16", "fp32", "exact"],
+    p.add_argument("--serve-mode", default="bf16r", choices=["bf16", "bf16r", "fp32", "exact"],
                    help="candidate mode of the served index (c5)")
     p.add_argument("--serve-seconds", type=float, default=4.0, help="seconds of arrivals per rate (c5)")
     p.add_argument("--serve-rows", type=int, default=10_000_000, help="rows of the served index (c5)")

@@ -28,6 +28,7 @@ SOURCES = [
     "cluster.cpp",
     "serve.cpp",
     "batch.cpp",
+    "resid.hip",
     "scan_exact.hip",
     "scan_mfma.hip",
     "finalize_rerank.hip",

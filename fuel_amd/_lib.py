@@ -29,10 +29,11 @@ MODE_AUTO = 0
 MODE_EXACT = 1
 MODE_FP32 = 2
 MODE_BF16 = 3
+MODE_BF16R = 4
 FORM_SQUARE = 0
 FORM_CONDENSED = 1
 
-MODES = {"auto": MODE_AUTO, "exact": MODE_EXACT, "fp32": MODE_FP32, "bf16": MODE_BF16}
+MODES = {"auto": MODE_AUTO, "exact": MODE_EXACT, "fp32": MODE_FP32, "bf16": MODE_BF16, "bf16r": MODE_BF16R}
 
 # every symbol include/fuel_amd.h declares
 EXPORTED = [

@@ -183,7 +183,7 @@ static int32_t index_create_common(const void* data, bool on_device, int64_t n, 
   o.widen = 4;
   if (opts) o = *opts;
   if (o.widen <= 0) o.widen = 4;
-  if (o.mode < FUEL_AMD_MODE_AUTO || o.mode > FUEL_AMD_MODE_BF16) return fail(FUEL_AMD_ERR_INVALID, "bad mode");
+  if (o.mode < FUEL_AMD_MODE_AUTO || o.mode > FUEL_AMD_MODE_BF16R) return fail(FUEL_AMD_ERR_INVALID, "bad mode");
   int32_t rc = check_device(o.device);
   if (rc) return rc;
   auto* ix = new (std::nothrow) fuel_amd_index();

@@ -53,7 +53,7 @@ hipError_t upload_rows(void* dst, int64_t ld, const void* host, int64_t rows, in
 // against one resident index (whose rows and operand copies are read-only).
 struct QueryCtx {
   struct LevelWs {  // one per level of the certification chain
-    DevBuf Qop, Q32, qnorm, qint, slots, lcnts, fail_count, fail_list, fail_theta, gthr;
+    DevBuf Qop, Q32, qnorm, qint, slots, lcnts, fail_count, fail_list, fail_theta, gthr, qtab;
   } lv[3];
   DevBuf slots, lcnts;  // exact scan
   fuel_amd_stats stats{};
@@ -70,6 +70,7 @@ struct QueryCtx {
       w.fail_list.release();
       w.fail_theta.release();
       w.gthr.release();
+      w.qtab.release();
     }
     slots.release();
     lcnts.release();
@@ -92,17 +93,20 @@ struct fuel_amd_index {
   fa::DevBuf upload;                     // staging of host query rows whose pitch differs from ld
   fa::DevBuf hq, hout_idx, hout_dist;    // staging for host-pointer calls
   fa::DevBuf ws_a, ws_b, ws_c, ws_d, ws_e, ws_tmp;  // range / pairwise / outlier workspaces
-  // mixed-precision candidate path: one operand copy per kind (0 bf16, 1 f32),
-  // prepared once (under prep_mu) on first use, read-only afterwards
+  // mixed-precision candidate path: one operand copy per kind (0 bf16, 1 f32,
+  // 2 bf16 residual), prepared once (under prep_mu) on first use, read-only
+  // afterwards
   struct MfmaOperand {
     std::atomic<bool> ready{false};
     bool usable = true;  // false when some squared norm reaches MFMA_NORM2_LIMIT
     int ks = 0;  // K steps
     int kp = 0;  // padded K (elements)
-    fa::DevBuf Xop, hx, consts;  // operand copy, ||x||^2/2, bound constants + flags
-  } mop[2];
+    int64_t n_pad = 0;  // operand rows (kind 2: lists padded to 64-row blocks)
+    fa::DevBuf Xop, hx, consts;  // operand copy, ||x||^2/2 (kind 2: of the residual), bound constants + flags
+    int L = 0;                   // kind 2: lists, centroids (L x d), position -> row, list of each
+    fa::DevBuf cents, rperm, tile_list, lconsts;  // 64-row block, per-list bound constants
+  } mop[3];
   std::mutex prep_mu;
-  int64_t n_pad = 0;
   fa::QueryCtx ctx;  // the synchronous entry points' query workspaces and stats
   // profiling: event pairs around the dominant kernel of each call
   bool timing = false;
@@ -125,6 +129,10 @@ struct fuel_amd_index {
       m.Xop.release();
       m.hx.release();
       m.consts.release();
+      m.cents.release();
+      m.rperm.release();
+      m.tile_list.release();
+      m.lconsts.release();
       m.ready = false;
     }
   }
@@ -150,7 +158,8 @@ int32_t optics_run(fuel_amd_index* ix, double max_eps, int64_t min_points, int64
                    void* core, int64_t* pred, int64_t* labels);
 int32_t pairwise_run(fuel_amd_index* ix, int form, void* out);
 int32_t outlier_run(fuel_amd_index* ix, int k, int knn_outlier, void* out_scores);
-// Mixed-precision kNN (f32 indexes): kind 0 bf16 MFMA, 1 f32 MFMA.  Returns
+// Mixed-precision kNN (f32 indexes): kind 0 bf16 MFMA, 1 f32 MFMA, 2 bf16 MFMA
+// on the residual operand.  Returns
 // FUEL_AMD_ERR_UNSUPPORTED when the shape has no MFMA variant.
 // With d_rows, slot i is query row d_rows[i] of Q and writes output row d_rows[i].
 // kp = candidates kept per split (0: from the index's widening factor); level =

@@ -31,11 +31,138 @@ int kp_for(const fuel_amd_index* ix, int k, int exclude) {
   return std::max(KK, std::min(KP_MAX, w * k));
 }
 
+// Lists of the residual operand: about 256 rows per list, at most 256.  Many
+// more lists than clusters: a list that straddles two clusters has a wide
+// per-list bound, and evenly spaced starting rows miss few clusters when
+// they outnumber them several times.
+int resid_lists(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(256, n / 256)); }
+
+// Partition (Lloyd steps from evenly spaced rows), sorted padded layout and the
+// residual operand with its per-list bound constants.
+int32_t prepare_resid(fuel_amd_index* ix, fuel_amd_index::MfmaOperand& op, hipStream_t st) {
+  const int64_t n = ix->n;
+  const int d = ix->d;
+  const int ks = mfma_ksteps_bucket(0, d);
+  if (ks < 0 || d > 1024) return fail(FUEL_AMD_ERR_UNSUPPORTED, "dimension too large for the MFMA candidate scan");
+  const int Kp = ks * 16;
+  const int L = resid_lists(n);
+  const float* X = ix->X.as<float>();
+  DevBuf list, counts, keys_out, vals_in, vals_out, tmp, offs, poffs;
+  hipError_t e = list.ensure(sizeof(uint32_t) * std::max<int64_t>(n, 1));
+  if (e == hipSuccess) e = counts.ensure(sizeof(uint32_t) * L);
+  if (e == hipSuccess) e = keys_out.ensure(sizeof(uint32_t) * std::max<int64_t>(n, 1));
+  if (e == hipSuccess) e = vals_in.ensure(sizeof(uint32_t) * std::max<int64_t>(n, 1));
+  if (e == hipSuccess) e = vals_out.ensure(sizeof(uint32_t) * std::max<int64_t>(n, 1));
+  if (e == hipSuccess) e = offs.ensure(sizeof(int64_t) * (L + 1));
+  if (e == hipSuccess) e = poffs.ensure(sizeof(int64_t) * (L + 1));
+  if (e == hipSuccess) e = op.cents.ensure(sizeof(float) * size_t(L) * d);
+  if (e != hipSuccess) return hip_fail(e, "partition workspaces");
+  int bits = 1;
+  while ((1 << bits) < L) ++bits;
+  size_t tmp_bytes = 0;
+  e = sort_pairs_u32(list.as<uint32_t>(), keys_out.as<uint32_t>(), vals_in.as<uint32_t>(), vals_out.as<uint32_t>(),
+                     n, bits, nullptr, tmp_bytes, st);
+  if (e == hipSuccess) e = tmp.ensure(std::max<size_t>(tmp_bytes, 256));
+  // initial centroids: rows floor(i n / L)
+  for (int i = 0; i < L && e == hipSuccess; ++i)
+    e = hipMemcpyAsync(op.cents.as<float>() + size_t(i) * d, X + (int64_t(i) * n / L) * ix->ld, sizeof(float) * d,
+                       hipMemcpyDeviceToDevice, st);
+  if (e != hipSuccess) return hip_fail(e, "partition init");
+  std::vector<uint32_t> hcnt(L);
+  std::vector<int64_t> hoff(L + 1);
+  constexpr int kLloydSteps = 4;
+  for (int it = 0; it <= kLloydSteps; ++it) {
+    AssignArgs aa{};
+    aa.X = X;
+    aa.ldx = ix->ld;
+    aa.n = n;
+    aa.d = d;
+    aa.cents = op.cents.as<float>();
+    aa.L = L;
+    aa.list = list.as<uint32_t>();
+    aa.counts = counts.as<uint32_t>();
+    e = launch_fill_u32(counts.as<uint32_t>(), L, 0u, st);
+    if (e == hipSuccess) e = launch_assign(aa, st);
+    if (e == hipSuccess) e = launch_iota_u32(vals_in.as<uint32_t>(), n, st);
+    if (e == hipSuccess)
+      e = sort_pairs_u32(list.as<uint32_t>(), keys_out.as<uint32_t>(), vals_in.as<uint32_t>(),
+                         vals_out.as<uint32_t>(), n, bits, tmp.p, tmp_bytes, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(hcnt.data(), counts.p, sizeof(uint32_t) * L, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "partition");
+    hoff[0] = 0;
+    for (int j = 0; j < L; ++j) hoff[j + 1] = hoff[j] + hcnt[j];
+    e = hipMemcpyAsync(offs.p, hoff.data(), sizeof(int64_t) * (L + 1), hipMemcpyHostToDevice, st);
+    if (it < kLloydSteps && e == hipSuccess)
+      e = launch_list_means(X, ix->ld, d, vals_out.as<uint32_t>(), offs.as<int64_t>(), counts.as<uint32_t>(), L,
+                            op.cents.as<float>(), st);
+    if (e != hipSuccess) return hip_fail(e, "partition");
+  }
+  // padded layout: every list starts on a 64-row block
+  std::vector<int64_t> hpad(L + 1);
+  hpad[0] = 0;
+  for (int j = 0; j < L; ++j) hpad[j + 1] = hpad[j] + ((int64_t(hcnt[j]) + 63) & ~int64_t(63));
+  const int64_t n_rpad = std::max<int64_t>(256, (hpad[L] + 255) & ~int64_t(255));
+  std::vector<int> htl(n_rpad / 64, 0);
+  for (int j = 0; j < L; ++j)
+    for (int64_t b = hpad[j] / 64; b < hpad[j + 1] / 64; ++b) htl[b] = j;
+  for (int64_t b = hpad[L] / 64; b < n_rpad / 64; ++b) htl[b] = L - 1;
+  e = op.rperm.ensure(sizeof(uint32_t) * n_rpad);
+  if (e == hipSuccess) e = op.tile_list.ensure(sizeof(int) * htl.size());
+  if (e == hipSuccess) e = op.lconsts.ensure(sizeof(unsigned long long) * 8 * L);
+  if (e == hipSuccess) e = op.Xop.ensure(sizeof(uint16_t) * size_t(n_rpad) * Kp);
+  if (e == hipSuccess) e = op.hx.ensure(sizeof(float) * n_rpad);
+  if (e == hipSuccess) e = op.consts.ensure(128);
+  if (e != hipSuccess) return hip_fail(e, "residual operand");
+  e = hipMemcpyAsync(poffs.p, hpad.data(), sizeof(int64_t) * (L + 1), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(op.tile_list.p, htl.data(), sizeof(int) * htl.size(), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = launch_fill_u32(op.rperm.as<uint32_t>(), n_rpad, RPERM_NONE, st);
+  if (e == hipSuccess)
+    e = launch_scatter_rperm(vals_out.as<uint32_t>(), keys_out.as<uint32_t>(), n, offs.as<int64_t>(),
+                             poffs.as<int64_t>(), op.rperm.as<uint32_t>(), st);
+  if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(op.lconsts.p), 16 * L, 0u, st);
+  if (e == hipSuccess) e = launch_fill_u32(op.consts.as<uint32_t>(), 32, 0u, st);  // constants and flags
+  PrepResidArgs pr{};
+  pr.X = X;
+  pr.ldx = ix->ld;
+  pr.d = d;
+  pr.Kp = Kp;
+  pr.n_rpad = n_rpad;
+  pr.rperm = op.rperm.as<uint32_t>();
+  pr.tile_list = op.tile_list.as<int>();
+  pr.cents = op.cents.as<float>();
+  pr.ldc = d;
+  pr.Xop = op.Xop.as<uint16_t>();
+  pr.hx = op.hx.as<float>();
+  pr.lconsts = op.lconsts.as<unsigned long long>();
+  pr.gconsts = op.consts.as<unsigned long long>();
+  if (e == hipSuccess) e = launch_prep_resid(pr, st);
+  unsigned long long x2bits = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&x2bits, op.consts.p, sizeof(x2bits), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);  // also before the workspaces above are released
+  if (e != hipSuccess) return hip_fail(e, "residual operand");
+  double x2;
+  std::memcpy(&x2, &x2bits, sizeof(x2));
+  op.usable = x2 < MFMA_NORM2_LIMIT;
+  op.ks = ks;
+  op.kp = Kp;
+  op.n_pad = n_rpad;
+  op.L = L;
+  return FUEL_AMD_OK;
+}
+
 int32_t ensure_prepared(fuel_amd_index* ix, int kind, hipStream_t st) {
   auto& op = ix->mop[kind];
   if (op.ready.load(std::memory_order_acquire)) return FUEL_AMD_OK;
   std::lock_guard<std::mutex> lock(ix->prep_mu);
   if (op.ready.load(std::memory_order_relaxed)) return FUEL_AMD_OK;
+  if (kind == 2) {
+    int32_t rc = prepare_resid(ix, op, st);
+    if (rc) return rc;
+    op.ready.store(true, std::memory_order_release);
+    return FUEL_AMD_OK;
+  }
   const int ks = mfma_ksteps_bucket(kind, ix->d);
   if (ks < 0) return fail(FUEL_AMD_ERR_UNSUPPORTED, "dimension too large for the MFMA candidate scan");
   const int kp = ks * (kind == 0 ? 16 : 8);
@@ -74,7 +201,7 @@ int32_t ensure_prepared(fuel_amd_index* ix, int kind, hipStream_t st) {
   op.usable = x2 < MFMA_NORM2_LIMIT;
   op.ks = ks;
   op.kp = kp;
-  ix->n_pad = n_pad;
+  op.n_pad = n_pad;
   op.ready.store(true, std::memory_order_release);
   return FUEL_AMD_OK;
 }
@@ -106,7 +233,8 @@ int mfma_splits(int64_t nqb, int64_t n_pad, int cus, int64_t rows_tile, int wg_p
 
 bool mfma_supported(const fuel_amd_index* ix, int kind, int k, int exclude) {
   if (ix->dtype != FUEL_AMD_F32) return false;
-  if (mfma_ksteps_bucket(kind, ix->d) < 0) return false;
+  const int ks = mfma_ksteps_bucket(kind == 2 ? 0 : kind, ix->d);
+  if (ks < 0 || (kind == 2 && ks > mfma_resid_max_ksteps())) return false;
   const int KK = k + (exclude != EXCL_NONE ? 1 : 0);
   return KK <= KP_MAX;
 }
@@ -115,11 +243,12 @@ int32_t prepare_mode(fuel_amd_index* ix, hipStream_t st) {
   int mode = ix->opts.mode;
   if (mode == FUEL_AMD_MODE_AUTO) mode = FUEL_AMD_MODE_BF16;
   if (mode == FUEL_AMD_MODE_EXACT || ix->dtype != FUEL_AMD_F32) return FUEL_AMD_OK;
-  const int kind = mode == FUEL_AMD_MODE_BF16 ? 0 : 1;
-  if (mfma_ksteps_bucket(kind, ix->d) < 0) return FUEL_AMD_OK;
+  int kind = mode == FUEL_AMD_MODE_BF16 ? 0 : (mode == FUEL_AMD_MODE_BF16R ? 2 : 1);
+  if (kind == 2 && !mfma_supported(ix, 2, 1, 0)) kind = 0;  // as knn_run
+  if (mfma_ksteps_bucket(kind == 2 ? 0 : kind, ix->d) < 0) return FUEL_AMD_OK;
   int32_t rc = ensure_prepared(ix, kind, st);
   // the certification chain continues bf16 with f32
-  if (rc == FUEL_AMD_OK && kind == 0 && mfma_ksteps_bucket(1, ix->d) >= 0) rc = ensure_prepared(ix, 1, st);
+  if (rc == FUEL_AMD_OK && kind != 1 && mfma_ksteps_bucket(1, ix->d) >= 0) rc = ensure_prepared(ix, 1, st);
   return rc;
 }
 
@@ -142,11 +271,14 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
   const int csub = (NS - KP) / 2;
   const int ks = op.ks;
   const int Kp = op.kp;
-  const int qpb = mfma_queries_per_block(kind, ks);
-  const int rows_tile = mfma_rows_per_tile(kind, ks);
-  const int wg_per_cu = mfma_workgroups_per_cu(kind, ks);
+  const bool resid = kind == 2;
+  const int mk = resid ? 0 : kind;  // MFMA operand type
+  const int qpb = mfma_queries_per_block(mk, ks);
+  const int rows_tile = mfma_rows_per_tile(mk, ks);
+  const int wg_per_cu = mfma_workgroups_per_cu(mk, ks);
   const int cus = device_cu_count(ix->device);
-  const size_t es = kind == 0 ? 2 : 4;
+  const size_t es = mk == 0 ? 2 : 4;
+  const int64_t n_pad = op.n_pad;
   const int64_t ld32 = ix->ld;
   const bool timed = level == 0 && ctx->timed;
 
@@ -155,7 +287,7 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
     if (cancelled()) return fail(FUEL_AMD_ERR_INTERRUPTED, "interrupted");
     int64_t chunk = m - done;
     int64_t nqb = (chunk + qpb - 1) / qpb;
-    int nsplit = mfma_splits(nqb, ix->n_pad, cus, rows_tile, wg_per_cu);
+    int nsplit = mfma_splits(nqb, n_pad, cus, rows_tile, wg_per_cu);
     const size_t per_q = size_t(nsplit) * NS * 8;
     const int64_t max_chunk = std::max<int64_t>(qpb, int64_t(MFMA_SLOT_BUDGET / per_q) / qpb * qpb);
     if (chunk > max_chunk) {
@@ -163,11 +295,11 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
       nqb = chunk / qpb;
     }
     const int64_t nq_pad = nqb * qpb;
-    int64_t rps = (ix->n_pad + nsplit - 1) / nsplit;
+    int64_t rps = (n_pad + nsplit - 1) / nsplit;
     rps = (rps + rows_tile - 1) / rows_tile * rows_tile;
-    nsplit = (int)((ix->n_pad + rps - 1) / rps);
+    nsplit = (int)((n_pad + rps - 1) / rps);
 
-    hipError_t e = ws.Qop.ensure(size_t(nq_pad) * Kp * es);
+    hipError_t e = resid ? ws.qtab.ensure(sizeof(float) * size_t(op.L) * nq_pad) : ws.Qop.ensure(size_t(nq_pad) * Kp * es);
     if (e == hipSuccess) e = ws.Q32.ensure(size_t(chunk) * ld32 * 4);
     if (e == hipSuccess) e = ws.qnorm.ensure(sizeof(double) * 3 * size_t(chunk));
     if (e == hipSuccess) e = ws.qint.ensure(sizeof(int) * size_t(chunk));
@@ -190,8 +322,8 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
     qa.m_pad = nq_pad;
     qa.d = ix->d;
     qa.Kp = Kp;
-    qa.bf16 = kind == 0;
-    qa.Qop = ws.Qop.p;
+    qa.bf16 = mk == 0;
+    qa.Qop = resid ? nullptr : ws.Qop.p;
     qa.Q32 = ws.Q32.as<float>();
     qa.ld32 = ld32;
     qa.qn2 = ws.qnorm.as<double>();
@@ -202,15 +334,29 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
     qa.rows = rows;
     qa.theta = d_theta ? d_theta + done : nullptr;
     e = launch_prep_queries(qa, st);
-    if (e == hipSuccess) e = hipMemsetAsync(ws.fail_count.p, 0, sizeof(int), st);
+    if (resid && e == hipSuccess) {
+      QListArgs la{};
+      la.Q32 = qa.Q32;
+      la.ld32 = ld32;
+      la.m = chunk;
+      la.nq_pad = nq_pad;
+      la.d = ix->d;
+      la.cents = op.cents.as<float>();
+      la.ldc = ix->d;
+      la.L = op.L;
+      la.lconsts = op.lconsts.as<unsigned long long>();
+      la.qtab = ws.qtab.as<float>();
+      e = launch_prep_qlist(la, st);
+    }
+    if (e == hipSuccess) e = launch_fill_u32(ws.fail_count.as<uint32_t>(), 1, 0u, st);
     if (e != hipSuccess) return hip_fail(e, "query preparation");
 
     ScanMfmaArgs sa{};
     sa.Xop = op.Xop.p;
     sa.hx = op.hx.as<float>();
     sa.Kp = Kp;
-    sa.n_pad = ix->n_pad;
-    sa.Qop = ws.Qop.p;
+    sa.n_pad = n_pad;
+    sa.Qop = resid ? nullptr : ws.Qop.p;
     sa.nq_pad = nq_pad;
     sa.nqb = (int)nqb;
     sa.rows_per_split = rps;
@@ -219,8 +365,21 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
     sa.slots = ws.slots.as<uint64_t>();
     sa.lcnts = ws.lcnts.as<int>();
     sa.gthr = ws.gthr.as<uint32_t>();
-    if (timed) timing_mark(ix, st, true, kind == 0 ? "scan_mfma_kernel<bf16>" : "scan_mfma_kernel<f32>");
-    e = launch_scan_mfma(sa, kind, ks, V, nsplit, st);
+    if (resid) {
+      sa.resid = 1;
+      sa.tile_list = op.tile_list.as<int>();
+      sa.cents = op.cents.as<float>();
+      sa.ldc = ix->d;
+      sa.Q32 = qa.Q32;
+      sa.ld32 = ld32;
+      sa.m = chunk;
+      sa.d = ix->d;
+      sa.qtab = ws.qtab.as<float>();
+    }
+    if (timed)
+      timing_mark(ix, st, true,
+                  kind == 0 ? "scan_mfma_kernel<bf16>" : (kind == 1 ? "scan_mfma_kernel<f32>" : "scan_mfma_kernel<bf16r>"));
+    e = launch_scan_mfma(sa, mk, ks, V, nsplit, st);
     if (timed) timing_mark(ix, st, false, nullptr);
     if (e != hipSuccess) return hip_fail(e, "MFMA scan launch");
 
@@ -238,7 +397,9 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
     ra.ld32 = ld32;
     ra.d = ix->d;
     ra.metric = ix->metric;
-    ra.bf16 = kind == 0;
+    ra.bf16 = mk == 0;
+    ra.resid = resid ? 1 : 0;
+    ra.rperm = resid ? op.rperm.as<uint32_t>() : nullptr;
     ra.qn2 = qa.qn2;
     ra.eq2 = qa.eq2;
     ra.qt2 = qa.qt2;
@@ -268,11 +429,14 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
       int next_kind = -1;
       if (level == 0)
         next_kind = kind;
-      else if (level == 1 && kind == 0 && mfma_supported(ix, 1, k, exclude))
+      else if (level == 1 && kind != 1 && mfma_supported(ix, 1, k, exclude))
         next_kind = 1;
       if (next_kind >= 0) {
+        // theta is in the key units of this kind's scan: the residual keys
+        // (||q - x||^2 / 2) are not those of the f32 form
+        const float* theta = (next_kind == kind || !resid) ? ra.fail_theta : nullptr;
         rc = knn_mfma(ix, ctx, Qb, nfail, ldq, k, exclude, next_kind, ra.out_idx, ra.out_dist, st, ra.fail_list,
-                      ra.qbase, level + 1, KP_MAX, ra.fail_theta);
+                      ra.qbase, level + 1, KP_MAX, theta);
       } else {
         ctx->stats.exact_queries += nfail;  // queries answered by the exact scan
         rc = knn_exact_rows(ix, ctx, Qb, ldq, ra.fail_list, nfail, ra.qbase, k, exclude, ra.out_idx, ra.out_dist,

@@ -154,6 +154,44 @@ __device__ __forceinline__ float certified_threshold(const RerankArgs& a, const 
   return certify(a, b, Tf, kk) ? Tf : INFINITY;
 }
 
+// Residual operand: an excluded row's stored key s >= T is, up to the row tag
+// (<= 15 ulps of the MFMA result), a lower bound of D(x) / 2, so
+// D(x) >= 2 (T - 2^-19 |T| (1 + 2^-10) - 2^-145) = L (s - c|s| increases with s),
+// and its restated f32 value r(x) >= L (1 - gamma_r) - (2 d + 16) 2^-149.
+__device__ __forceinline__ double resid_lower(double T) {
+  return 2.0 * (T - 0x1p-19 * fabs(T) * (1 + 0x1p-10) - 0x1p-145);
+}
+
+__device__ __forceinline__ bool certify_resid(const RerankArgs& a, const BoundTerms& b, float T, float kk) {
+  if (!b.usable) return false;
+  const double L = resid_lower((double)T);
+  const double Lr = L * (1 - b.gr) - (2.0 * b.d + 16) * 0x1p-149;
+  if (!(Lr > 0.0)) return false;
+  float B = (float)Lr;
+  if ((double)B > Lr) B = nextafterf(B, -INFINITY);
+  return a.metric == 0 ? (sqrt_rn(B) > kk) : (B > kk);
+}
+
+__device__ __forceinline__ float certified_threshold_resid(const RerankArgs& a, const BoundTerms& b, float kk) {
+  if (!b.usable || !(kk < INFINITY)) return INFINITY;
+  double target;
+  if (a.metric == 0) {
+    const double kn = (double)nextafterf(kk, INFINITY);
+    target = kn * kn;
+  } else {
+    target = (double)kk;
+  }
+  target = target * (1 + 0x1p-20) + 0x1p-140;
+  const double needL = (target + (2.0 * b.d + 16) * 0x1p-149) / (1 - b.gr);
+  // 2 (T - c T - 2^-145) >= needL for T >= 0
+  double T = (0.5 * needL + 0x1p-145) / (1 - 0x1p-19 * (1 + 0x1p-10));
+  T += fabs(T) * 0x1p-40 + 0x1p-140;
+  float Tf = (float)T;
+  if ((double)Tf < T) Tf = nextafterf(Tf, INFINITY);
+  for (int i = 0; i < 8 && !certify_resid(a, b, Tf, kk); ++i) Tf = nextafterf(Tf, INFINITY);
+  return certify_resid(a, b, Tf, kk) ? Tf : INFINITY;
+}
+
 template <int V>
 __global__ __launch_bounds__(64) void rerank_kernel(RerankArgs a) {
   using C = Comp<float>;
@@ -219,10 +257,13 @@ __global__ __launch_bounds__(64) void rerank_kernel(RerankArgs a) {
     const int e = v * 64 + lane;
     y[v] = C::MAX;
     if (e < ncand) {
-      const uint32_t idx = C::idx(x[v]);
-      const float s = restated_aligned(qrow, a.X + size_t(idx) * a.ldx, a.d);
-      const float key = a.metric == 0 ? sqrt_rn(s) : s;
-      y[v] = C::make(key, idx);
+      uint32_t idx = C::idx(x[v]);
+      if (a.resid) idx = a.rperm[idx];  // padded position -> row (padding rows never survive)
+      if (idx != RPERM_NONE) {
+        const float s = restated_aligned(qrow, a.X + size_t(idx) * a.ldx, a.d);
+        const float key = a.metric == 0 ? sqrt_rn(s) : s;
+        y[v] = C::make(key, idx);
+      }
     }
   }
   wave_bitonic_sort<T, V>(y);
@@ -234,14 +275,15 @@ __global__ __launch_bounds__(64) void rerank_kernel(RerankArgs a) {
   if (T_C < MFMA_KEY_LIMIT) {
     kk = ncand >= a.KK ? C::key(wave_get<T, V>(y, a.KK - 1)) : INFINITY;
     bt = bound_terms(a, qs);
-    cert = certify(a, bt, T_C, kk);
+    cert = a.resid ? certify_resid(a, bt, T_C, kk) : certify(a, bt, T_C, kk);
   }
   const int64_t orow = a.rows ? a.rows[qs] : qs;
   if (cert) {
     const int cnt = ncand < a.KK ? ncand : a.KK;
     write_knn_row<float, V>(y, cnt, a.qbase + orow, a.k, a.mode, a.out_idx + orow * a.k, a.out_dist + orow * a.k);
   } else {
-    const float theta = a.fail_theta ? certified_threshold(a, bt, kk) : INFINITY;
+    const float theta = !a.fail_theta ? INFINITY
+                        : (a.resid ? certified_threshold_resid(a, bt, kk) : certified_threshold(a, bt, kk));
     if (lane == 0) {
       const int pos = atomicAdd(a.fail_count, 1);
       a.fail_list[pos] = orow;

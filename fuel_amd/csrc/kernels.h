@@ -236,7 +236,90 @@ struct ScanMfmaArgs {
   uint64_t* slots;         // [nsplit][nq_pad][NS]
   int* lcnts;              // [nsplit][nq_pad]
   uint32_t* gthr;          // [nq_pad] per-query threshold shared by the splits (order-preserving bits)
+  // residual operand (bf16 only; resid = 0 otherwise): rows sorted by list,
+  // every list padded to whole 64-row blocks; the B operand of a block of list
+  // j is bf16(fl32(q - c_j)), rebuilt from Q32 when the list changes, and the
+  // per-(list, query) term qtab[j * nq_pad + q] joins the C input
+  int resid;
+  const int* tile_list;    // list of each 64-row block
+  const float* cents;      // L x ldc centroids
+  int ldc;
+  const float* Q32;        // m x ld32 f32 queries
+  int64_t ld32;
+  int64_t m;
+  int d;
+  const float* qtab;       // [L][nq_pad]
 };
+
+// ---- residual (centroid-relative) operand ------------------------------------
+// Nearest centroid of each row (ties: the smaller centroid index), and the
+// number of rows per centroid.
+struct AssignArgs {
+  const float* X;
+  int64_t ldx;
+  int64_t n;
+  int d;
+  const float* cents;  // L x d
+  int L;
+  uint32_t* list;      // n
+  uint32_t* counts;    // L (incremented)
+};
+hipError_t launch_assign(const AssignArgs& a, hipStream_t st);
+// New centroid of every non-empty list: the f64 mean of its rows, in sorted
+// row order (deterministic); empty lists keep their centroid.
+hipError_t launch_list_means(const float* X, int64_t ldx, int d, const uint32_t* sorted_rows,
+                             const int64_t* offsets, const uint32_t* counts, int L, float* cents,
+                             hipStream_t st);
+// Padded position of every row: rperm[pad_off[list] + rank] = row.
+hipError_t launch_scatter_rperm(const uint32_t* sorted_rows, const uint32_t* sorted_list, int64_t n,
+                                const int64_t* offsets, const int64_t* pad_offsets, uint32_t* rperm,
+                                hipStream_t st);
+hipError_t launch_iota_u32(uint32_t* p, int64_t n, hipStream_t st);
+// Stable sort of (key, value) u32 pairs by key (rocPRIM radix sort, low `bits` bits).
+hipError_t sort_pairs_u32(const uint32_t* keys_in, uint32_t* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
+                          int64_t n, int bits, void* tmp, size_t& tmp_bytes, hipStream_t st);
+
+// Residual operand rows: position p holds row rperm[p] (RPERM_NONE: padding)
+// of list tile_list[p / 64] as r~ = bf16(fl32(x - c_j)), hxn = -fl32(||r||^2/2)
+// with r = x - c_j exact in f64; per-list maxima (order-preserving u64 bits of
+// non-negative doubles) lconsts[8 j + i]: [0] ||r||^2, [1] ||r~ - r||^2,
+// [2] ||r~||^2, [3] |fl32(||r||^2/2) - ||r||^2/2| (+ the f64 error of the
+// sum), [4] fl32(||r||^2/2).
+constexpr uint32_t RPERM_NONE = 0xFFFFFFFFu;
+struct PrepResidArgs {
+  const float* X;
+  int64_t ldx;
+  int d;
+  int Kp;
+  int64_t n_rpad;
+  const uint32_t* rperm;
+  const int* tile_list;
+  const float* cents;
+  int ldc;
+  uint16_t* Xop;  // n_rpad x Kp bf16
+  float* hx;
+  unsigned long long* lconsts;  // L x 8, zero before the launch
+  unsigned long long* gconsts;  // [0] max ||x||^2 (the sentinel-range check); zero before the launch
+};
+hipError_t launch_prep_resid(const PrepResidArgs& a, hipStream_t st);
+// C-input term of every (list, query slot): a float no smaller than
+// -||q - c_j||^2 / 2 + e_j(q), e_j(q) bounding the residual product's
+// operand rounding, the hxn rounding, the MFMA accumulation and the C-input
+// addition, so that -(MFMA result) never exceeds ||q - x||^2 / 2 before the
+// row tag.  Slots >= m repeat query m - 1.
+struct QListArgs {
+  const float* Q32;
+  int64_t ld32;
+  int64_t m;
+  int64_t nq_pad;
+  int d;
+  const float* cents;
+  int ldc;
+  int L;
+  const unsigned long long* lconsts;
+  float* qtab;  // [L][nq_pad]
+};
+hipError_t launch_prep_qlist(const QListArgs& a, hipStream_t st);
 
 struct RerankArgs {
   const uint64_t* slots;
@@ -271,6 +354,10 @@ struct RerankArgs {
   const uint32_t* gthr;  // final shared thresholds of the scan, per slot
   float* fail_theta;     // optional: beside fail_list, a key threshold below which
                          // every row that can still enter the query's answer lies
+  // residual operand: candidates are padded positions (rperm maps them to rows)
+  // and keys are lower bounds of ||q - x||^2 / 2 up to the row tag
+  int resid;
+  const uint32_t* rperm;
 };
 
 // A kernel that stores v into count u32s (unlike hipMemsetAsync, safe to
@@ -300,6 +387,8 @@ hipError_t launch_scan_mfma_batch(const ScanBatchArgs& b, int kind, int ksteps, 
 // Supported K step counts of the MFMA scan (bf16: 16 dims per step; f32: 8 dims
 // per step group); the padded K is steps * 16 (bf16) or steps * 8 (f32).
 int mfma_ksteps_bucket(int kind, int d);
+// Largest K step count with a residual-operand variant of the bf16 scan.
+int mfma_resid_max_ksteps();
 int mfma_queries_per_block(int kind, int ksteps);
 int mfma_rows_per_tile(int kind, int ksteps);
 int mfma_workgroups_per_cu(int kind, int ksteps);

@@ -47,9 +47,9 @@ __device__ __forceinline__ RowStats prep_row(const float* __restrict__ src, int 
     float vt = v;
     if (bf16) {
       const uint16_t h = bf16_rne(v);
-      reinterpret_cast<uint16_t*>(dst)[c] = h;
+      if (dst) reinterpret_cast<uint16_t*>(dst)[c] = h;
       vt = bf16_val(h);
-    } else {
+    } else if (dst) {
       reinterpret_cast<float*>(dst)[c] = v;
     }
     const double dv = (double)v, dt = (double)vt;
@@ -107,10 +107,12 @@ __global__ __launch_bounds__(256) void prep_query_kernel(PrepQArgs a) {
   if (row >= a.m_pad) return;
   const int lane = __lane_id();
   const size_t es = a.bf16 ? 2 : 4;
-  char* dst = reinterpret_cast<char*>(a.Qop) + size_t(row) * a.Kp * es;
+  // no operand copy for the residual operand (the scan builds it per list)
+  char* dst = a.Qop ? reinterpret_cast<char*>(a.Qop) + size_t(row) * a.Kp * es : nullptr;
   if (lane == 0 && a.gthr)  // +inf, or the caller's threshold for this slot
     a.gthr[row] = (a.theta && row < a.m) ? ord_f32(a.theta[row]) : ord_f32(INFINITY);
   if (row >= a.m) {
+    if (!dst) return;
     for (int c = lane; c < a.Kp; c += 64) {
       if (a.bf16)
         reinterpret_cast<uint16_t*>(dst)[c] = 0;

@@ -1,0 +1,260 @@
+// Residual (centroid-relative) operand of the bf16 candidate scan.
+//
+// On clustered data ||x|| and ||q|| are large next to the neighbour distances,
+// and the bf16 bound of the plain form, ~2^-9 ||q|| ||x||, swamps the gaps
+// between them.  Rows are partitioned around L centroids (a few Lloyd steps),
+// sorted by list, and each list padded to whole 64-row blocks.  For a row x of
+// list j with r = x - c_j,
+//     ||q - x||^2 / 2 = ||q - c_j||^2 / 2 - (q - c_j).r + ||r||^2 / 2,
+// so the MFMA contracts bf16(q - c_j) with bf16(r): the rounding error scales
+// with ||q - c_j|| ||r||, the distances to the centroid, instead of ||q|| ||x||.
+// ||r||^2 / 2 rides in the per-row C input as before and ||q - c_j||^2 / 2,
+// minus a proven bound on every other error, in a per-(list, query) term, so
+// the scan's keys are lower bounds of ||q - x||^2 / 2 (up to the row tag).
+// The partition only affects speed: every answer is still certified against
+// the exact restated distances.
+#include "kernels.h"
+
+namespace fa {
+
+namespace {
+
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_val(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+  return v;
+}
+
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double v) {
+  atomicMax(p, (unsigned long long)__double_as_longlong(v));
+}
+
+constexpr int ASSIGN_CC = 16;     // centroids per LDS chunk
+constexpr int ASSIGN_MAXD = 1024;  // largest dimension of the bf16 scan
+
+}  // namespace
+
+// One thread per row; centroids staged through LDS 16 at a time (zero-padded
+// to a multiple of 16 coordinates, as are the row's coordinates).
+__global__ __launch_bounds__(256) void assign_kernel(AssignArgs a) {
+  __shared__ float cs[ASSIGN_CC * ASSIGN_MAXD];
+  const int64_t row = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int dp = (a.d + 15) & ~15;
+  float best = INFINITY;
+  uint32_t bi = 0;
+  for (int c0 = 0; c0 < a.L; c0 += ASSIGN_CC) {
+    const int cc = a.L - c0 < ASSIGN_CC ? a.L - c0 : ASSIGN_CC;
+    __syncthreads();
+    for (int e = threadIdx.x; e < ASSIGN_CC * dp; e += 256) {
+      const int c = e / dp, k = e - c * dp;
+      cs[e] = (c < cc && k < a.d) ? a.cents[size_t(c0 + c) * a.d + k] : 0.0f;
+    }
+    __syncthreads();
+    if (row < a.n) {
+      float acc[ASSIGN_CC];
+#pragma unroll
+      for (int c = 0; c < ASSIGN_CC; ++c) acc[c] = 0.0f;
+      const float* x = a.X + row * a.ldx;
+      for (int k0 = 0; k0 < dp; k0 += 16) {
+        float xv[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) xv[e] = k0 + e < a.d ? x[k0 + e] : 0.0f;
+#pragma unroll
+        for (int c = 0; c < ASSIGN_CC; ++c)
+#pragma unroll
+          for (int e = 0; e < 16; ++e) {
+            const float t = xv[e] - cs[c * dp + k0 + e];
+            acc[c] = fmaf(t, t, acc[c]);
+          }
+      }
+      for (int c = 0; c < cc; ++c)
+        if (acc[c] < best) {
+          best = acc[c];
+          bi = uint32_t(c0 + c);
+        }
+    }
+  }
+  if (row < a.n) {
+    a.list[row] = bi;
+    atomicAdd(&a.counts[bi], 1u);
+  }
+}
+
+hipError_t launch_assign(const AssignArgs& a, hipStream_t st) {
+  if (a.n == 0) return hipSuccess;
+  if (a.d > ASSIGN_MAXD || a.L < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(assign_kernel, dim3((unsigned)((a.n + 255) / 256)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+// One workgroup per list; thread t sums coordinates t, t + 256, ... over the
+// list's rows in sorted order (f64).
+__global__ __launch_bounds__(256) void list_means_kernel(const float* X, int64_t ldx, int d,
+                                                         const uint32_t* sorted_rows, const int64_t* offsets,
+                                                         const uint32_t* counts, float* cents) {
+  const int j = blockIdx.x;
+  const uint32_t cnt = counts[j];
+  if (cnt == 0) return;
+  const int64_t off = offsets[j];
+  for (int k = threadIdx.x; k < d; k += 256) {
+    double s = 0.0;
+    for (uint32_t i = 0; i < cnt; ++i) s += (double)X[int64_t(sorted_rows[off + i]) * ldx + k];
+    cents[size_t(j) * d + k] = (float)(s / (double)cnt);
+  }
+}
+
+hipError_t launch_list_means(const float* X, int64_t ldx, int d, const uint32_t* sorted_rows,
+                             const int64_t* offsets, const uint32_t* counts, int L, float* cents,
+                             hipStream_t st) {
+  if (L <= 0 || d <= 0) return hipSuccess;
+  hipLaunchKernelGGL(list_means_kernel, dim3((unsigned)L), dim3(256), 0, st, X, ldx, d, sorted_rows, offsets, counts,
+                     cents);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void scatter_rperm_kernel(const uint32_t* sorted_rows, const uint32_t* sorted_list,
+                                                            int64_t n, const int64_t* offsets,
+                                                            const int64_t* pad_offsets, uint32_t* rperm) {
+  const int64_t s = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t j = sorted_list[s];
+  rperm[pad_offsets[j] + (s - offsets[j])] = sorted_rows[s];
+}
+
+hipError_t launch_scatter_rperm(const uint32_t* sorted_rows, const uint32_t* sorted_list, int64_t n,
+                                const int64_t* offsets, const int64_t* pad_offsets, uint32_t* rperm,
+                                hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(scatter_rperm_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, sorted_rows,
+                     sorted_list, n, offsets, pad_offsets, rperm);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void iota_u32_kernel(uint32_t* p, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i < n) p[i] = (uint32_t)i;
+}
+
+hipError_t launch_iota_u32(uint32_t* p, int64_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(iota_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+// One wave per padded position.
+__global__ __launch_bounds__(256) void prep_resid_kernel(PrepResidArgs a) {
+  const int64_t pos = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (pos >= a.n_rpad) return;
+  const int lane = __lane_id();
+  uint16_t* dst = a.Xop + size_t(pos) * a.Kp;
+  const uint32_t row = a.rperm[pos];
+  if (row == RPERM_NONE) {
+    for (int c = lane; c < a.Kp; c += 64) dst[c] = 0;
+    if (lane == 0) a.hx[pos] = MFMA_PAD_HXN;
+    return;
+  }
+  const int j = a.tile_list[pos >> 6];
+  const float* x = a.X + int64_t(row) * a.ldx;
+  const float* cj = a.cents + size_t(j) * a.ldc;
+  double s2 = 0.0, e2 = 0.0, t2 = 0.0, x2 = 0.0;
+  for (int c = lane; c < a.Kp; c += 64) {
+    const double xv = c < a.d ? (double)x[c] : 0.0;
+    x2 += xv * xv;
+    const double r = c < a.d ? xv - (double)cj[c] : 0.0;
+    const uint16_t h = bf16_rne((float)r);
+    dst[c] = h;
+    const double rt = (double)bf16_val(h);
+    s2 += r * r;
+    e2 += (rt - r) * (rt - r);
+    t2 += rt * rt;
+  }
+  s2 = wave_sum(s2);
+  e2 = wave_sum(e2);
+  t2 = wave_sum(t2);
+  x2 = wave_sum(x2);
+  if (lane == 0) {
+    atomic_max_nonneg(&a.gconsts[0], x2 * (1.0 + (a.d + 6) * 0x1p-52));
+    const double v = s2 * 0.5;
+    const float hf = (float)v;
+    a.hx[pos] = -hf;
+    const double slack = 1.0 + (a.d + 6) * 0x1p-52;
+    unsigned long long* lc = a.lconsts + size_t(j) * 8;
+    atomic_max_nonneg(&lc[0], s2 * slack);
+    atomic_max_nonneg(&lc[1], e2 * slack);
+    atomic_max_nonneg(&lc[2], t2 * slack);
+    atomic_max_nonneg(&lc[3], fabs((double)hf - v) + v * (a.d + 6) * 0x1p-52);
+    atomic_max_nonneg(&lc[4], (double)hf);
+  }
+}
+
+hipError_t launch_prep_resid(const PrepResidArgs& a, hipStream_t st) {
+  if (a.n_rpad == 0) return hipSuccess;
+  hipLaunchKernelGGL(prep_resid_kernel, dim3((unsigned)((a.n_rpad + 3) / 4)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+// One wave per (list, query slot).  With N = ||q - c|| (upper bound), the B
+// operand b~ = bf16(fl32(q - c)) has ||b~ - (q - c)|| <= EA = N (2^-9 + 2^-23)
+// and ||b~|| <= N + EA; per list EX = max ||r~ - r||, RN = max ||r||, RT = max
+// ||r~||, EH = max |fl32(||r||^2/2) - ||r||^2/2|, H = max fl32(||r||^2/2).
+//   operand rounding   N EX + EA RN + EA EX
+//   hxn rounding       EH
+//   accumulation       4 (d + 2) u (|b~| RT + H + Q2 + e)   (C input included)
+//   C-input addition   u (H + Q2 + e)
+//   underflow          2 (d + 2) 2^-126
+// e solves e >= base + (4 (d + 2) u + u) e; the term is -Q2lo + e rounded up,
+// Q2lo <= ||q - c||^2 / 2 <= Q2.
+__global__ __launch_bounds__(256) void prep_qlist_kernel(QListArgs a) {
+  const int64_t w = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (w >= int64_t(a.L) * a.nq_pad) return;
+  const int j = (int)(w / a.nq_pad);
+  const int64_t q = w - int64_t(j) * a.nq_pad;
+  const int lane = __lane_id();
+  const float* qr = a.Q32 + (q < a.m ? q : a.m - 1) * a.ld32;
+  const float* cj = a.cents + size_t(j) * a.ldc;
+  double s2 = 0.0;
+  for (int c = lane; c < a.d; c += 64) {
+    const double t = (double)qr[c] - (double)cj[c];
+    s2 += t * t;
+  }
+  s2 = wave_sum(s2);
+  if (lane != 0) return;
+  const double u = 0x1p-24;
+  const int d = a.d;
+  const double sl = (d + 8) * 0x1p-52;
+  const unsigned long long* lc = a.lconsts + size_t(j) * 8;
+  const double RN = sqrt(__longlong_as_double((long long)lc[0])) * (1 + 0x1p-50);
+  const double EX = sqrt(__longlong_as_double((long long)lc[1])) * (1 + 0x1p-50);
+  const double RT = sqrt(__longlong_as_double((long long)lc[2])) * (1 + 0x1p-50);
+  const double EH = __longlong_as_double((long long)lc[3]);
+  const double H = __longlong_as_double((long long)lc[4]);
+  const double N = sqrt(s2 * (1 + sl)) * (1 + 0x1p-50);
+  const double EA = N * (0x1p-9 + 0x1p-23) * (1 + 0x1p-40);
+  const double B = (N + EA) * (1 + 0x1p-50);
+  const double Q2 = 0.5 * s2 * (1 + sl);
+  const double Q2lo = 0.5 * s2 * (1 - sl);
+  const double gacc = 4.0 * (d + 2) * u;
+  const double base = N * EX + EA * RN + EA * EX + EH + gacc * (B * RT + H + Q2) + u * (H + Q2) +
+                      2.0 * (d + 2) * 0x1p-126;
+  const double e = base / (1.0 - gacc - u) * (1 + 0x1p-40) + 0x1p-140;
+  const double t = -Q2lo + e;
+  float tf = (float)t;
+  if ((double)tf < t) tf = nextafterf(tf, INFINITY);
+  a.qtab[w] = tf;
+}
+
+hipError_t launch_prep_qlist(const QListArgs& a, hipStream_t st) {
+  const int64_t waves = int64_t(a.L) * a.nq_pad;
+  if (waves == 0 || a.m == 0) return hipSuccess;
+  hipLaunchKernelGGL(prep_qlist_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+}  // namespace fa

@@ -91,6 +91,12 @@ __device__ __forceinline__ void lds_dma4(uint32_t lds_base, const void* src) {
       : "memory");
 }
 
+__device__ __forceinline__ uint16_t bf16_bits_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
 __device__ __forceinline__ int lane_prefix(uint64_t m) {
   return (int)__builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0));
 }
@@ -177,7 +183,7 @@ __device__ __forceinline__ int merge_select(uint64_t* __restrict__ slot, int KP,
     }                                                                                     \
   } while (0)
 
-template <int KIND, int KS, int A, int R, int V>
+template <int KIND, int KS, int A, int R, int V, bool RESID = false>
 __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int qb, const int split) {
   using C = Comp<float>;
   constexpr int NS = 64 * V;
@@ -208,13 +214,20 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
   const size_t kp_bytes = size_t(KS) * 32;
   const char* xop = reinterpret_cast<const char*>(a.Xop);
 
-  // query fragments, resident for the whole scan
+  // query fragments, resident for the whole scan (residual operand: rebuilt at
+  // every change of list, below)
   uint4 qf[A][KS];
+  float qt[A];
 #pragma unroll
-  for (int g = 0; g < A; ++g) {
-    const char* qrow = reinterpret_cast<const char*>(a.Qop) + size_t(q0 + g * 32 + col) * kp_bytes + half * 16;
+  for (int g = 0; g < A; ++g) qt[g] = 0.0f;
+  int cur_list = -1;
+  if constexpr (!RESID) {
 #pragma unroll
-    for (int s = 0; s < KS; ++s) qf[g][s] = *reinterpret_cast<const uint4*>(qrow + s * 32);
+    for (int g = 0; g < A; ++g) {
+      const char* qrow = reinterpret_cast<const char*>(a.Qop) + size_t(q0 + g * 32 + col) * kp_bytes + half * 16;
+#pragma unroll
+      for (int s = 0; s < KS; ++s) qf[g][s] = *reinterpret_cast<const uint4*>(qrow + s * 32);
+    }
   }
   float thr[A];
   int cnt[A], lcnt[A];
@@ -266,6 +279,34 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
   for (int64_t t = r0; t < r1; t += BN) {
     const char* tb = tile[buf];
     const float* hb = tile_hx[buf];
+    if constexpr (RESID) {
+      const int jl = __builtin_amdgcn_readfirstlane(a.tile_list[t >> 6]);
+      if (jl != cur_list) {
+        // B operand of this list: bf16(fl32(q - c_j)) of the lane's 8 coordinates
+        // per K step (16 s + 8 half ...), zero beyond d; padding query slots
+        // repeat the last query (their results are never read)
+        cur_list = jl;
+        const float* cj = a.cents + size_t(jl) * a.ldc;
+#pragma unroll
+        for (int g = 0; g < A; ++g) {
+          const int64_t qs = q0 + g * 32 + col;
+          const float* qv = a.Q32 + (qs < a.m ? qs : a.m - 1) * a.ld32;
+#pragma unroll
+          for (int s = 0; s < KS; ++s) {
+            uint32_t wv[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const int k = s * 16 + half * 8 + 2 * e;
+              const float v0 = k < a.d ? qv[k] - cj[k] : 0.0f;
+              const float v1 = k + 1 < a.d ? qv[k + 1] - cj[k + 1] : 0.0f;
+              wv[e] = uint32_t(bf16_bits_rne(v0)) | (uint32_t(bf16_bits_rne(v1)) << 16);
+            }
+            qf[g][s] = make_uint4(wv[0], wv[1], wv[2], wv[3]);
+          }
+          qt[g] = a.qtab[size_t(jl) * a.nq_pad + qs];
+        }
+      }
+    }
     // C input: hxn of the 16 rows this lane holds, shared by the query groups
     floatx16 acc[A][R];
 #pragma unroll
@@ -279,8 +320,15 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
         h[q4 * 4 + 2] = h4.z;
         h[q4 * 4 + 3] = h4.w;
       }
+      if constexpr (RESID) {
 #pragma unroll
-      for (int g = 0; g < A; ++g) acc[g][rb] = h;
+        for (int g = 0; g < A; ++g)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) acc[g][rb][r] = h[r] + qt[g];
+      } else {
+#pragma unroll
+        for (int g = 0; g < A; ++g) acc[g][rb] = h;
+      }
     }
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
@@ -356,13 +404,15 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
 // selection epilogue); wider shapes keep one wave per SIMD and no spills.
 // Split-major order: all query blocks of split 0 start first, so that the
 // later splits of a query begin with the thresholds the earlier ones reached.
-template <int KIND, int KS, int A, int R, int V>
+template <int KIND, int KS, int A, int R, int V, bool RESID>
 __global__ __launch_bounds__(256, 2) void scan_mfma_kernel(ScanMfmaArgs a) {
-  scan_mfma_body<KIND, KS, A, R, V>(a, (int)(blockIdx.x % (unsigned)a.nqb), (int)(blockIdx.x / (unsigned)a.nqb));
+  scan_mfma_body<KIND, KS, A, R, V, RESID>(a, (int)(blockIdx.x % (unsigned)a.nqb),
+                                           (int)(blockIdx.x / (unsigned)a.nqb));
 }
-template <int KIND, int KS, int A, int R, int V>
+template <int KIND, int KS, int A, int R, int V, bool RESID>
 __global__ __launch_bounds__(256) void scan_mfma_kernel_wide(ScanMfmaArgs a) {
-  scan_mfma_body<KIND, KS, A, R, V>(a, (int)(blockIdx.x % (unsigned)a.nqb), (int)(blockIdx.x / (unsigned)a.nqb));
+  scan_mfma_body<KIND, KS, A, R, V, RESID>(a, (int)(blockIdx.x % (unsigned)a.nqb),
+                                           (int)(blockIdx.x / (unsigned)a.nqb));
 }
 
 // Persistent workgroups over the work items of many problems.  One lane takes
@@ -422,7 +472,7 @@ int mfma_queries_per_block(int kind, int ksteps) { return 4 * 32 * (ksteps <= 16
 int mfma_rows_per_tile(int kind, int ksteps) { return 32 * (ksteps <= 16 ? 2 : 1); }
 int mfma_workgroups_per_cu(int kind, int ksteps) { return (kind == 0 ? ksteps <= 12 : ksteps <= 8) ? 2 : 1; }
 
-template <int KIND, int KS>
+template <int KIND, int KS, bool RESID>
 static hipError_t launch_ks(const ScanMfmaArgs& a, int V, int nsplit, hipStream_t st) {
   constexpr int A = KS <= 16 ? 2 : 1;
   constexpr int R = KS <= 16 ? 2 : 1;
@@ -430,14 +480,14 @@ static hipError_t launch_ks(const ScanMfmaArgs& a, int V, int nsplit, hipStream_
   const dim3 grid((unsigned)(a.nqb * nsplit)), block(256);
   if constexpr (two_waves) {
     if (V == 4)
-      hipLaunchKernelGGL((scan_mfma_kernel<KIND, KS, A, R, 4>), grid, block, 0, st, a);
+      hipLaunchKernelGGL((scan_mfma_kernel<KIND, KS, A, R, 4, RESID>), grid, block, 0, st, a);
     else
-      hipLaunchKernelGGL((scan_mfma_kernel<KIND, KS, A, R, 8>), grid, block, 0, st, a);
+      hipLaunchKernelGGL((scan_mfma_kernel<KIND, KS, A, R, 8, RESID>), grid, block, 0, st, a);
   } else {
     if (V == 4)
-      hipLaunchKernelGGL((scan_mfma_kernel_wide<KIND, KS, A, R, 4>), grid, block, 0, st, a);
+      hipLaunchKernelGGL((scan_mfma_kernel_wide<KIND, KS, A, R, 4, RESID>), grid, block, 0, st, a);
     else
-      hipLaunchKernelGGL((scan_mfma_kernel_wide<KIND, KS, A, R, 8>), grid, block, 0, st, a);
+      hipLaunchKernelGGL((scan_mfma_kernel_wide<KIND, KS, A, R, 8, RESID>), grid, block, 0, st, a);
   }
   return hipGetLastError();
 }
@@ -489,13 +539,28 @@ hipError_t launch_scan_mfma_batch(const ScanBatchArgs& b, int kind, int ks, int 
   }
 }
 
+int mfma_resid_max_ksteps() { return 48; }
+
 hipError_t launch_scan_mfma(const ScanMfmaArgs& a, int kind, int ks, int V, int nsplit, hipStream_t st) {
   if (V != 4 && V != 8) return hipErrorInvalidValue;
+  if (kind == 0 && a.resid) {
+    switch (ks) {
+#define KS_CASE(K) \
+  case K:          \
+    return launch_ks<0, K, true>(a, V, nsplit, st);
+      KS_CASE(1) KS_CASE(2) KS_CASE(3) KS_CASE(4) KS_CASE(5) KS_CASE(6) KS_CASE(7) KS_CASE(8) KS_CASE(9)
+      KS_CASE(10) KS_CASE(12) KS_CASE(14) KS_CASE(16) KS_CASE(20) KS_CASE(24) KS_CASE(28) KS_CASE(32)
+      KS_CASE(40) KS_CASE(48)
+#undef KS_CASE
+      default:
+        return hipErrorInvalidValue;
+    }
+  }
   if (kind == 0) {
     switch (ks) {
 #define KS_CASE(K) \
   case K:          \
-    return launch_ks<0, K>(a, V, nsplit, st);
+    return launch_ks<0, K, false>(a, V, nsplit, st);
       KS_CASE(1) KS_CASE(2) KS_CASE(3) KS_CASE(4) KS_CASE(5) KS_CASE(6) KS_CASE(7) KS_CASE(8) KS_CASE(9)
       KS_CASE(10) KS_CASE(12) KS_CASE(14) KS_CASE(16) KS_CASE(20) KS_CASE(24) KS_CASE(28) KS_CASE(32)
       KS_CASE(40) KS_CASE(48) KS_CASE(56) KS_CASE(64)
@@ -507,7 +572,7 @@ hipError_t launch_scan_mfma(const ScanMfmaArgs& a, int kind, int ks, int V, int 
   switch (ks) {
 #define KS_CASE(K) \
   case K:          \
-    return launch_ks<1, K>(a, V, nsplit, st);
+    return launch_ks<1, K, false>(a, V, nsplit, st);
     KS_CASE(1) KS_CASE(2) KS_CASE(3) KS_CASE(4) KS_CASE(6) KS_CASE(8) KS_CASE(12) KS_CASE(16) KS_CASE(20)
     KS_CASE(24) KS_CASE(32)
 #undef KS_CASE

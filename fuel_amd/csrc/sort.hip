@@ -1,6 +1,7 @@
 // rocPRIM-backed helpers: exclusive scan of per-(query, split) counts and the
 // stable segmented sort that orders range results by (distance, index).
 #include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 #include <rocprim/device/device_segmented_radix_sort.hpp>
 
@@ -30,6 +31,13 @@ hipError_t segmented_sort_pairs(const K* keys_in, K* keys_out, const int64_t* va
   return settle(rocprim::segmented_radix_sort_pairs(tmp, tmp_bytes, keys_in, keys_out, vals_in, vals_out,
                                                     (unsigned int)nnz, (unsigned int)nseg, offsets, offsets + 1, 0,
                                                     (unsigned int)(8 * sizeof(K)), st));
+}
+
+// Stable by construction (LSD radix sort): rows of one list keep their order.
+hipError_t sort_pairs_u32(const uint32_t* keys_in, uint32_t* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
+                          int64_t n, int bits, void* tmp, size_t& tmp_bytes, hipStream_t st) {
+  return settle(rocprim::radix_sort_pairs(tmp, tmp_bytes, keys_in, keys_out, vals_in, vals_out, (size_t)n, 0,
+                                          bits, st));
 }
 
 template hipError_t segmented_sort_pairs<uint32_t>(const uint32_t*, uint32_t*, const int64_t*, int64_t*, int64_t,

@@ -59,6 +59,9 @@ extern "C" {
 #define FUEL_AMD_MODE_EXACT 1 /* restated distance for every pair */
 #define FUEL_AMD_MODE_FP32 2  /* FP32 MFMA ||x||^2-2x.y candidates + exact rerank */
 #define FUEL_AMD_MODE_BF16 3  /* BF16 MFMA candidates + exact FP32 rerank + FP64 bound */
+#define FUEL_AMD_MODE_BF16R 4 /* as BF16, on residuals against coarse centroids: rows partitioned
+                                 into lists at index build; for clustered data, where the plain
+                                 form's bound (~ |q| |x|) is wider than the neighbour gaps */
 
 /* pairwise output forms (fuel: compute_pairwise_dense / _condensed) */
 #define FUEL_AMD_FORM_SQUARE 0

@@ -11,7 +11,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-MODES = ["bf16", "fp32"]
+MODES = ["bf16", "fp32", "bf16r"]
 
 
 def _same(a, b):
@@ -37,10 +37,11 @@ def test_c1_like(gpu, oracle, mode, exclude):
     from fuel_amd import workloads
     X = workloads.c1(n=5000)
     # clusters far from the origin: plain bf16 rounding (relative to |x|) is too
-    # coarse to certify most queries, so only the f32 form is held to a rate
-    frac = 0.02 if mode == "fp32" else None
+    # coarse to certify most queries; the f32 form and the residual operand
+    # (rounding relative to the distance to a centroid) are held to a rate
+    frac = {"fp32": 0.02, "bf16r": 0.10}.get(mode)
     st = _run(gpu, oracle, X, X, 10, "euclidean", exclude, mode, max_fallback_frac=frac)
-    assert st["mode_used"] == (3 if mode == "bf16" else 2)
+    assert st["mode_used"] == {"bf16": 3, "fp32": 2, "bf16r": 4}[mode]
 
 
 @pytest.mark.parametrize("mode", MODES)
