@@ -57,6 +57,8 @@ def parse():
     p.add_argument("--serve-mode", default="bf16r", choices=["bf16", "bf16r", "fp32", "exact"],
                    help="candidate mode of the served index (c5)")
     p.add_argument("--serve-seconds", type=float, default=4.0, help="seconds of arrivals per rate (c5)")
+    p.add_argument("--serve-p99-seconds", type=float, default=3.0,
+                   help="seconds of serving at 10,000 queries/s measured for the headline line's p99 (0: skip)")
     p.add_argument("--serve-rows", type=int, default=10_000_000, help="rows of the served index (c5)")
     p.add_argument("--host-path", action="store_true",
                    help="only time the host-pointer entry point (queries and results cross PCIe each call)")
@@ -212,6 +214,33 @@ def serve_run(srv, pool, lam, seconds, rng, k=10, mean_batch=64, cap=4096, keep=
             "max_launch_queries": st["max_batch_queries"], "gpu_busy_frac": round(st["busy_s"] / elapsed, 3)}, kept
 
 
+def c5_data(args):
+    """The serving index rows and a pool of queries from the same mixture."""
+    from fuel_amd import workloads
+    X, means = workloads.c5(n=args.serve_rows)
+    rng = np.random.default_rng(66)
+    P = 65536
+    pool = (means[rng.integers(0, means.shape[0], P)] + rng.normal(0.0, 1.0, (P, X.shape[1]))).astype(np.float32)
+    return X, pool, rng
+
+
+def serve_headline(args):
+    """p50/p99 request latency of the serving loop on the serving configuration
+    at one arrival rate (10,000 queries/s), for the headline line."""
+    from fuel_amd import Server, _lib
+    X, pool, rng = c5_data(args)
+    ix = _lib.Index(X, "euclidean", mode=args.serve_mode)
+    with Server(ix, workers=2, max_batch=4096) as srv:
+        serve_run(srv, pool, 2000, 0.5, rng)  # warm-up
+        r, _ = serve_run(srv, pool, 10000, args.serve_p99_seconds, rng)
+    ix.close()
+    r = dict(r)
+    r.update({"workload": f"serving: resident index {X.shape[0]:,} x {X.shape[1]} f32 (50-component mixture), "
+                          "k=10, Poisson arrivals, geometric batch sizes (mean 64, cap 4096)",
+              "mode": args.serve_mode, "seconds": args.serve_p99_seconds})
+    return r
+
+
 def secondary(args):
     """Lines for the other configurations and the widened rows: GPU time beside
     the oracle's CPU time on the same inputs, parity checked on the way."""
@@ -303,6 +332,9 @@ def secondary(args):
                      "rows": int(sum(X.shape[0] for X in probs)), "algorithmic_flop": flop,
                      "ms_per_pass": round(ms, 2), "steps": args.steps, "warmup": args.warmup,
                      "plan_seconds": round(tune_s, 1), "plan": info["groups"], "graph": info["graph"],
+                     "tuning_best_ms": {str(dd): {m: min(t["ms"] for t in info["tuning"] if t["d"] == dd and t["mode"] == m)
+                                                  for m in {t["mode"] for t in info["tuning"] if t["d"] == dd}}
+                                        for dd in sorted({t["d"] for t in info["tuning"]})},
                      "fallback_queries": info["last_fallback_queries"], "exact_queries": info["last_exact_queries"],
                      "roofline": {"bound": "mfma", "achieved": round(achieved / 1e12, 2),
                                   "peak": PEAKS["bf16"][0] / 1e12, "unit": "TFLOP/s",
@@ -317,10 +349,7 @@ def secondary(args):
     elif args.workload == "c5":
         from fuel_amd import Server
         t0 = time.time()
-        X, means = workloads.c5(n=args.serve_rows)
-        rng = np.random.default_rng(66)
-        P = 65536
-        pool = (means[rng.integers(0, means.shape[0], P)] + rng.normal(0.0, 1.0, (P, X.shape[1]))).astype(np.float32)
+        X, pool, rng = c5_data(args)
         ix = _lib.Index(X, "euclidean", mode=args.serve_mode)
         print(f"# c5 data + index in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
         rates, kept = [], []
@@ -507,6 +536,11 @@ def main():
                "sample": f"{S} of the {m} queries against the full {n}x{d} database, k={k}, "
                          f"{c1 - c0:.1f}s; restated AVX2 kernel + KNNHeap, one thread per query batch"}
 
+    serve = None
+    if args.serve_p99_seconds > 0:
+        serve = serve_headline(args)
+        print(f"# serve: {serve}", file=sys.stderr, flush=True)
+
     stats = main_run["stats"]
     line = {
         "metric": METRIC,
@@ -528,6 +562,8 @@ def main():
         "fp32": fp32,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "p99_serve_ms": serve["p99_ms"] if serve else None,
+        "serve": serve,
     }
     print(json.dumps(line), flush=True)
 

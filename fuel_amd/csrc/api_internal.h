@@ -53,7 +53,7 @@ hipError_t upload_rows(void* dst, int64_t ld, const void* host, int64_t rows, in
 // against one resident index (whose rows and operand copies are read-only).
 struct QueryCtx {
   struct LevelWs {  // one per level of the certification chain
-    DevBuf Qop, Q32, qnorm, qint, slots, lcnts, fail_count, fail_list, fail_theta, gthr, qtab;
+    DevBuf Qop, Q32, qnorm, qint, slots, lcnts, fail_count, fail_list, fail_theta, gthr, qtab, qs2;
   } lv[3];
   DevBuf slots, lcnts;  // exact scan
   fuel_amd_stats stats{};
@@ -71,6 +71,7 @@ struct QueryCtx {
       w.fail_theta.release();
       w.gthr.release();
       w.qtab.release();
+      w.qs2.release();
     }
     slots.release();
     lcnts.release();
@@ -170,6 +171,10 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
                  int kind, int64_t* out_idx, float* out_dist, hipStream_t st, const int64_t* d_rows = nullptr,
                  int64_t qbase = 0, int level = 0, int kp = 0, const float* d_theta = nullptr);
 bool mfma_supported(const fuel_amd_index* ix, int kind, int k, int exclude);
+// Residual operand layout of n rows (f32, pitch ld): centroids (L x d), padded
+// position -> row (RPERM_NONE for padding) and the list of every 64-row block.
+int32_t build_partition(const float* X, int64_t ld, int64_t n, int d, hipStream_t st, DevBuf& cents,
+                        DevBuf& rperm, DevBuf& tile_list, int& L, int64_t& n_rpad);
 // Exact path on an explicit list of query rows (the certification fallback).
 int32_t knn_exact_rows(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t ldq, const int64_t* d_rows,
                        int64_t nrows, int64_t qbase, int k, int exclude, int64_t* out_idx, float* out_dist,

@@ -43,7 +43,7 @@ constexpr int KP_LIMIT = 128;  // widest kept list of the MFMA path
 using Clock = std::chrono::steady_clock;
 
 struct Cfg {
-  int kind = 0;  // 0 bf16 MFMA, 1 f32 MFMA, -1 exact scan
+  int kind = 0;  // 0 bf16 MFMA, 1 f32 MFMA, 2 bf16 MFMA on the residual operand, -1 exact scan
   int widen = 4;
   int fill = 1;  // work items per persistent workgroup slot, / 4
 };
@@ -57,8 +57,16 @@ struct ProbPlan {
   int64_t n_pad = 0, nqb = 0, nq_pad = 0, rps = 0;
   int nsplit = 0;
   size_t o_xop = 0, o_hx = 0, o_consts = 0, o_qop = 0, o_q32 = 0, o_qn = 0, o_qint = 0, o_gthr = 0, o_slots = 0,
-         o_lcnts = 0;      // level-0 pool (shared by the groups in turn)
+         o_lcnts = 0, o_lconsts = 0, o_qtab = 0, o_qs2 = 0;  // level-0 pool (shared by the groups in turn)
   size_t o_fl = 0, o_ft = 0;  // kept pool: certification failures of every problem
+};
+
+// Residual layout of one problem (plan time; the graph rebuilds the operand
+// from the current rows over it).
+struct ResidPart {
+  DevBuf cents, rperm, tile_list;
+  int L = 0;
+  int64_t n_rpad = 0;
 };
 
 struct Bucket {
@@ -79,6 +87,7 @@ struct fuel_amd_batch {
   bool use_graph = true;
   std::vector<fuel_amd_problem> probs;
   std::vector<char> force_exact;  // norms beyond the MFMA scan's sentinels
+  std::vector<std::unique_ptr<fa::ResidPart>> parts;  // residual layouts, per problem
   std::vector<fa::ProbPlan> plan;
   std::deque<fa::Bucket> buckets;  // in place: a group owns device buffers
   hipStream_t stream = nullptr;
@@ -122,7 +131,9 @@ int32_t plan_batch(fuel_amd_batch* b, const std::map<int, Cfg>& cfg_by_d) {
     pp.KK = pr.k + (b->exclude != EXCL_NONE ? 1 : 0);
     auto it = cfg_by_d.find(pr.d);
     const Cfg c = it != cfg_by_d.end() ? it->second : Cfg{};
-    if (c.kind < 0 || b->force_exact[i] || pp.KK > KP_LIMIT || mfma_ksteps_bucket(c.kind, pr.d) < 0 || pr.d == 0) {
+    const int ks = c.kind < 0 ? -1 : mfma_ksteps_bucket(c.kind == 2 ? 0 : c.kind, pr.d);
+    if (c.kind < 0 || b->force_exact[i] || pp.KK > KP_LIMIT || ks < 0 || pr.d == 0 ||
+        (c.kind == 2 && ks > mfma_resid_max_ksteps())) {
       pp.exact = true;
       continue;
     }
@@ -148,15 +159,17 @@ int32_t plan_batch(fuel_amd_batch* b, const std::map<int, Cfg>& cfg_by_d) {
     bk.d = kv.first;
     bk.cfg = cfg_by_d.at(kv.first);
     const int kind = bk.cfg.kind;
-    bk.ks = mfma_ksteps_bucket(kind, bk.d);
+    const bool resid = kind == 2;
+    const int mk = resid ? 0 : kind;  // MFMA operand type
+    bk.ks = mfma_ksteps_bucket(mk, bk.d);
     bk.members = kv.second;
     std::sort(bk.members.begin(), bk.members.end(),
               [&](int x, int y) { return b->probs[x].n > b->probs[y].n || (b->probs[x].n == b->probs[y].n && x < y); });
-    const int qpb = mfma_queries_per_block(kind, bk.ks);
-    const int rows_tile = mfma_rows_per_tile(kind, bk.ks);
-    const int64_t slots = int64_t(cus) * mfma_workgroups_per_cu(kind, bk.ks);
-    const int Kp = bk.ks * (kind == 0 ? 16 : 8);
-    const size_t es = kind == 0 ? 2 : 4;
+    const int qpb = mfma_queries_per_block(mk, bk.ks);
+    const int rows_tile = mfma_rows_per_tile(mk, bk.ks);
+    const int64_t slots = int64_t(cus) * mfma_workgroups_per_cu(mk, bk.ks);
+    const int Kp = bk.ks * (mk == 0 ? 16 : 8);
+    const size_t es = mk == 0 ? 2 : 4;
     double work = 0.0;
     bk.V = 4;
     for (int i : bk.members) {
@@ -169,6 +182,16 @@ int32_t plan_batch(fuel_amd_batch* b, const std::map<int, Cfg>& cfg_by_d) {
       pp.KP = std::max(pp.KK, std::min(KP_LIMIT, bk.cfg.widen * pr.k));
       if (pp.KP > 64) bk.V = 8;
       pp.n_pad = std::max<int64_t>(256, (pr.n + 255) & ~int64_t(255));
+      if (resid) {
+        if (!b->parts[i]) {
+          auto part = std::make_unique<ResidPart>();
+          int32_t rc = build_partition(static_cast<const float*>(pr.data), pr.ld, pr.n, pr.d, b->stream, part->cents,
+                                       part->rperm, part->tile_list, part->L, part->n_rpad);
+          if (rc) return rc;
+          b->parts[i] = std::move(part);
+        }
+        pp.n_pad = b->parts[i]->n_rpad;
+      }
       pp.nqb = (pr.n + qpb - 1) / qpb;
       pp.nq_pad = pp.nqb * qpb;
       work += double(pp.nqb) * double(pp.n_pad);
@@ -192,8 +215,18 @@ int32_t plan_batch(fuel_amd_batch* b, const std::map<int, Cfg>& cfg_by_d) {
       off += align256(size_t(pp.n_pad) * 4);
       pp.o_consts = off;
       off += 256;
-      pp.o_qop = off;
-      off += align256(size_t(pp.nq_pad) * Kp * es);
+      if (resid) {
+        const int L = b->parts[i]->L;
+        pp.o_lconsts = off;
+        off += align256(size_t(L) * 64);
+        pp.o_qtab = off;
+        off += align256(size_t(L) * pp.nq_pad * 4);
+        pp.o_qs2 = off;
+        off += align256(size_t(L) * pr.n * 4);
+      } else {
+        pp.o_qop = off;
+        off += align256(size_t(pp.nq_pad) * Kp * es);
+      }
       pp.o_q32 = off;
       off += align256(size_t(pr.n) * pr.ld * 4);
       pp.o_qn = off;
@@ -228,12 +261,25 @@ int32_t plan_batch(fuel_amd_batch* b, const std::map<int, Cfg>& cfg_by_d) {
     std::vector<ScanMfmaArgs> tab;
     for (int i : B.members) {
       const auto& pp = b->plan[i];
+      const auto& pr = b->probs[i];
       ScanMfmaArgs sa{};
       sa.Xop = at<void>(b->pool, pp.o_xop);
       sa.hx = at<float>(b->pool, pp.o_hx);
       sa.Kp = pp.Kp;
       sa.n_pad = pp.n_pad;
-      sa.Qop = at<void>(b->pool, pp.o_qop);
+      sa.Qop = pp.kind == 2 ? nullptr : at<void>(b->pool, pp.o_qop);
+      if (pp.kind == 2) {
+        const auto& part = *b->parts[i];
+        sa.resid = 1;
+        sa.tile_list = part.tile_list.as<int>();
+        sa.cents = part.cents.as<float>();
+        sa.ldc = pr.d;
+        sa.Q32 = at<float>(b->pool, pp.o_q32);
+        sa.ld32 = pr.ld;
+        sa.m = pr.n;
+        sa.d = pr.d;
+        sa.qtab = at<float>(b->pool, pp.o_qtab);
+      }
       sa.nq_pad = pp.nq_pad;
       sa.nqb = (int)pp.nqb;
       sa.rows_per_split = pp.rps;
@@ -271,16 +317,45 @@ PrepDbArgs prep_db_args(fuel_amd_batch* b, int i) {
   return pa;
 }
 
+// Operand copy of problem i (and its bound constants, reset first).
+hipError_t enqueue_operand(fuel_amd_batch* b, int i, hipStream_t st) {
+  const auto& pp = b->plan[i];
+  const auto& pr = b->probs[i];
+  const PrepDbArgs pa = prep_db_args(b, i);
+  if (pp.kind != 2) {
+    hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t*>(pa.consts), 16, 0u, st);
+    if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(pa.flags), 2, 1u, st);
+    if (e == hipSuccess) e = launch_prep_db(pa, st);
+    return e;
+  }
+  const auto& part = *b->parts[i];
+  hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t*>(pa.consts), 32, 0u, st);  // constants, zero flags
+  if (e == hipSuccess) e = launch_fill_u32(at<uint32_t>(b->pool, pp.o_lconsts), 16 * part.L, 0u, st);
+  PrepResidArgs r{};
+  r.X = static_cast<const float*>(pr.data);
+  r.ldx = pr.ld;
+  r.d = pr.d;
+  r.Kp = pp.Kp;
+  r.n_rpad = part.n_rpad;
+  r.rperm = part.rperm.as<uint32_t>();
+  r.tile_list = part.tile_list.as<int>();
+  r.cents = part.cents.as<float>();
+  r.ldc = pr.d;
+  r.Xop = at<uint16_t>(b->pool, pp.o_xop);
+  r.hx = at<float>(b->pool, pp.o_hx);
+  r.lconsts = at<unsigned long long>(b->pool, pp.o_lconsts);
+  r.gconsts = pa.consts;
+  if (e == hipSuccess) e = launch_prep_resid(r, st);
+  return e;
+}
+
 // Level 0 of one group on stream st (no host synchronisation: capturable).
 hipError_t enqueue_bucket(fuel_amd_batch* b, const Bucket& B, hipStream_t st) {
   hipError_t e = hipSuccess;
   for (int i : B.members) {
     const auto& pp = b->plan[i];
     const auto& pr = b->probs[i];
-    const PrepDbArgs pa = prep_db_args(b, i);
-    e = launch_fill_u32(reinterpret_cast<uint32_t*>(pa.consts), 16, 0u, st);
-    if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(pa.flags), 2, 1u, st);
-    if (e == hipSuccess) e = launch_prep_db(pa, st);
+    e = enqueue_operand(b, i, st);
     PrepQArgs qa{};
     qa.Q = static_cast<const float*>(pr.data);
     qa.ldq = pr.ld;
@@ -288,8 +363,8 @@ hipError_t enqueue_bucket(fuel_amd_batch* b, const Bucket& B, hipStream_t st) {
     qa.m_pad = pp.nq_pad;
     qa.d = pr.d;
     qa.Kp = pp.Kp;
-    qa.bf16 = pp.kind == 0;
-    qa.Qop = at<void>(b->pool, pp.o_qop);
+    qa.bf16 = pp.kind != 1;
+    qa.Qop = pp.kind == 2 ? nullptr : at<void>(b->pool, pp.o_qop);
     qa.Q32 = at<float>(b->pool, pp.o_q32);
     qa.ld32 = pr.ld;
     qa.qn2 = at<double>(b->pool, pp.o_qn);
@@ -298,6 +373,28 @@ hipError_t enqueue_bucket(fuel_amd_batch* b, const Bucket& B, hipStream_t st) {
     qa.qint = at<int>(b->pool, pp.o_qint);
     qa.gthr = at<uint32_t>(b->pool, pp.o_gthr);
     if (e == hipSuccess) e = launch_prep_queries(qa, st);
+    if (pp.kind == 2 && e == hipSuccess) {
+      KmeansPairwiseArgs<float> pw{};
+      pw.P1 = qa.Q32;
+      pw.ld1 = pr.ld;
+      pw.m = pr.n;
+      pw.P2 = b->parts[i]->cents.as<float>();
+      pw.ld2 = pr.d;
+      pw.n = b->parts[i]->L;
+      pw.d = pr.d;
+      pw.out = at<float>(b->pool, pp.o_qs2);
+      pw.ldo = b->parts[i]->L;
+      e = launch_kmeans_pairwise<float>(pw, st);
+      QListArgs la{};
+      la.s2 = pw.out;
+      la.m = pr.n;
+      la.nq_pad = pp.nq_pad;
+      la.d = pr.d;
+      la.L = b->parts[i]->L;
+      la.lconsts = at<unsigned long long>(b->pool, pp.o_lconsts);
+      la.qtab = at<float>(b->pool, pp.o_qtab);
+      if (e == hipSuccess) e = launch_prep_qlist(la, st);
+    }
     if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(b->fail_counts.as<int>() + i), 1, 0u, st);
     if (e != hipSuccess) return e;
   }
@@ -308,7 +405,8 @@ hipError_t enqueue_bucket(fuel_amd_batch* b, const Bucket& B, hipStream_t st) {
   sb.nprob = (int)B.members.size();
   sb.nitems = B.nitems;
   sb.counter = b->counter.as<unsigned int>();
-  if (e == hipSuccess) e = launch_scan_mfma_batch(sb, B.cfg.kind, B.ks, B.V, B.grid, st);
+  if (e == hipSuccess)
+    e = launch_scan_mfma_batch(sb, B.cfg.kind == 2 ? 0 : B.cfg.kind, B.ks, B.V, B.grid, B.cfg.kind == 2, st);
   for (int i : B.members) {
     if (e != hipSuccess) return e;
     const auto& pp = b->plan[i];
@@ -327,7 +425,9 @@ hipError_t enqueue_bucket(fuel_amd_batch* b, const Bucket& B, hipStream_t st) {
     ra.ld32 = pr.ld;
     ra.d = pr.d;
     ra.metric = b->metric;
-    ra.bf16 = pp.kind == 0;
+    ra.bf16 = pp.kind != 1;
+    ra.resid = pp.kind == 2 ? 1 : 0;
+    ra.rperm = pp.kind == 2 ? b->parts[i]->rperm.as<uint32_t>() : nullptr;
     ra.qn2 = at<double>(b->pool, pp.o_qn);
     ra.eq2 = ra.qn2 + pr.n;
     ra.qt2 = ra.eq2 + pr.n;
@@ -384,9 +484,7 @@ int32_t drop_unusable(fuel_amd_batch* b, bool& changed) {
   for (const auto& B : b->buckets) {
     for (int i : B.members) {
       const PrepDbArgs pa = prep_db_args(b, i);
-      hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t*>(pa.consts), 16, 0u, b->stream);
-      if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(pa.flags), 2, 1u, b->stream);
-      if (e == hipSuccess) e = launch_prep_db(pa, b->stream);
+      hipError_t e = enqueue_operand(b, i, b->stream);
       if (e == hipSuccess)
         e = hipMemcpyAsync(&x2[i], pa.consts, sizeof(unsigned long long), hipMemcpyDeviceToHost, b->stream);
       if (e != hipSuccess) return hip_fail(e, "batch operand check");
@@ -421,11 +519,23 @@ int32_t finish(fuel_amd_batch* b) {
     if (cancelled()) return fail(FUEL_AMD_ERR_INTERRUPTED, "interrupted");
     if (pp.bucket >= 0 && b->h_fail[i] > 0) {
       fallback += b->h_fail[i];
+      // bf16 / f32: the same kind from the certified threshold, then the chain;
+      // residual: the f32 form (its threshold is in other key units), then exact
       point_shell(b, i, pp.kind == 0 ? FUEL_AMD_MODE_BF16 : FUEL_AMD_MODE_FP32);
       const int64_t before = b->shell.ctx.stats.exact_queries;
-      int32_t rc = knn_mfma(&b->shell, &b->shell.ctx, static_cast<const float*>(pr.data), b->h_fail[i], pr.ld,
-                            pr.k, b->exclude, pp.kind, pr.out_idx, static_cast<float*>(pr.out_dist), b->stream,
-                            at<int64_t>(b->keep, pp.o_fl), 0, 1, KP_LIMIT, at<float>(b->keep, pp.o_ft));
+      const bool resid = pp.kind == 2;
+      int32_t rc;
+      if (resid && !mfma_supported(&b->shell, 1, pr.k, b->exclude)) {
+        rc = knn_exact_rows(&b->shell, &b->shell.ctx, static_cast<const float*>(pr.data), pr.ld,
+                            at<int64_t>(b->keep, pp.o_fl), b->h_fail[i], 0, pr.k, b->exclude, pr.out_idx,
+                            static_cast<float*>(pr.out_dist), b->stream);
+        b->shell.ctx.stats.exact_queries += b->h_fail[i];
+      } else {
+        rc = knn_mfma(&b->shell, &b->shell.ctx, static_cast<const float*>(pr.data), b->h_fail[i], pr.ld, pr.k,
+                      b->exclude, resid ? 1 : pp.kind, pr.out_idx, static_cast<float*>(pr.out_dist), b->stream,
+                      at<int64_t>(b->keep, pp.o_fl), 0, resid ? 2 : 1, KP_LIMIT,
+                      resid ? nullptr : at<float>(b->keep, pp.o_ft));
+      }
       if (rc) return rc;
       exact += b->shell.ctx.stats.exact_queries - before;
     } else if (pp.exact) {
@@ -466,6 +576,7 @@ int32_t make_batch(const fuel_amd_problem* probs, const std::vector<int>& which,
   b->use_graph = use_graph;
   for (int i : which) b->probs.push_back(probs[i]);
   b->force_exact.assign(b->probs.size(), 0);
+  b->parts.resize(b->probs.size());
   hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
   if (e != hipSuccess) return hip_fail(e, "batch stream");
   b->shell.stream = b->stream;
@@ -492,7 +603,9 @@ int32_t make_batch(const fuel_amd_problem* probs, const std::vector<int>& which,
   return FUEL_AMD_OK;
 }
 
-const char* kind_name(int kind) { return kind == 0 ? "bf16" : (kind == 1 ? "fp32" : "exact"); }
+const char* kind_name(int kind) {
+  return kind == 0 ? "bf16" : (kind == 1 ? "fp32" : (kind == 2 ? "bf16r" : "exact"));
+}
 
 // Times every configuration on a sample of each dimension's problems.
 int32_t autotune(const fuel_amd_problem* probs, int count, int metric, int exclude, int device,
@@ -511,8 +624,9 @@ int32_t autotune(const fuel_amd_problem* probs, int count, int metric, int exclu
     for (size_t j = 0; j < members.size(); j += step) sample.push_back(members[j]);
     double best_ms = 1e300;
     Cfg best_cfg{-1, 4, 1};
-    for (int kind = 0; kind <= 1; ++kind) {
-      if (mfma_ksteps_bucket(kind, d) < 0) continue;
+    for (int kind : {0, 2, 1}) {
+      const int ks = mfma_ksteps_bucket(kind == 2 ? 0 : kind, d);
+      if (ks < 0 || (kind == 2 && ks > mfma_resid_max_ksteps())) continue;
       for (int widen : {2, 4, 8}) {
         for (int fill : {1, 4}) {
           Cfg c{kind, widen, fill};
@@ -560,7 +674,7 @@ int32_t fuel_amd_batch_create(const fuel_amd_problem* problems, int32_t count, i
   if (exclude_self < 0 || exclude_self > 2) return fail(FUEL_AMD_ERR_INVALID, "bad exclude_self");
   fuel_amd_batch_opts o{};
   if (opts) o = *opts;
-  if (o.mode < FUEL_AMD_MODE_AUTO || o.mode > FUEL_AMD_MODE_BF16) return fail(FUEL_AMD_ERR_INVALID, "bad mode");
+  if (o.mode < FUEL_AMD_MODE_AUTO || o.mode > FUEL_AMD_MODE_BF16R) return fail(FUEL_AMD_ERR_INVALID, "bad mode");
   for (int i = 0; i < count; ++i) {
     const auto& p = problems[i];
     if (p.n < 0 || p.d < 0 || p.k < 0) return fail(FUEL_AMD_ERR_INVALID, "problem sizes must be non-negative");
@@ -580,8 +694,15 @@ int32_t fuel_amd_batch_create(const fuel_amd_problem* problems, int32_t count, i
     rc = autotune(problems, count, metric, exclude_self, o.device, cfg, log);
     if (rc) return rc;
   } else {
-    const int kind = o.mode == FUEL_AMD_MODE_BF16 ? 0 : (o.mode == FUEL_AMD_MODE_FP32 ? 1 : -1);
-    for (int i = 0; i < count; ++i) cfg[problems[i].d] = Cfg{kind, o.widen > 0 ? o.widen : 4, 1};
+    const int kind = o.mode == FUEL_AMD_MODE_BF16 ? 0
+                     : o.mode == FUEL_AMD_MODE_FP32 ? 1
+                     : o.mode == FUEL_AMD_MODE_BF16R ? 2 : -1;
+    for (int i = 0; i < count; ++i) {
+      const int d = problems[i].d;
+      // no residual variant beyond the widest one: the plain bf16 form
+      const int k2 = (kind == 2 && mfma_ksteps_bucket(0, d) > mfma_resid_max_ksteps()) ? 0 : kind;
+      cfg[d] = Cfg{k2, o.widen > 0 ? o.widen : 4, 1};
+    }
   }
   std::vector<int> all(count);
   for (int i = 0; i < count; ++i) all[i] = i;

@@ -37,16 +37,13 @@ int kp_for(const fuel_amd_index* ix, int k, int exclude) {
 // they outnumber them several times.
 int resid_lists(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(256, n / 256)); }
 
-// Partition (Lloyd steps from evenly spaced rows), sorted padded layout and the
-// residual operand with its per-list bound constants.
-int32_t prepare_resid(fuel_amd_index* ix, fuel_amd_index::MfmaOperand& op, hipStream_t st) {
-  const int64_t n = ix->n;
-  const int d = ix->d;
-  const int ks = mfma_ksteps_bucket(0, d);
-  if (ks < 0 || d > 1024) return fail(FUEL_AMD_ERR_UNSUPPORTED, "dimension too large for the MFMA candidate scan");
-  const int Kp = ks * 16;
+}  // namespace
+
+// Partition (Lloyd steps from evenly spaced rows) and the sorted padded layout
+// of the residual operand.
+int32_t build_partition(const float* X, int64_t ld, int64_t n, int d, hipStream_t st, DevBuf& cents,
+                        DevBuf& rperm, DevBuf& tile_list, int& L_out, int64_t& n_rpad_out) {
   const int L = resid_lists(n);
-  const float* X = ix->X.as<float>();
   DevBuf list, counts, keys_out, vals_in, vals_out, tmp, offs, poffs;
   hipError_t e = list.ensure(sizeof(uint32_t) * std::max<int64_t>(n, 1));
   if (e == hipSuccess) e = counts.ensure(sizeof(uint32_t) * L);
@@ -55,7 +52,7 @@ int32_t prepare_resid(fuel_amd_index* ix, fuel_amd_index::MfmaOperand& op, hipSt
   if (e == hipSuccess) e = vals_out.ensure(sizeof(uint32_t) * std::max<int64_t>(n, 1));
   if (e == hipSuccess) e = offs.ensure(sizeof(int64_t) * (L + 1));
   if (e == hipSuccess) e = poffs.ensure(sizeof(int64_t) * (L + 1));
-  if (e == hipSuccess) e = op.cents.ensure(sizeof(float) * size_t(L) * d);
+  if (e == hipSuccess) e = cents.ensure(sizeof(float) * size_t(L) * d);
   if (e != hipSuccess) return hip_fail(e, "partition workspaces");
   int bits = 1;
   while ((1 << bits) < L) ++bits;
@@ -65,7 +62,7 @@ int32_t prepare_resid(fuel_amd_index* ix, fuel_amd_index::MfmaOperand& op, hipSt
   if (e == hipSuccess) e = tmp.ensure(std::max<size_t>(tmp_bytes, 256));
   // initial centroids: rows floor(i n / L)
   for (int i = 0; i < L && e == hipSuccess; ++i)
-    e = hipMemcpyAsync(op.cents.as<float>() + size_t(i) * d, X + (int64_t(i) * n / L) * ix->ld, sizeof(float) * d,
+    e = hipMemcpyAsync(cents.as<float>() + size_t(i) * d, X + (int64_t(i) * n / L) * ld, sizeof(float) * d,
                        hipMemcpyDeviceToDevice, st);
   if (e != hipSuccess) return hip_fail(e, "partition init");
   std::vector<uint32_t> hcnt(L);
@@ -74,10 +71,10 @@ int32_t prepare_resid(fuel_amd_index* ix, fuel_amd_index::MfmaOperand& op, hipSt
   for (int it = 0; it <= kLloydSteps; ++it) {
     AssignArgs aa{};
     aa.X = X;
-    aa.ldx = ix->ld;
+    aa.ldx = ld;
     aa.n = n;
     aa.d = d;
-    aa.cents = op.cents.as<float>();
+    aa.cents = cents.as<float>();
     aa.L = L;
     aa.list = list.as<uint32_t>();
     aa.counts = counts.as<uint32_t>();
@@ -94,8 +91,8 @@ int32_t prepare_resid(fuel_amd_index* ix, fuel_amd_index::MfmaOperand& op, hipSt
     for (int j = 0; j < L; ++j) hoff[j + 1] = hoff[j] + hcnt[j];
     e = hipMemcpyAsync(offs.p, hoff.data(), sizeof(int64_t) * (L + 1), hipMemcpyHostToDevice, st);
     if (it < kLloydSteps && e == hipSuccess)
-      e = launch_list_means(X, ix->ld, d, vals_out.as<uint32_t>(), offs.as<int64_t>(), counts.as<uint32_t>(), L,
-                            op.cents.as<float>(), st);
+      e = launch_list_means(X, ld, d, vals_out.as<uint32_t>(), offs.as<int64_t>(), counts.as<uint32_t>(), L,
+                            cents.as<float>(), st);
     if (e != hipSuccess) return hip_fail(e, "partition");
   }
   // padded layout: every list starts on a 64-row block
@@ -107,21 +104,44 @@ int32_t prepare_resid(fuel_amd_index* ix, fuel_amd_index::MfmaOperand& op, hipSt
   for (int j = 0; j < L; ++j)
     for (int64_t b = hpad[j] / 64; b < hpad[j + 1] / 64; ++b) htl[b] = j;
   for (int64_t b = hpad[L] / 64; b < n_rpad / 64; ++b) htl[b] = L - 1;
-  e = op.rperm.ensure(sizeof(uint32_t) * n_rpad);
-  if (e == hipSuccess) e = op.tile_list.ensure(sizeof(int) * htl.size());
-  if (e == hipSuccess) e = op.lconsts.ensure(sizeof(unsigned long long) * 8 * L);
+  e = rperm.ensure(sizeof(uint32_t) * n_rpad);
+  if (e == hipSuccess) e = tile_list.ensure(sizeof(int) * htl.size());
+  if (e != hipSuccess) return hip_fail(e, "residual layout");
+  e = hipMemcpyAsync(poffs.p, hpad.data(), sizeof(int64_t) * (L + 1), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(tile_list.p, htl.data(), sizeof(int) * htl.size(), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = launch_fill_u32(rperm.as<uint32_t>(), n_rpad, RPERM_NONE, st);
+  if (e == hipSuccess)
+    e = launch_scatter_rperm(vals_out.as<uint32_t>(), keys_out.as<uint32_t>(), n, offs.as<int64_t>(),
+                             poffs.as<int64_t>(), rperm.as<uint32_t>(), st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);  // before the workspaces above are released
+  if (e != hipSuccess) return hip_fail(e, "residual layout");
+  L_out = L;
+  n_rpad_out = n_rpad;
+  return FUEL_AMD_OK;
+}
+
+namespace {
+
+// The residual operand over the partition: operand rows, per-list constants.
+int32_t prepare_resid(fuel_amd_index* ix, fuel_amd_index::MfmaOperand& op, hipStream_t st) {
+  const int64_t n = ix->n;
+  const int d = ix->d;
+  const int ks = mfma_ksteps_bucket(0, d);
+  if (ks < 0 || ks > mfma_resid_max_ksteps())
+    return fail(FUEL_AMD_ERR_UNSUPPORTED, "dimension too large for the residual operand");
+  const int Kp = ks * 16;
+  const float* X = ix->X.as<float>();
+  int L = 0;
+  int64_t n_rpad = 0;
+  int32_t rc = build_partition(X, ix->ld, n, d, st, op.cents, op.rperm, op.tile_list, L, n_rpad);
+  if (rc) return rc;
+  hipError_t e = op.lconsts.ensure(sizeof(unsigned long long) * 8 * L);
   if (e == hipSuccess) e = op.Xop.ensure(sizeof(uint16_t) * size_t(n_rpad) * Kp);
   if (e == hipSuccess) e = op.hx.ensure(sizeof(float) * n_rpad);
   if (e == hipSuccess) e = op.consts.ensure(128);
   if (e != hipSuccess) return hip_fail(e, "residual operand");
-  e = hipMemcpyAsync(poffs.p, hpad.data(), sizeof(int64_t) * (L + 1), hipMemcpyHostToDevice, st);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(op.tile_list.p, htl.data(), sizeof(int) * htl.size(), hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = launch_fill_u32(op.rperm.as<uint32_t>(), n_rpad, RPERM_NONE, st);
-  if (e == hipSuccess)
-    e = launch_scatter_rperm(vals_out.as<uint32_t>(), keys_out.as<uint32_t>(), n, offs.as<int64_t>(),
-                             poffs.as<int64_t>(), op.rperm.as<uint32_t>(), st);
-  if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(op.lconsts.p), 16 * L, 0u, st);
+  e = launch_fill_u32(reinterpret_cast<uint32_t*>(op.lconsts.p), 16 * L, 0u, st);
   if (e == hipSuccess) e = launch_fill_u32(op.consts.as<uint32_t>(), 32, 0u, st);  // constants and flags
   PrepResidArgs pr{};
   pr.X = X;
@@ -140,7 +160,7 @@ int32_t prepare_resid(fuel_amd_index* ix, fuel_amd_index::MfmaOperand& op, hipSt
   if (e == hipSuccess) e = launch_prep_resid(pr, st);
   unsigned long long x2bits = 0;
   if (e == hipSuccess) e = hipMemcpyAsync(&x2bits, op.consts.p, sizeof(x2bits), hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);  // also before the workspaces above are released
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return hip_fail(e, "residual operand");
   double x2;
   std::memcpy(&x2, &x2bits, sizeof(x2));
@@ -300,6 +320,7 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
     nsplit = (int)((n_pad + rps - 1) / rps);
 
     hipError_t e = resid ? ws.qtab.ensure(sizeof(float) * size_t(op.L) * nq_pad) : ws.Qop.ensure(size_t(nq_pad) * Kp * es);
+    if (resid && e == hipSuccess) e = ws.qs2.ensure(sizeof(float) * size_t(op.L) * chunk);
     if (e == hipSuccess) e = ws.Q32.ensure(size_t(chunk) * ld32 * 4);
     if (e == hipSuccess) e = ws.qnorm.ensure(sizeof(double) * 3 * size_t(chunk));
     if (e == hipSuccess) e = ws.qint.ensure(sizeof(int) * size_t(chunk));
@@ -335,18 +356,27 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
     qa.theta = d_theta ? d_theta + done : nullptr;
     e = launch_prep_queries(qa, st);
     if (resid && e == hipSuccess) {
+      // ||q - c_j||^2 of every (query, list), then the C-input terms
+      KmeansPairwiseArgs<float> pw{};
+      pw.P1 = qa.Q32;
+      pw.ld1 = ld32;
+      pw.m = chunk;
+      pw.P2 = op.cents.as<float>();
+      pw.ld2 = ix->d;
+      pw.n = op.L;
+      pw.d = ix->d;
+      pw.out = ws.qs2.as<float>();
+      pw.ldo = op.L;
+      e = launch_kmeans_pairwise<float>(pw, st);
       QListArgs la{};
-      la.Q32 = qa.Q32;
-      la.ld32 = ld32;
+      la.s2 = ws.qs2.as<float>();
       la.m = chunk;
       la.nq_pad = nq_pad;
       la.d = ix->d;
-      la.cents = op.cents.as<float>();
-      la.ldc = ix->d;
       la.L = op.L;
       la.lconsts = op.lconsts.as<unsigned long long>();
       la.qtab = ws.qtab.as<float>();
-      e = launch_prep_qlist(la, st);
+      if (e == hipSuccess) e = launch_prep_qlist(la, st);
     }
     if (e == hipSuccess) e = launch_fill_u32(ws.fail_count.as<uint32_t>(), 1, 0u, st);
     if (e != hipSuccess) return hip_fail(e, "query preparation");

@@ -306,15 +306,14 @@ hipError_t launch_prep_resid(const PrepResidArgs& a, hipStream_t st);
 // -||q - c_j||^2 / 2 + e_j(q), e_j(q) bounding the residual product's
 // operand rounding, the hxn rounding, the MFMA accumulation and the C-input
 // addition, so that -(MFMA result) never exceeds ||q - x||^2 / 2 before the
-// row tag.  Slots >= m repeat query m - 1.
+// row tag.  s2[q * L + j] is ||q - c_j||^2 as the f32 pairwise kernel computes
+// it (launch_kmeans_pairwise of the queries against the centroids).  Slots >= m
+// repeat query m - 1.
 struct QListArgs {
-  const float* Q32;
-  int64_t ld32;
+  const float* s2;  // m x L
   int64_t m;
   int64_t nq_pad;
   int d;
-  const float* cents;
-  int ldc;
   int L;
   const unsigned long long* lconsts;
   float* qtab;  // [L][nq_pad]
@@ -383,7 +382,8 @@ hipError_t launch_prep_queries(const PrepQArgs& a, hipStream_t st);
 hipError_t launch_scan_mfma(const ScanMfmaArgs& a, int kind, int ksteps, int V, int nsplit, hipStream_t st);
 hipError_t launch_rerank(const RerankArgs& a, int V, hipStream_t st);
 // grid = the number of persistent workgroups
-hipError_t launch_scan_mfma_batch(const ScanBatchArgs& b, int kind, int ksteps, int V, int grid, hipStream_t st);
+hipError_t launch_scan_mfma_batch(const ScanBatchArgs& b, int kind, int ksteps, int V, int grid, bool resid,
+                                  hipStream_t st);
 // Supported K step counts of the MFMA scan (bf16: 16 dims per step; f32: 8 dims
 // per step group); the padded K is steps * 16 (bf16) or steps * 8 (f32).
 int mfma_ksteps_bucket(int kind, int d);

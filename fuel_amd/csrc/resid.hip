@@ -200,10 +200,13 @@ hipError_t launch_prep_resid(const PrepResidArgs& a, hipStream_t st) {
   return hipGetLastError();
 }
 
-// One wave per (list, query slot).  With N = ||q - c|| (upper bound), the B
-// operand b~ = bf16(fl32(q - c)) has ||b~ - (q - c)|| <= EA = N (2^-9 + 2^-23)
-// and ||b~|| <= N + EA; per list EX = max ||r~ - r||, RN = max ||r||, RT = max
-// ||r~||, EH = max |fl32(||r||^2/2) - ||r||^2/2|, H = max fl32(||r||^2/2).
+// One thread per (list, query slot).  s^ = the f32 value of ||q - c||^2 (a
+// sum of d rounded squares of rounded differences, fused or not): |s^ - s| <=
+// g s + (d + 1) 2^-126 with g = (d + 3) u / (1 - (d + 3) u).  With N = ||q - c||
+// (upper bound), the B operand b~ = bf16(fl32(q - c)) has ||b~ - (q - c)|| <=
+// EA = N (2^-9 + 2^-23) and ||b~|| <= N + EA; per list EX = max ||r~ - r||,
+// RN = max ||r||, RT = max ||r~||, EH = max |fl32(||r||^2/2) - ||r||^2/2|,
+// H = max fl32(||r||^2/2).
 //   operand rounding   N EX + EA RN + EA EX
 //   hxn rounding       EH
 //   accumulation       4 (d + 2) u (|b~| RT + H + Q2 + e)   (C input included)
@@ -212,34 +215,28 @@ hipError_t launch_prep_resid(const PrepResidArgs& a, hipStream_t st) {
 // e solves e >= base + (4 (d + 2) u + u) e; the term is -Q2lo + e rounded up,
 // Q2lo <= ||q - c||^2 / 2 <= Q2.
 __global__ __launch_bounds__(256) void prep_qlist_kernel(QListArgs a) {
-  const int64_t w = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  const int64_t w = int64_t(blockIdx.x) * 256 + threadIdx.x;
   if (w >= int64_t(a.L) * a.nq_pad) return;
   const int j = (int)(w / a.nq_pad);
   const int64_t q = w - int64_t(j) * a.nq_pad;
-  const int lane = __lane_id();
-  const float* qr = a.Q32 + (q < a.m ? q : a.m - 1) * a.ld32;
-  const float* cj = a.cents + size_t(j) * a.ldc;
-  double s2 = 0.0;
-  for (int c = lane; c < a.d; c += 64) {
-    const double t = (double)qr[c] - (double)cj[c];
-    s2 += t * t;
-  }
-  s2 = wave_sum(s2);
-  if (lane != 0) return;
+  const double sh = (double)a.s2[(q < a.m ? q : a.m - 1) * a.L + j];
   const double u = 0x1p-24;
   const int d = a.d;
-  const double sl = (d + 8) * 0x1p-52;
+  const double g = (d + 3) * u / (1.0 - (d + 3) * u) * (1 + 0x1p-20);
+  const double tiny = (d + 1) * 0x1p-126;
+  const double s_hi = (sh * (1 + g) + tiny) * (1 + 0x1p-50);
+  const double s_lo = fmax(0.0, (sh - g * sh - tiny) * (1 - 0x1p-50));
   const unsigned long long* lc = a.lconsts + size_t(j) * 8;
   const double RN = sqrt(__longlong_as_double((long long)lc[0])) * (1 + 0x1p-50);
   const double EX = sqrt(__longlong_as_double((long long)lc[1])) * (1 + 0x1p-50);
   const double RT = sqrt(__longlong_as_double((long long)lc[2])) * (1 + 0x1p-50);
   const double EH = __longlong_as_double((long long)lc[3]);
   const double H = __longlong_as_double((long long)lc[4]);
-  const double N = sqrt(s2 * (1 + sl)) * (1 + 0x1p-50);
+  const double N = sqrt(s_hi) * (1 + 0x1p-50);
   const double EA = N * (0x1p-9 + 0x1p-23) * (1 + 0x1p-40);
   const double B = (N + EA) * (1 + 0x1p-50);
-  const double Q2 = 0.5 * s2 * (1 + sl);
-  const double Q2lo = 0.5 * s2 * (1 - sl);
+  const double Q2 = 0.5 * s_hi;
+  const double Q2lo = 0.5 * s_lo;
   const double gacc = 4.0 * (d + 2) * u;
   const double base = N * EX + EA * RN + EA * EX + EH + gacc * (B * RT + H + Q2) + u * (H + Q2) +
                       2.0 * (d + 2) * 0x1p-126;
@@ -251,9 +248,9 @@ __global__ __launch_bounds__(256) void prep_qlist_kernel(QListArgs a) {
 }
 
 hipError_t launch_prep_qlist(const QListArgs& a, hipStream_t st) {
-  const int64_t waves = int64_t(a.L) * a.nq_pad;
-  if (waves == 0 || a.m == 0) return hipSuccess;
-  hipLaunchKernelGGL(prep_qlist_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, a);
+  const int64_t cnt = int64_t(a.L) * a.nq_pad;
+  if (cnt == 0 || a.m == 0) return hipSuccess;
+  hipLaunchKernelGGL(prep_qlist_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 

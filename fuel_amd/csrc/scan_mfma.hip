@@ -420,7 +420,7 @@ __global__ __launch_bounds__(256) void scan_mfma_kernel_wide(ScanMfmaArgs a) {
 // around the exchange also separate one item's LDS tiles from the next one's
 // (the body ends with every tile read and every copy into LDS completed).  Every
 // workgroup leaves once the counter passes the last item.
-template <int KIND, int KS, int A, int R, int V>
+template <int KIND, int KS, int A, int R, int V, bool RESID>
 __device__ __forceinline__ void scan_mfma_batch_body(const ScanBatchArgs& b) {
   __shared__ int64_t item_s;
   for (;;) {
@@ -438,16 +438,16 @@ __device__ __forceinline__ void scan_mfma_batch_body(const ScanBatchArgs& b) {
     const int p = __builtin_amdgcn_readfirstlane(lo);
     const ScanMfmaArgs a = b.probs[p];
     const int64_t local = item - b.item_start[p];
-    scan_mfma_body<KIND, KS, A, R, V>(a, (int)(local % a.nqb), (int)(local / a.nqb));
+    scan_mfma_body<KIND, KS, A, R, V, RESID>(a, (int)(local % a.nqb), (int)(local / a.nqb));
   }
 }
-template <int KIND, int KS, int A, int R, int V>
+template <int KIND, int KS, int A, int R, int V, bool RESID>
 __global__ __launch_bounds__(256, 2) void scan_mfma_batch_kernel(ScanBatchArgs b) {
-  scan_mfma_batch_body<KIND, KS, A, R, V>(b);
+  scan_mfma_batch_body<KIND, KS, A, R, V, RESID>(b);
 }
-template <int KIND, int KS, int A, int R, int V>
+template <int KIND, int KS, int A, int R, int V, bool RESID>
 __global__ __launch_bounds__(256) void scan_mfma_batch_kernel_wide(ScanBatchArgs b) {
-  scan_mfma_batch_body<KIND, KS, A, R, V>(b);
+  scan_mfma_batch_body<KIND, KS, A, R, V, RESID>(b);
 }
 
 // K step buckets: bf16 covers 16 coordinates per step, f32 covers 8.  Up to 16
@@ -492,33 +492,47 @@ static hipError_t launch_ks(const ScanMfmaArgs& a, int V, int nsplit, hipStream_
   return hipGetLastError();
 }
 
-template <int KIND, int KS>
+template <int KIND, int KS, bool RESID>
 static hipError_t launch_batch_ks(const ScanBatchArgs& b, int V, int grid, hipStream_t st) {
   constexpr int A = KS <= 16 ? 2 : 1;
   constexpr int R = KS <= 16 ? 2 : 1;
   constexpr bool two_waves = KIND == 0 ? KS <= 12 : KS <= 8;
   if constexpr (two_waves) {
     if (V == 4)
-      hipLaunchKernelGGL((scan_mfma_batch_kernel<KIND, KS, A, R, 4>), dim3(grid), dim3(256), 0, st, b);
+      hipLaunchKernelGGL((scan_mfma_batch_kernel<KIND, KS, A, R, 4, RESID>), dim3(grid), dim3(256), 0, st, b);
     else
-      hipLaunchKernelGGL((scan_mfma_batch_kernel<KIND, KS, A, R, 8>), dim3(grid), dim3(256), 0, st, b);
+      hipLaunchKernelGGL((scan_mfma_batch_kernel<KIND, KS, A, R, 8, RESID>), dim3(grid), dim3(256), 0, st, b);
   } else {
     if (V == 4)
-      hipLaunchKernelGGL((scan_mfma_batch_kernel_wide<KIND, KS, A, R, 4>), dim3(grid), dim3(256), 0, st, b);
+      hipLaunchKernelGGL((scan_mfma_batch_kernel_wide<KIND, KS, A, R, 4, RESID>), dim3(grid), dim3(256), 0, st, b);
     else
-      hipLaunchKernelGGL((scan_mfma_batch_kernel_wide<KIND, KS, A, R, 8>), dim3(grid), dim3(256), 0, st, b);
+      hipLaunchKernelGGL((scan_mfma_batch_kernel_wide<KIND, KS, A, R, 8, RESID>), dim3(grid), dim3(256), 0, st, b);
   }
   return hipGetLastError();
 }
 
-hipError_t launch_scan_mfma_batch(const ScanBatchArgs& b, int kind, int ks, int V, int grid, hipStream_t st) {
+hipError_t launch_scan_mfma_batch(const ScanBatchArgs& b, int kind, int ks, int V, int grid, bool resid,
+                                  hipStream_t st) {
   if (V != 4 && V != 8) return hipErrorInvalidValue;
   if (b.nitems <= 0) return hipSuccess;
+  if (kind == 0 && resid) {
+    switch (ks) {
+#define KS_CASE(K) \
+  case K:          \
+    return launch_batch_ks<0, K, true>(b, V, grid, st);
+      KS_CASE(1) KS_CASE(2) KS_CASE(3) KS_CASE(4) KS_CASE(5) KS_CASE(6) KS_CASE(7) KS_CASE(8) KS_CASE(9)
+      KS_CASE(10) KS_CASE(12) KS_CASE(14) KS_CASE(16) KS_CASE(20) KS_CASE(24) KS_CASE(28) KS_CASE(32)
+      KS_CASE(40) KS_CASE(48)
+#undef KS_CASE
+      default:
+        return hipErrorInvalidValue;
+    }
+  }
   if (kind == 0) {
     switch (ks) {
 #define KS_CASE(K) \
   case K:          \
-    return launch_batch_ks<0, K>(b, V, grid, st);
+    return launch_batch_ks<0, K, false>(b, V, grid, st);
       KS_CASE(1) KS_CASE(2) KS_CASE(3) KS_CASE(4) KS_CASE(5) KS_CASE(6) KS_CASE(7) KS_CASE(8) KS_CASE(9)
       KS_CASE(10) KS_CASE(12) KS_CASE(14) KS_CASE(16) KS_CASE(20) KS_CASE(24) KS_CASE(28) KS_CASE(32)
       KS_CASE(40) KS_CASE(48) KS_CASE(56) KS_CASE(64)
@@ -530,7 +544,7 @@ hipError_t launch_scan_mfma_batch(const ScanBatchArgs& b, int kind, int ks, int 
   switch (ks) {
 #define KS_CASE(K) \
   case K:          \
-    return launch_batch_ks<1, K>(b, V, grid, st);
+    return launch_batch_ks<1, K, false>(b, V, grid, st);
     KS_CASE(1) KS_CASE(2) KS_CASE(3) KS_CASE(4) KS_CASE(6) KS_CASE(8) KS_CASE(12) KS_CASE(16) KS_CASE(20)
     KS_CASE(24) KS_CASE(32)
 #undef KS_CASE

@@ -31,7 +31,7 @@ def _check(oracle, probs, results, k, mode="for_each_knn", metric="euclidean"):
         assert gd.tobytes() == od.tobytes(), X.shape
 
 
-@pytest.mark.parametrize("mode", ["bf16", "fp32", "exact"])
+@pytest.mark.parametrize("mode", ["bf16", "bf16r", "fp32", "exact"])
 def test_batch_fixed_modes(gpu, oracle, problems, mode):
     from fuel_amd import BatchKnn
     b = BatchKnn(problems, 15, mode=mode)
@@ -45,13 +45,14 @@ def test_batch_fixed_modes(gpu, oracle, problems, mode):
     b.close()
 
 
-def test_graph_replays_with_eager_work_between(gpu, oracle, problems):
+@pytest.mark.parametrize("mode", ["bf16", "bf16r"])
+def test_graph_replays_with_eager_work_between(gpu, oracle, problems, mode):
     from fuel_amd import BatchKnn
-    eager = BatchKnn(problems, 15, mode="bf16", graph=False)
+    eager = BatchKnn(problems, 15, mode=mode, graph=False)
     eager.run()
     ref = eager.results()
     _check(oracle, problems, ref, 15)
-    g = BatchKnn(problems, 15, mode="bf16", graph=True)
+    g = BatchKnn(problems, 15, mode=mode, graph=True)
     other = gpu.Index(problems[0], "euclidean")
     for rep in range(4):
         g.run()
@@ -75,7 +76,7 @@ def test_autotuned_plan(gpu, oracle):
     tuned = {(t["d"], t["mode"], t["widen"], t["fill"]) for t in info["tuning"]}
     for dd in dims:  # every configuration was timed for every dimension
         assert {(m, w, f) for (d2, m, w, f) in tuned if d2 == dd} == {
-            (m, w, f) for m in ("bf16", "fp32") for w in (2, 4, 8) for f in (1, 4)}
+            (m, w, f) for m in ("bf16", "bf16r", "fp32") for w in (2, 4, 8) for f in (1, 4)}
     for g in info["groups"]:  # the chosen one is the fastest timed
         times = [t for t in info["tuning"] if t["d"] == g["d"]]
         best = min(times, key=lambda t: t["ms"])
@@ -83,11 +84,12 @@ def test_autotuned_plan(gpu, oracle):
     b.close()
 
 
+@pytest.mark.parametrize("mode", ["bf16", "bf16r"])
 @pytest.mark.parametrize("exclude,omode", [(0, "none"), (2, "python_linear")])
-def test_other_exclusion_and_metric(gpu, oracle, problems, exclude, omode):
+def test_other_exclusion_and_metric(gpu, oracle, problems, exclude, omode, mode):
     from fuel_amd import BatchKnn
     probs = problems[:6]
-    b = BatchKnn(probs, 9, metric="sqeuclidean", exclude=exclude, mode="bf16")
+    b = BatchKnn(probs, 9, metric="sqeuclidean", exclude=exclude, mode=mode)
     b.run()
     _check(oracle, probs, b.results(), 9, omode, "sqeuclidean")
     b.close()
@@ -99,7 +101,7 @@ def test_edge_problems(gpu, oracle):
     probs = [rng.random((1, 8), dtype=np.float32), rng.random((10, 32), dtype=np.float32),
              rng.random((0, 16), dtype=np.float32), rng.random((700, 64), dtype=np.float32)]
     ks = [5, 12, 3, 0]
-    b = BatchKnn(probs, ks, mode="bf16")
+    b = BatchKnn(probs, ks, mode="bf16r")
     b.run()
     res = b.results()
     for X, kk, (gi, gd) in zip(probs, ks, res):
