@@ -105,7 +105,7 @@ struct fuel_amd_index {
     int64_t n_pad = 0;  // operand rows (kind 2: lists padded to 64-row blocks)
     fa::DevBuf Xop, hx, consts;  // operand copy, ||x||^2/2 (kind 2: of the residual), bound constants + flags
     int L = 0;                   // kind 2: lists, centroids (L x d), position -> row, list of each
-    fa::DevBuf cents, rperm, tile_list, lconsts;  // 64-row block, per-list bound constants
+    fa::DevBuf cents, rperm, tile_list, lconsts, lcount;  // 64-row block, per-list bound constants, rows
   } mop[3];
   std::mutex prep_mu;
   fa::QueryCtx ctx;  // the synchronous entry points' query workspaces and stats
@@ -134,6 +134,7 @@ struct fuel_amd_index {
       m.rperm.release();
       m.tile_list.release();
       m.lconsts.release();
+      m.lcount.release();
       m.ready = false;
     }
   }
@@ -174,7 +175,7 @@ bool mfma_supported(const fuel_amd_index* ix, int kind, int k, int exclude);
 // Residual operand layout of n rows (f32, pitch ld): centroids (L x d), padded
 // position -> row (RPERM_NONE for padding) and the list of every 64-row block.
 int32_t build_partition(const float* X, int64_t ld, int64_t n, int d, hipStream_t st, DevBuf& cents,
-                        DevBuf& rperm, DevBuf& tile_list, int& L, int64_t& n_rpad);
+                        DevBuf& rperm, DevBuf& tile_list, DevBuf& lcount, int& L, int64_t& n_rpad);
 // Exact path on an explicit list of query rows (the certification fallback).
 int32_t knn_exact_rows(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t ldq, const int64_t* d_rows,
                        int64_t nrows, int64_t qbase, int k, int exclude, int64_t* out_idx, float* out_dist,

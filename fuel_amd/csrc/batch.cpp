@@ -64,7 +64,7 @@ struct ProbPlan {
 // Residual layout of one problem (plan time; the graph rebuilds the operand
 // from the current rows over it).
 struct ResidPart {
-  DevBuf cents, rperm, tile_list;
+  DevBuf cents, rperm, tile_list, lcount;
   int L = 0;
   int64_t n_rpad = 0;
 };
@@ -186,7 +186,7 @@ int32_t plan_batch(fuel_amd_batch* b, const std::map<int, Cfg>& cfg_by_d) {
         if (!b->parts[i]) {
           auto part = std::make_unique<ResidPart>();
           int32_t rc = build_partition(static_cast<const float*>(pr.data), pr.ld, pr.n, pr.d, b->stream, part->cents,
-                                       part->rperm, part->tile_list, part->L, part->n_rpad);
+                                       part->rperm, part->tile_list, part->lcount, part->L, part->n_rpad);
           if (rc) return rc;
           b->parts[i] = std::move(part);
         }
@@ -393,6 +393,9 @@ hipError_t enqueue_bucket(fuel_amd_batch* b, const Bucket& B, hipStream_t st) {
       la.L = b->parts[i]->L;
       la.lconsts = at<unsigned long long>(b->pool, pp.o_lconsts);
       la.qtab = at<float>(b->pool, pp.o_qtab);
+      la.lcount = b->parts[i]->lcount.as<uint32_t>();
+      la.KP = pp.KP;
+      la.gthr = at<uint32_t>(b->pool, pp.o_gthr);
       if (e == hipSuccess) e = launch_prep_qlist(la, st);
     }
     if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(b->fail_counts.as<int>() + i), 1, 0u, st);

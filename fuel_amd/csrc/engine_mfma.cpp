@@ -42,7 +42,7 @@ int resid_lists(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(
 // Partition (Lloyd steps from evenly spaced rows) and the sorted padded layout
 // of the residual operand.
 int32_t build_partition(const float* X, int64_t ld, int64_t n, int d, hipStream_t st, DevBuf& cents,
-                        DevBuf& rperm, DevBuf& tile_list, int& L_out, int64_t& n_rpad_out) {
+                        DevBuf& rperm, DevBuf& tile_list, DevBuf& lcount, int& L_out, int64_t& n_rpad_out) {
   const int L = resid_lists(n);
   DevBuf list, counts, keys_out, vals_in, vals_out, tmp, offs, poffs;
   hipError_t e = list.ensure(sizeof(uint32_t) * std::max<int64_t>(n, 1));
@@ -106,6 +106,8 @@ int32_t build_partition(const float* X, int64_t ld, int64_t n, int d, hipStream_
   for (int64_t b = hpad[L] / 64; b < n_rpad / 64; ++b) htl[b] = L - 1;
   e = rperm.ensure(sizeof(uint32_t) * n_rpad);
   if (e == hipSuccess) e = tile_list.ensure(sizeof(int) * htl.size());
+  if (e == hipSuccess) e = lcount.ensure(sizeof(uint32_t) * L);
+  if (e == hipSuccess) e = hipMemcpyAsync(lcount.p, hcnt.data(), sizeof(uint32_t) * L, hipMemcpyHostToDevice, st);
   if (e != hipSuccess) return hip_fail(e, "residual layout");
   e = hipMemcpyAsync(poffs.p, hpad.data(), sizeof(int64_t) * (L + 1), hipMemcpyHostToDevice, st);
   if (e == hipSuccess)
@@ -134,7 +136,7 @@ int32_t prepare_resid(fuel_amd_index* ix, fuel_amd_index::MfmaOperand& op, hipSt
   const float* X = ix->X.as<float>();
   int L = 0;
   int64_t n_rpad = 0;
-  int32_t rc = build_partition(X, ix->ld, n, d, st, op.cents, op.rperm, op.tile_list, L, n_rpad);
+  int32_t rc = build_partition(X, ix->ld, n, d, st, op.cents, op.rperm, op.tile_list, op.lcount, L, n_rpad);
   if (rc) return rc;
   hipError_t e = op.lconsts.ensure(sizeof(unsigned long long) * 8 * L);
   if (e == hipSuccess) e = op.Xop.ensure(sizeof(uint16_t) * size_t(n_rpad) * Kp);
@@ -293,9 +295,11 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
   const int Kp = op.kp;
   const bool resid = kind == 2;
   const int mk = resid ? 0 : kind;  // MFMA operand type
-  const int qpb = mfma_queries_per_block(mk, ks);
+  // small residual batches (serving): one-wave workgroups of 64 query slots
+  const int waves = (m <= 192 && mfma_one_wave_supported(mk, ks, V, resid)) ? 1 : 4;
+  const int qpb = mfma_queries_per_block(mk, ks, waves);
   const int rows_tile = mfma_rows_per_tile(mk, ks);
-  const int wg_per_cu = mfma_workgroups_per_cu(mk, ks);
+  const int wg_per_cu = mfma_workgroups_per_cu(mk, ks, waves);
   const int cus = device_cu_count(ix->device);
   const size_t es = mk == 0 ? 2 : 4;
   const int64_t n_pad = op.n_pad;
@@ -376,6 +380,9 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
       la.L = op.L;
       la.lconsts = op.lconsts.as<unsigned long long>();
       la.qtab = ws.qtab.as<float>();
+      la.lcount = op.lcount.as<uint32_t>();
+      la.KP = KP;
+      la.gthr = ws.gthr.as<uint32_t>();
       if (e == hipSuccess) e = launch_prep_qlist(la, st);
     }
     if (e == hipSuccess) e = launch_fill_u32(ws.fail_count.as<uint32_t>(), 1, 0u, st);
@@ -395,6 +402,7 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
     sa.slots = ws.slots.as<uint64_t>();
     sa.lcnts = ws.lcnts.as<int>();
     sa.gthr = ws.gthr.as<uint32_t>();
+    sa.waves = waves;
     if (resid) {
       sa.resid = 1;
       sa.tile_list = op.tile_list.as<int>();

@@ -212,31 +212,46 @@ __global__ __launch_bounds__(64) void rerank_kernel(RerankArgs a) {
   T cut = C::MAX;
   int fill = 0;
   int64_t total = 0;
-  for (int s = 0; s < a.nsplit; ++s) {
-    const T* sl = a.slots + (size_t(s) * a.nq_pad + qs) * a.slot_ns;
-    const int cnt = a.lcnts[size_t(s) * a.nq_pad + qs];
-    total += cnt;
-    for (int off = 0; off < cnt; off += 64) {
-      const T val = off + lane < cnt ? sl[off + lane] : C::MAX;
-      const bool keep = val < cut;
-      const uint64_t mask = __ballot(keep);
-      if (keep) buf[fill + __popcll(mask & ((uint64_t(1) << lane) - 1))] = val;
-      fill += __popcll(mask);
-      if (fill >= NS) {
-        __syncthreads();
+  auto absorb = [&](const T val) {
+    const bool keep = val < cut;
+    const uint64_t mask = __ballot(keep);
+    if (keep) buf[fill + __popcll(mask & ((uint64_t(1) << lane) - 1))] = val;
+    fill += __popcll(mask);
+    if (fill >= NS) {
+      __syncthreads();
 #pragma unroll
-        for (int v = 0; v < V; ++v) x[v] = buf[v * 64 + lane];
-        const T over = lane < fill - NS ? buf[NS + lane] : C::MAX;
-        wave_bitonic_sort<T, V>(x);
-        cut = wave_get<T, V>(x, KP - 1);
-        __syncthreads();
+      for (int v = 0; v < V; ++v) x[v] = buf[v * 64 + lane];
+      const T over = lane < fill - NS ? buf[NS + lane] : C::MAX;
+      wave_bitonic_sort<T, V>(x);
+      cut = wave_get<T, V>(x, KP - 1);
+      __syncthreads();
 #pragma unroll
-        for (int v = 0; v < V; ++v)
-          if (v * 64 + lane < KP) buf[v * 64 + lane] = x[v];
-        if (lane < fill - NS) buf[KP + lane] = over;
-        fill = KP + (fill - NS);
-        __syncthreads();
-      }
+      for (int v = 0; v < V; ++v)
+        if (v * 64 + lane < KP) buf[v * 64 + lane] = x[v];
+      if (lane < fill - NS) buf[KP + lane] = over;
+      fill = KP + (fill - NS);
+      __syncthreads();
+    }
+  };
+  // splits in groups of PF: their counts, then their first 64 entries, are
+  // loaded before any is processed (small batches run hundreds of splits)
+  constexpr int PF = 4;
+  for (int s0 = 0; s0 < a.nsplit; s0 += PF) {
+    int cnt[PF];
+    T v0[PF];
+#pragma unroll
+    for (int u = 0; u < PF; ++u) cnt[u] = s0 + u < a.nsplit ? a.lcnts[size_t(s0 + u) * a.nq_pad + qs] : 0;
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      const T* sl = a.slots + (size_t(s0 + u < a.nsplit ? s0 + u : 0) * a.nq_pad + qs) * a.slot_ns;
+      v0[u] = lane < cnt[u] ? sl[lane] : C::MAX;
+    }
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      total += cnt[u];
+      absorb(v0[u]);
+      const T* sl = a.slots + (size_t(s0 + u < a.nsplit ? s0 + u : 0) * a.nq_pad + qs) * a.slot_ns;
+      for (int off = 64; off < cnt[u]; off += 64) absorb(off + lane < cnt[u] ? sl[off + lane] : C::MAX);
     }
   }
   __syncthreads();

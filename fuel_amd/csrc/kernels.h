@@ -249,6 +249,7 @@ struct ScanMfmaArgs {
   int64_t m;
   int d;
   const float* qtab;       // [L][nq_pad]
+  int waves;               // waves per workgroup: 4, or 1 (see mfma_one_wave_supported); 0 means 4
 };
 
 // ---- residual (centroid-relative) operand ------------------------------------
@@ -309,6 +310,12 @@ hipError_t launch_prep_resid(const PrepResidArgs& a, hipStream_t st);
 // row tag.  s2[q * L + j] is ||q - c_j||^2 as the f32 pairwise kernel computes
 // it (launch_kmeans_pairwise of the queries against the centroids).  Slots >= m
 // repeat query m - 1.
+// The same launch seeds each slot's shared threshold: with N_j an upper bound of
+// ||q - c_j|| and RN_j = max ||r|| over list j, every row of list j has
+// ||q - x||^2 / 2 <= (N_j + RN_j)^2 / 2, and its key is no larger (up to the
+// row tag); so theta = min over lists with >= KP rows of that bound, rounded
+// up, lies at or above the KP-th smallest key, and gthr = min(gthr, theta)
+// drops only rows that cannot enter any split's kept list.
 struct QListArgs {
   const float* s2;  // m x L
   int64_t m;
@@ -317,6 +324,9 @@ struct QListArgs {
   int L;
   const unsigned long long* lconsts;
   float* qtab;  // [L][nq_pad]
+  const uint32_t* lcount;  // rows per list
+  int KP;
+  uint32_t* gthr;          // [nq_pad], initialised by the query preparation
 };
 hipError_t launch_prep_qlist(const QListArgs& a, hipStream_t st);
 
@@ -389,9 +399,12 @@ hipError_t launch_scan_mfma_batch(const ScanBatchArgs& b, int kind, int ksteps, 
 int mfma_ksteps_bucket(int kind, int d);
 // Largest K step count with a residual-operand variant of the bf16 scan.
 int mfma_resid_max_ksteps();
-int mfma_queries_per_block(int kind, int ksteps);
+// Launch shape: waves per workgroup is 4, or 1 where mfma_one_wave_supported
+// (residual bf16 scans with V = 4 and at most 12 K steps; small batches).
+int mfma_queries_per_block(int kind, int ksteps, int waves = 4);
 int mfma_rows_per_tile(int kind, int ksteps);
-int mfma_workgroups_per_cu(int kind, int ksteps);
+int mfma_workgroups_per_cu(int kind, int ksteps, int waves = 4);
+bool mfma_one_wave_supported(int kind, int ksteps, int V, bool resid);
 
 template <typename F>
 hipError_t launch_scan_exact(const ScanExactArgs<F>& a, int qw, int nsplit, hipStream_t st);

@@ -18,8 +18,8 @@
 // 32*s + 16*h of its row (8 bf16, or 4 f32 feeding 4 chained 32x32x2 MFMAs);
 // A and B use the same assignment, so the sums run over every coordinate once.
 //
-// Workgroup: 4 waves, each owning A*32 queries; the tile of R*32 DB rows is
-// shared through LDS by the 4 waves and double-buffered: the global->LDS copy
+// Workgroup: 4 waves (1 for small residual batches), each owning A*32 queries;
+// the tile of R*32 DB rows is shared through LDS by the waves and double-buffered: the global->LDS copy
 // of tile t+2 is issued as soon as tile t's buffer is free.
 //
 // Selection: a row block is skipped when no lane's maximum beats its query's
@@ -183,22 +183,22 @@ __device__ __forceinline__ int merge_select(uint64_t* __restrict__ slot, int KP,
     }                                                                                     \
   } while (0)
 
-template <int KIND, int KS, int A, int R, int V, bool RESID = false>
+template <int KIND, int KS, int A, int R, int V, bool RESID = false, int W = 4>
 __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int qb, const int split) {
   using C = Comp<float>;
   constexpr int NS = 64 * V;
   constexpr int QPW = 32 * A;
-  constexpr int QPB = 4 * QPW;
+  constexpr int QPB = W * QPW;  // W waves per workgroup (4; 1 for small residual batches)
   constexpr int BN = 32 * R;
   constexpr int CH = KS * 2;                  // 16-byte chunks of a row
   constexpr int ROWC = CH + 1;                // + one pad chunk (bank spread)
   constexpr int ROWB = ROWC * 16;             // LDS row stride in bytes
   constexpr int TILE_CHUNKS = BN * ROWC;
-  constexpr int WAVE_LOADS = (TILE_CHUNKS + 255) / 256;  // 64-chunk loads per wave per tile
-  constexpr int TILE_BYTES = WAVE_LOADS * 256 * 16;       // rounded up to whole wave loads
+  constexpr int WAVE_LOADS = (TILE_CHUNKS + 64 * W - 1) / (64 * W);  // 64-chunk loads per wave per tile
+  constexpr int TILE_BYTES = WAVE_LOADS * 64 * W * 16;                // rounded up to whole wave loads
   __shared__ __attribute__((aligned(16))) char tile[2][TILE_BYTES];
   __shared__ __attribute__((aligned(16))) float tile_hx[2][64];      // one dword per lane of wave 0
-  __shared__ __attribute__((aligned(16))) uint32_t thr_lds[2][4][64];  // published thresholds, per wave
+  __shared__ __attribute__((aligned(16))) uint32_t thr_lds[2][W][64];  // published thresholds, per wave
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -239,13 +239,13 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
     lcnt[g] = 0;
   }
 
-  // Tile staging: lane i of DMA j of wave w fills LDS chunk p = (j*4 + w)*64 + i
+  // Tile staging: lane i of DMA j of wave w fills LDS chunk p = (j*W + w)*64 + i
   // = row*ROWC + c; the pad chunk (c == CH) and the round-up tail re-read a
   // valid chunk.  The per-lane source offsets do not depend on the tile.
   uint32_t soff[WAVE_LOADS];
 #pragma unroll
   for (int j = 0; j < WAVE_LOADS; ++j) {
-    const int p = (j * 4 + wave) * 64 + lane;
+    const int p = (j * W + wave) * 64 + lane;
     int row = p / ROWC, c = p - row * ROWC;
     if (c >= CH) c = CH - 1;
     if (row >= BN) {
@@ -262,13 +262,13 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
     const char* base = xop + size_t(t) * kp_bytes;
 #pragma unroll
     for (int j = 0; j < WAVE_LOADS; ++j)
-      lds_dma16(__builtin_amdgcn_readfirstlane(lds_tile + b * TILE_BYTES + (j * 4 + wave) * 1024), base + soff[j]);
+      lds_dma16(__builtin_amdgcn_readfirstlane(lds_tile + b * TILE_BYTES + (j * W + wave) * 1024), base + soff[j]);
     if (wave == 0) {
       const int r = lane < BN ? lane : BN - 1;
       lds_dma4(__builtin_amdgcn_readfirstlane(lds_hx + b * 64 * 4), a.hx + t + r);
     }
     // the thresholds other splits published for this wave's queries ride along
-    lds_dma4(__builtin_amdgcn_readfirstlane(lds_thr + (b * 4 + wave) * 256), gthr_src);
+    lds_dma4(__builtin_amdgcn_readfirstlane(lds_thr + (b * W + wave) * 256), gthr_src);
   };
 
   if (r0 < r1) stage(0, r0);
@@ -414,6 +414,13 @@ __global__ __launch_bounds__(256) void scan_mfma_kernel_wide(ScanMfmaArgs a) {
   scan_mfma_body<KIND, KS, A, R, V, RESID>(a, (int)(blockIdx.x % (unsigned)a.nqb),
                                            (int)(blockIdx.x / (unsigned)a.nqb));
 }
+// One wave per workgroup (64 query slots): small batches (serving) waste
+// fewer padded query slots, and more workgroups share a CU, each on its own split.
+template <int KIND, int KS, int A, int R, int V, bool RESID>
+__global__ __launch_bounds__(64, 8) void scan_mfma_kernel_1w(ScanMfmaArgs a) {
+  scan_mfma_body<KIND, KS, A, R, V, RESID, 1>(a, (int)(blockIdx.x % (unsigned)a.nqb),
+                                              (int)(blockIdx.x / (unsigned)a.nqb));
+}
 
 // Persistent workgroups over the work items of many problems.  One lane takes
 // the next item from the counter and publishes it through LDS; the barriers
@@ -468,15 +475,30 @@ int mfma_ksteps_bucket(int kind, int d) {
   return -1;
 }
 
-int mfma_queries_per_block(int kind, int ksteps) { return 4 * 32 * (ksteps <= 16 ? 2 : 1); }
+int mfma_queries_per_block(int kind, int ksteps, int waves) { return waves * 32 * (ksteps <= 16 ? 2 : 1); }
 int mfma_rows_per_tile(int kind, int ksteps) { return 32 * (ksteps <= 16 ? 2 : 1); }
-int mfma_workgroups_per_cu(int kind, int ksteps) { return (kind == 0 ? ksteps <= 12 : ksteps <= 8) ? 2 : 1; }
+int mfma_workgroups_per_cu(int kind, int ksteps, int waves) {
+  const int two = (kind == 0 ? ksteps <= 12 : ksteps <= 8) ? 2 : 1;
+  return waves == 1 ? 4 * two : two;
+}
+bool mfma_one_wave_supported(int kind, int ksteps, int V, bool resid) {
+  return kind == 0 && resid && V == 4 && ksteps <= 12;
+}
 
 template <int KIND, int KS, bool RESID>
 static hipError_t launch_ks(const ScanMfmaArgs& a, int V, int nsplit, hipStream_t st) {
   constexpr int A = KS <= 16 ? 2 : 1;
   constexpr int R = KS <= 16 ? 2 : 1;
   constexpr bool two_waves = KIND == 0 ? KS <= 12 : KS <= 8;
+  if constexpr (KIND == 0 && RESID && KS <= 12) {
+    if (a.waves == 1) {
+      if (V != 4) return hipErrorInvalidValue;
+      hipLaunchKernelGGL((scan_mfma_kernel_1w<KIND, KS, A, R, 4, RESID>), dim3((unsigned)(a.nqb * nsplit)), dim3(64),
+                         0, st, a);
+      return hipGetLastError();
+    }
+  }
+  if (a.waves == 1) return hipErrorInvalidValue;
   const dim3 grid((unsigned)(a.nqb * nsplit)), block(256);
   if constexpr (two_waves) {
     if (V == 4)
