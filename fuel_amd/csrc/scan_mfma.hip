@@ -34,6 +34,8 @@
 // KP-th key is the new threshold.  Thresholds are shared by the splits of a
 // query through a per-query atomicMin; every tile's copy brings the values the
 // other splits published.
+#include <algorithm>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -293,14 +295,25 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
           const float* qv = a.Q32 + (qs < a.m ? qs : a.m - 1) * a.ld32;
 #pragma unroll
           for (int s = 0; s < KS; ++s) {
+            const int k0 = s * 16 + half * 8;
+            float v[8];
+            if (k0 + 8 <= a.d && (a.ldc & 3) == 0) {
+              // rows of Q32 are 32-byte aligned and k0 is a multiple of 8; centroid
+              // rows are 16-byte aligned when their pitch is a multiple of 4
+              const float4 q0 = *reinterpret_cast<const float4*>(qv + k0);
+              const float4 q1 = *reinterpret_cast<const float4*>(qv + k0 + 4);
+              const float4 c0 = *reinterpret_cast<const float4*>(cj + k0);
+              const float4 c1 = *reinterpret_cast<const float4*>(cj + k0 + 4);
+              v[0] = q0.x - c0.x; v[1] = q0.y - c0.y; v[2] = q0.z - c0.z; v[3] = q0.w - c0.w;
+              v[4] = q1.x - c1.x; v[5] = q1.y - c1.y; v[6] = q1.z - c1.z; v[7] = q1.w - c1.w;
+            } else {
+#pragma unroll
+              for (int e = 0; e < 8; ++e) v[e] = k0 + e < a.d ? qv[k0 + e] - cj[k0 + e] : 0.0f;
+            }
             uint32_t wv[4];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              const int k = s * 16 + half * 8 + 2 * e;
-              const float v0 = k < a.d ? qv[k] - cj[k] : 0.0f;
-              const float v1 = k + 1 < a.d ? qv[k + 1] - cj[k + 1] : 0.0f;
-              wv[e] = uint32_t(bf16_bits_rne(v0)) | (uint32_t(bf16_bits_rne(v1)) << 16);
-            }
+            for (int e = 0; e < 4; ++e)
+              wv[e] = uint32_t(bf16_bits_rne(v[2 * e])) | (uint32_t(bf16_bits_rne(v[2 * e + 1])) << 16);
             qf[g][s] = make_uint4(wv[0], wv[1], wv[2], wv[3]);
           }
           qt[g] = a.qtab[size_t(jl) * a.nq_pad + qs];
@@ -414,6 +427,13 @@ __global__ __launch_bounds__(256) void scan_mfma_kernel_wide(ScanMfmaArgs a) {
   scan_mfma_body<KIND, KS, A, R, V, RESID>(a, (int)(blockIdx.x % (unsigned)a.nqb),
                                            (int)(blockIdx.x / (unsigned)a.nqb));
 }
+// Shapes of at most 4 bf16 K steps: tiles are small (a few KB) and a tile's work
+// is short next to the latency of its copy, so four workgroups share a CU.
+template <int KIND, int KS, int A, int R, int V, bool RESID>
+__global__ __launch_bounds__(256, 4) void scan_mfma_kernel_4wg(ScanMfmaArgs a) {
+  scan_mfma_body<KIND, KS, A, R, V, RESID>(a, (int)(blockIdx.x % (unsigned)a.nqb),
+                                           (int)(blockIdx.x / (unsigned)a.nqb));
+}
 // One wave per workgroup (64 query slots): small batches (serving) waste
 // fewer padded query slots, and more workgroups share a CU, each on its own split.
 template <int KIND, int KS, int A, int R, int V, bool RESID>
@@ -456,6 +476,10 @@ template <int KIND, int KS, int A, int R, int V, bool RESID>
 __global__ __launch_bounds__(256) void scan_mfma_batch_kernel_wide(ScanBatchArgs b) {
   scan_mfma_batch_body<KIND, KS, A, R, V, RESID>(b);
 }
+template <int KIND, int KS, int A, int R, int V, bool RESID>
+__global__ __launch_bounds__(256, 4) void scan_mfma_batch_kernel_4wg(ScanBatchArgs b) {
+  scan_mfma_batch_body<KIND, KS, A, R, V, RESID>(b);
+}
 
 // K step buckets: bf16 covers 16 coordinates per step, f32 covers 8.  Up to 16
 // steps a wave holds 2 query groups and the tile has 64 rows; beyond, 1 query
@@ -478,8 +502,8 @@ int mfma_ksteps_bucket(int kind, int d) {
 int mfma_queries_per_block(int kind, int ksteps, int waves) { return waves * 32 * (ksteps <= 16 ? 2 : 1); }
 int mfma_rows_per_tile(int kind, int ksteps) { return 32 * (ksteps <= 16 ? 2 : 1); }
 int mfma_workgroups_per_cu(int kind, int ksteps, int waves) {
-  const int two = (kind == 0 ? ksteps <= 12 : ksteps <= 8) ? 2 : 1;
-  return waves == 1 ? 4 * two : two;
+  const int per = (kind == 0 && ksteps <= 4) ? 4 : ((kind == 0 ? ksteps <= 12 : ksteps <= 8) ? 2 : 1);
+  return waves == 1 ? 4 * std::min(per, 2) : per;
 }
 bool mfma_one_wave_supported(int kind, int ksteps, int V, bool resid) {
   return kind == 0 && resid && V == 4 && ksteps <= 12;
@@ -500,7 +524,12 @@ static hipError_t launch_ks(const ScanMfmaArgs& a, int V, int nsplit, hipStream_
   }
   if (a.waves == 1) return hipErrorInvalidValue;
   const dim3 grid((unsigned)(a.nqb * nsplit)), block(256);
-  if constexpr (two_waves) {
+  if constexpr (KIND == 0 && KS <= 4) {
+    if (V == 4)
+      hipLaunchKernelGGL((scan_mfma_kernel_4wg<KIND, KS, A, R, 4, RESID>), grid, block, 0, st, a);
+    else
+      hipLaunchKernelGGL((scan_mfma_kernel_4wg<KIND, KS, A, R, 8, RESID>), grid, block, 0, st, a);
+  } else if constexpr (two_waves) {
     if (V == 4)
       hipLaunchKernelGGL((scan_mfma_kernel<KIND, KS, A, R, 4, RESID>), grid, block, 0, st, a);
     else
@@ -519,7 +548,12 @@ static hipError_t launch_batch_ks(const ScanBatchArgs& b, int V, int grid, hipSt
   constexpr int A = KS <= 16 ? 2 : 1;
   constexpr int R = KS <= 16 ? 2 : 1;
   constexpr bool two_waves = KIND == 0 ? KS <= 12 : KS <= 8;
-  if constexpr (two_waves) {
+  if constexpr (KIND == 0 && KS <= 4) {
+    if (V == 4)
+      hipLaunchKernelGGL((scan_mfma_batch_kernel_4wg<KIND, KS, A, R, 4, RESID>), dim3(grid), dim3(256), 0, st, b);
+    else
+      hipLaunchKernelGGL((scan_mfma_batch_kernel_4wg<KIND, KS, A, R, 8, RESID>), dim3(grid), dim3(256), 0, st, b);
+  } else if constexpr (two_waves) {
     if (V == 4)
       hipLaunchKernelGGL((scan_mfma_batch_kernel<KIND, KS, A, R, 4, RESID>), dim3(grid), dim3(256), 0, st, b);
     else
