@@ -68,37 +68,59 @@ __device__ __forceinline__ RowStats prep_row(const float* __restrict__ src, int 
   return r;
 }
 
+// One workgroup per 64 rows (16 per wave); the index-wide maxima and flags are
+// reduced in the workgroup and published with one atomic each (per-row
+// atomics on the same few addresses serialise).
 __global__ __launch_bounds__(256) void prep_db_kernel(PrepDbArgs a) {
-  const int64_t row = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
-  if (row >= a.n_pad) return;
+  __shared__ double red[4][5];
+  __shared__ int redf[4][2];
+  const int wave = threadIdx.x >> 6;
   const int lane = __lane_id();
   const size_t es = a.bf16 ? 2 : 4;
-  char* dst = reinterpret_cast<char*>(a.Xop) + size_t(row) * a.Kp * es;
-  if (row >= a.n) {
-    for (int c = lane; c < a.Kp; c += 64) {
-      if (a.bf16)
-        reinterpret_cast<uint16_t*>(dst)[c] = 0;
-      else
-        reinterpret_cast<float*>(dst)[c] = 0.0f;
+  const double slack = 1.0 + (a.d + 4) * 0x1p-52;
+  double m[5] = {0, 0, 0, 0, 0};
+  bool allint = true, allnonneg = true;
+  for (int i = 0; i < 16; ++i) {
+    const int64_t row = int64_t(blockIdx.x) * 64 + wave * 16 + i;
+    if (row >= a.n_pad) break;
+    char* dst = reinterpret_cast<char*>(a.Xop) + size_t(row) * a.Kp * es;
+    if (row >= a.n) {
+      for (int c = lane; c < a.Kp; c += 64) {
+        if (a.bf16)
+          reinterpret_cast<uint16_t*>(dst)[c] = 0;
+        else
+          reinterpret_cast<float*>(dst)[c] = 0.0f;
+      }
+      if (lane == 0) a.hx[row] = MFMA_PAD_HXN;
+      continue;
     }
-    if (lane == 0) a.hx[row] = MFMA_PAD_HXN;
-    return;
-  }
-  const RowStats r = prep_row(a.X + row * a.ldx, a.d, a.Kp, a.bf16, dst);
-  if (lane == 0) {
+    const RowStats r = prep_row(a.X + row * a.ldx, a.d, a.Kp, a.bf16, dst);
     const double v = r.s2 * 0.5;
     const float h = (float)v;
-    a.hx[row] = -h;
-    const double slack = 1.0 + (a.d + 4) * 0x1p-52;
-    atomic_max_nonneg(&a.consts[0], r.s2 * slack);
-    atomic_max_nonneg(&a.consts[1], r.e2 * slack);
-    atomic_max_nonneg(&a.consts[2], r.t2 * slack);
+    if (lane == 0) a.hx[row] = -h;
+    m[0] = fmax(m[0], r.s2 * slack);
+    m[1] = fmax(m[1], r.e2 * slack);
+    m[2] = fmax(m[2], r.t2 * slack);
     // |fl32(||x||^2/2) - ||x||^2/2|, including the f64 error of the sum of
     // squares (none for integer rows: every partial sum is an integer < 2^53)
-    atomic_max_nonneg(&a.consts[3], fabs((double)h - v) + (r.isint ? 0.0 : v * (a.d + 4) * 0x1p-53));
-    atomic_max_nonneg(&a.consts[4], (double)h);
-    if (!r.isint) atomicAnd(&a.flags[0], 0);
-    if (!r.nonneg) atomicAnd(&a.flags[1], 0);
+    m[3] = fmax(m[3], fabs((double)h - v) + (r.isint ? 0.0 : v * (a.d + 4) * 0x1p-53));
+    m[4] = fmax(m[4], (double)h);
+    allint = allint && r.isint;
+    allnonneg = allnonneg && r.nonneg;
+  }
+  if (lane == 0) {
+    for (int t = 0; t < 5; ++t) red[wave][t] = m[t];
+    redf[wave][0] = allint ? 1 : 0;
+    redf[wave][1] = allnonneg ? 1 : 0;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    const int t = threadIdx.x;
+    const double v = fmax(fmax(red[0][t], red[1][t]), fmax(red[2][t], red[3][t]));
+    if (v > 0.0) atomic_max_nonneg(&a.consts[t], v);
+  } else if (threadIdx.x < 7) {
+    const int f = threadIdx.x - 5;
+    if (!(redf[0][f] && redf[1][f] && redf[2][f] && redf[3][f])) atomicAnd(&a.flags[f], 0);
   }
 }
 
@@ -134,7 +156,7 @@ __global__ __launch_bounds__(256) void prep_query_kernel(PrepQArgs a) {
 
 hipError_t launch_prep_db(const PrepDbArgs& a, hipStream_t st) {
   if (a.n_pad == 0) return hipSuccess;
-  hipLaunchKernelGGL(prep_db_kernel, dim3((unsigned)((a.n_pad + 3) / 4)), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(prep_db_kernel, dim3((unsigned)((a.n_pad + 63) / 64)), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 

@@ -148,55 +148,74 @@ hipError_t launch_iota_u32(uint32_t* p, int64_t n, hipStream_t st) {
   return hipGetLastError();
 }
 
-// One wave per padded position.
+// One workgroup per 64-row block (every block lies in one list): each wave
+// prepares 16 positions, the per-list maxima are reduced in the workgroup and
+// published with one atomic per constant (a per-row atomic on the few
+// addresses of a list serialises).
 __global__ __launch_bounds__(256) void prep_resid_kernel(PrepResidArgs a) {
-  const int64_t pos = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
-  if (pos >= a.n_rpad) return;
+  __shared__ double red[4][6];
+  const int wave = threadIdx.x >> 6;
   const int lane = __lane_id();
-  uint16_t* dst = a.Xop + size_t(pos) * a.Kp;
-  const uint32_t row = a.rperm[pos];
-  if (row == RPERM_NONE) {
-    for (int c = lane; c < a.Kp; c += 64) dst[c] = 0;
-    if (lane == 0) a.hx[pos] = MFMA_PAD_HXN;
-    return;
-  }
-  const int j = a.tile_list[pos >> 6];
-  const float* x = a.X + int64_t(row) * a.ldx;
+  const int64_t blk = blockIdx.x;
+  const int j = a.tile_list[blk];
   const float* cj = a.cents + size_t(j) * a.ldc;
-  double s2 = 0.0, e2 = 0.0, t2 = 0.0, x2 = 0.0;
-  for (int c = lane; c < a.Kp; c += 64) {
-    const double xv = c < a.d ? (double)x[c] : 0.0;
-    x2 += xv * xv;
-    const double r = c < a.d ? xv - (double)cj[c] : 0.0;
-    const uint16_t h = bf16_rne((float)r);
-    dst[c] = h;
-    const double rt = (double)bf16_val(h);
-    s2 += r * r;
-    e2 += (rt - r) * (rt - r);
-    t2 += rt * rt;
-  }
-  s2 = wave_sum(s2);
-  e2 = wave_sum(e2);
-  t2 = wave_sum(t2);
-  x2 = wave_sum(x2);
-  if (lane == 0) {
-    atomic_max_nonneg(&a.gconsts[0], x2 * (1.0 + (a.d + 6) * 0x1p-52));
+  const double slack = 1.0 + (a.d + 6) * 0x1p-52;
+  double m[6] = {0, 0, 0, 0, 0, 0};  // ||r||^2, ||r~-r||^2, ||r~||^2, hxn error, fl32(||r||^2/2), ||x||^2
+  for (int i = 0; i < 16; ++i) {
+    const int64_t pos = blk * 64 + wave * 16 + i;
+    uint16_t* dst = a.Xop + size_t(pos) * a.Kp;
+    const uint32_t row = a.rperm[pos];
+    if (row == RPERM_NONE) {
+      for (int c = lane; c < a.Kp; c += 64) dst[c] = 0;
+      if (lane == 0) a.hx[pos] = MFMA_PAD_HXN;
+      continue;
+    }
+    const float* x = a.X + int64_t(row) * a.ldx;
+    double s2 = 0.0, e2 = 0.0, t2 = 0.0, x2 = 0.0;
+    for (int c = lane; c < a.Kp; c += 64) {
+      const double xv = c < a.d ? (double)x[c] : 0.0;
+      x2 += xv * xv;
+      const double r = c < a.d ? xv - (double)cj[c] : 0.0;
+      const uint16_t h = bf16_rne((float)r);
+      dst[c] = h;
+      const double rt = (double)bf16_val(h);
+      s2 += r * r;
+      e2 += (rt - r) * (rt - r);
+      t2 += rt * rt;
+    }
+    s2 = wave_sum(s2);
+    e2 = wave_sum(e2);
+    t2 = wave_sum(t2);
+    x2 = wave_sum(x2);
     const double v = s2 * 0.5;
     const float hf = (float)v;
-    a.hx[pos] = -hf;
-    const double slack = 1.0 + (a.d + 6) * 0x1p-52;
-    unsigned long long* lc = a.lconsts + size_t(j) * 8;
-    atomic_max_nonneg(&lc[0], s2 * slack);
-    atomic_max_nonneg(&lc[1], e2 * slack);
-    atomic_max_nonneg(&lc[2], t2 * slack);
-    atomic_max_nonneg(&lc[3], fabs((double)hf - v) + v * (a.d + 6) * 0x1p-52);
-    atomic_max_nonneg(&lc[4], (double)hf);
+    if (lane == 0) a.hx[pos] = -hf;
+    m[0] = fmax(m[0], s2 * slack);
+    m[1] = fmax(m[1], e2 * slack);
+    m[2] = fmax(m[2], t2 * slack);
+    m[3] = fmax(m[3], fabs((double)hf - v) + v * (a.d + 6) * 0x1p-52);
+    m[4] = fmax(m[4], (double)hf);
+    m[5] = fmax(m[5], x2 * slack);
+  }
+  if (lane == 0)
+    for (int t = 0; t < 6; ++t) red[wave][t] = m[t];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int t = threadIdx.x;
+    const double v = fmax(fmax(red[0][t], red[1][t]), fmax(red[2][t], red[3][t]));
+    if (v > 0.0) {
+      if (t < 5)
+        atomic_max_nonneg(&a.lconsts[size_t(j) * 8 + t], v);
+      else
+        atomic_max_nonneg(&a.gconsts[0], v);
+    }
   }
 }
 
 hipError_t launch_prep_resid(const PrepResidArgs& a, hipStream_t st) {
   if (a.n_rpad == 0) return hipSuccess;
-  hipLaunchKernelGGL(prep_resid_kernel, dim3((unsigned)((a.n_rpad + 3) / 4)), dim3(256), 0, st, a);
+  if (a.n_rpad % 64 != 0) return hipErrorInvalidValue;  // lists are whole 64-row blocks
+  hipLaunchKernelGGL(prep_resid_kernel, dim3((unsigned)(a.n_rpad / 64)), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
