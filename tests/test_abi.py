@@ -91,3 +91,50 @@ def test_condensed_helpers():
     assert (_condensed_to_square(internal) == sq).all()
     assert (pairwise_distances(sq, "precomputed", "square") == sq).all()
     assert (pairwise_distances(scipy_cond, "precomputed", "square") == sq).all()
+
+
+def test_struct_layouts_match_header(tmp_path):
+    # the ctypes mirrors of every struct in include/fuel_amd.h, against the C
+    # compiler's own sizeof / offsetof of the header's declarations
+    import subprocess
+    from fuel_amd import _lib
+    mirrors = {"fuel_amd_opts": _lib.Opts, "fuel_amd_csr": _lib.Csr, "fuel_amd_stats": _lib.Stats,
+               "fuel_amd_server_opts": _lib.ServerOpts, "fuel_amd_serve_stats": _lib.ServeStats,
+               "fuel_amd_problem": _lib.Problem, "fuel_amd_batch_opts": _lib.BatchOpts}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "fuel_amd.h"', "int main(void) {"]
+    for cname, ct in mirrors.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in ct._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, ct in mirrors.items():
+        assert got[(cname, "size")] == ctypes.sizeof(ct), cname
+        for f, _ in ct._fields_:
+            assert got[(cname, f)] == getattr(ct, f).offset, (cname, f)
+
+
+def test_server_and_batch_argument_errors():
+    from fuel_amd import _lib
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    assert lib.fuel_amd_server_create(None, None, ctypes.byref(h)) == _lib.ERR_INVALID
+    assert lib.fuel_amd_server_submit(None, None, 1, 1, None) == _lib.ERR_INVALID
+    assert lib.fuel_amd_server_destroy(None) == _lib.OK
+    t = ctypes.c_uint64()
+    assert lib.fuel_amd_server_wait(None, t, None, None, None, None) == _lib.ERR_INVALID
+    x = np.zeros((16, 12), np.float32)
+    oi = np.zeros((16, 3), np.int64)
+    od = np.zeros((16, 3), np.float32)
+    bad_ld = _lib.Problem(x.ctypes.data, 16, 12, 12, 3, oi.ctypes.data, od.ctypes.data)  # ld not a multiple of 8
+    arr = (_lib.Problem * 1)(bad_ld)
+    assert lib.fuel_amd_batch_create(arr, 1, _lib.EUCLIDEAN, 1, None, ctypes.byref(h)) == _lib.ERR_INVALID
+    opts = _lib.BatchOpts(9, 4, 0, 0)
+    assert lib.fuel_amd_batch_create(arr, 0, _lib.EUCLIDEAN, 1, ctypes.byref(opts), ctypes.byref(h)) == _lib.ERR_INVALID
+    assert lib.fuel_amd_batch_create(arr, 0, 5, 1, None, ctypes.byref(h)) == _lib.ERR_UNSUPPORTED
+    assert lib.fuel_amd_batch_destroy(None) == _lib.OK
