@@ -233,27 +233,43 @@ __global__ __launch_bounds__(64) void rerank_kernel(RerankArgs a) {
       __syncthreads();
     }
   };
-  // splits in groups of PF: their counts, then their first 64 entries, are
-  // loaded before any is processed (small batches run hundreds of splits)
+  // The counts of 64 splits come in one load (lane s: split s0 + s); only the
+  // non-empty lists are visited, PF at a time with their first 64 entries
+  // loaded together (small batches run hundreds of splits, and a query that
+  // starts from a tight threshold leaves most of their lists empty).
   constexpr int PF = 4;
-  for (int s0 = 0; s0 < a.nsplit; s0 += PF) {
-    int cnt[PF];
-    T v0[PF];
+  int64_t my_total = 0;
+  for (int s0 = 0; s0 < a.nsplit; s0 += 64) {
+    const int c = s0 + lane < a.nsplit ? a.lcnts[size_t(s0 + lane) * a.nq_pad + qs] : 0;
+    my_total += c;
+    uint64_t mask = __ballot(c > 0);
+    while (mask) {
+      int cnt[PF];
+      const T* sl[PF];
 #pragma unroll
-    for (int u = 0; u < PF; ++u) cnt[u] = s0 + u < a.nsplit ? a.lcnts[size_t(s0 + u) * a.nq_pad + qs] : 0;
+      for (int u = 0; u < PF; ++u) {
+        cnt[u] = 0;
+        sl[u] = a.slots;
+        if (mask) {
+          const int b = __builtin_ctzll(mask);
+          mask &= mask - 1;
+          cnt[u] = __builtin_amdgcn_readlane(c, b);
+          sl[u] = a.slots + (size_t(s0 + b) * a.nq_pad + qs) * a.slot_ns;
+        }
+      }
+      T v0[PF];
 #pragma unroll
-    for (int u = 0; u < PF; ++u) {
-      const T* sl = a.slots + (size_t(s0 + u < a.nsplit ? s0 + u : 0) * a.nq_pad + qs) * a.slot_ns;
-      v0[u] = lane < cnt[u] ? sl[lane] : C::MAX;
-    }
+      for (int u = 0; u < PF; ++u) v0[u] = lane < cnt[u] ? sl[u][lane] : C::MAX;
 #pragma unroll
-    for (int u = 0; u < PF; ++u) {
-      total += cnt[u];
-      absorb(v0[u]);
-      const T* sl = a.slots + (size_t(s0 + u < a.nsplit ? s0 + u : 0) * a.nq_pad + qs) * a.slot_ns;
-      for (int off = 64; off < cnt[u]; off += 64) absorb(off + lane < cnt[u] ? sl[off + lane] : C::MAX);
+      for (int u = 0; u < PF; ++u) {
+        absorb(v0[u]);
+        for (int off = 64; off < cnt[u]; off += 64) absorb(off + lane < cnt[u] ? sl[u][off + lane] : C::MAX);
+      }
     }
   }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) my_total += __shfl_xor(my_total, m);
+  total = my_total;
   __syncthreads();
 #pragma unroll
   for (int v = 0; v < V; ++v) x[v] = v * 64 + lane < fill ? buf[v * 64 + lane] : C::MAX;

@@ -266,15 +266,16 @@ __global__ __launch_bounds__(256) void prep_qlist_kernel(QListArgs a) {
   a.qtab[w] = tf;
 }
 
-// One thread per query slot (see QListArgs).
+// One wave per query slot, lanes over the lists (see QListArgs).
 __global__ __launch_bounds__(256) void seed_theta_kernel(QListArgs a) {
-  const int64_t q = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int64_t q = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
   if (q >= a.nq_pad) return;
+  const int lane = __lane_id();
   const float* s2 = a.s2 + (q < a.m ? q : a.m - 1) * a.L;
   const double g = (a.d + 3) * 0x1p-24 / (1.0 - (a.d + 3) * 0x1p-24) * (1 + 0x1p-20);
   const double tiny = (a.d + 1) * 0x1p-126;
   double best = INFINITY;
-  for (int j = 0; j < a.L; ++j) {
+  for (int j = lane; j < a.L; j += 64) {
     if (a.lcount[j] < (uint32_t)a.KP) continue;
     const double sh = (double)s2[j];
     const double N = sqrt((sh * (1 + g) + tiny) * (1 + 0x1p-50)) * (1 + 0x1p-50);
@@ -282,7 +283,9 @@ __global__ __launch_bounds__(256) void seed_theta_kernel(QListArgs a) {
     const double v = 0.5 * (N + RN) * (N + RN) * (1 + 0x1p-16) + 0x1p-140;
     best = fmin(best, v);
   }
-  if (best < (double)MFMA_KEY_LIMIT) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) best = fmin(best, __shfl_xor(best, m));
+  if (lane == 0 && best < (double)MFMA_KEY_LIMIT) {
     float t = (float)best;
     if ((double)t < best) t = nextafterf(t, INFINITY);
     const uint32_t o = ord_f32(t);
@@ -295,7 +298,7 @@ hipError_t launch_prep_qlist(const QListArgs& a, hipStream_t st) {
   if (cnt == 0 || a.m == 0) return hipSuccess;
   hipLaunchKernelGGL(prep_qlist_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st, a);
   if (a.gthr && a.lcount)
-    hipLaunchKernelGGL(seed_theta_kernel, dim3((unsigned)((a.nq_pad + 255) / 256)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(seed_theta_kernel, dim3((unsigned)((a.nq_pad + 3) / 4)), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
