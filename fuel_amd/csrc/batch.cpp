@@ -168,8 +168,8 @@ int32_t plan_batch(fuel_amd_batch* b, const std::map<int, Cfg>& cfg_by_d) {
     std::sort(bk.members.begin(), bk.members.end(),
               [&](int x, int y) { return b->probs[x].n > b->probs[y].n || (b->probs[x].n == b->probs[y].n && x < y); });
     const int qpb = mfma_queries_per_block(mk, bk.ks);
-    const int rows_tile = mfma_rows_per_tile(mk, bk.ks);
-    const int64_t slots = int64_t(cus) * mfma_workgroups_per_cu(mk, bk.ks);
+    const int rows_tile = mfma_rows_per_tile(mk, bk.ks, resid);
+    const int64_t slots = int64_t(cus) * mfma_workgroups_per_cu(mk, bk.ks, 4, resid);
     const int Kp = bk.ks * (mk == 0 ? 16 : 8);
     const size_t es = mk == 0 ? 2 : 4;
     double work = 0.0;

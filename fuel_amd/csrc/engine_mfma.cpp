@@ -299,8 +299,8 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
   // small residual batches (serving): one-wave workgroups of 64 query slots
   const int waves = (m <= 192 && mfma_one_wave_supported(mk, ks, V, resid)) ? 1 : 4;
   const int qpb = mfma_queries_per_block(mk, ks, waves);
-  const int rows_tile = mfma_rows_per_tile(mk, ks);
-  const int wg_per_cu = mfma_workgroups_per_cu(mk, ks, waves);
+  const int rows_tile = mfma_rows_per_tile(mk, ks, resid);
+  const int wg_per_cu = mfma_workgroups_per_cu(mk, ks, waves, resid);
   const int cus = device_cu_count(ix->device);
   const size_t es = mk == 0 ? 2 : 4;
   const int64_t n_pad = op.n_pad;

@@ -402,8 +402,8 @@ int mfma_resid_max_ksteps();
 // Launch shape: waves per workgroup is 4, or 1 where mfma_one_wave_supported
 // (residual bf16 scans with V = 4 and at most 12 K steps; small batches).
 int mfma_queries_per_block(int kind, int ksteps, int waves = 4);
-int mfma_rows_per_tile(int kind, int ksteps);
-int mfma_workgroups_per_cu(int kind, int ksteps, int waves = 4);
+int mfma_rows_per_tile(int kind, int ksteps, bool resid);
+int mfma_workgroups_per_cu(int kind, int ksteps, int waves, bool resid);
 bool mfma_one_wave_supported(int kind, int ksteps, int V, bool resid);
 
 template <typename F>

@@ -297,7 +297,7 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
           for (int s = 0; s < KS; ++s) {
             const int k0 = s * 16 + half * 8;
             float v[8];
-            if (k0 + 8 <= a.d && (a.ldc & 3) == 0) {
+            if (KS >= 14 && k0 + 8 <= a.d && (a.ldc & 3) == 0) {
               // rows of Q32 are 32-byte aligned and k0 is a multiple of 8; centroid
               // rows are 16-byte aligned when their pitch is a multiple of 4
               const float4 q0 = *reinterpret_cast<const float4*>(qv + k0);
@@ -481,6 +481,18 @@ __global__ __launch_bounds__(256, 4) void scan_mfma_batch_kernel_4wg(ScanBatchAr
   scan_mfma_batch_body<KIND, KS, A, R, V, RESID>(b);
 }
 
+// Shape of a K step count: query groups per wave (A), 32-row blocks per tile
+// (R), two waves per SIMD or one.  Plain bf16 shapes of 14 and 16 steps take
+// 32-row tiles so that two waves per SIMD fit (their query fragments take 128
+// registers); beyond 16 steps one query group per wave.
+// The residual variants keep 64-row tiles and one wave per SIMD from 14 steps
+// (their list-change rebuild of the fragments leaves no room for two).
+constexpr int scan_groups(int ks) { return ks <= 16 ? 2 : 1; }
+constexpr int scan_row_blocks(int kind, int ks, bool resid) {
+  return (kind == 0 && !resid ? ks <= 12 : ks <= 16) ? 2 : 1;
+}
+constexpr bool scan_two_waves(int kind, int ks, bool resid) { return kind == 0 ? ks <= (resid ? 12 : 16) : ks <= 8; }
+
 // K step buckets: bf16 covers 16 coordinates per step, f32 covers 8.  Up to 16
 // steps a wave holds 2 query groups and the tile has 64 rows; beyond, 1 query
 // group and 32-row tiles keep the query fragments at <= 256 registers and the
@@ -499,10 +511,10 @@ int mfma_ksteps_bucket(int kind, int d) {
   return -1;
 }
 
-int mfma_queries_per_block(int kind, int ksteps, int waves) { return waves * 32 * (ksteps <= 16 ? 2 : 1); }
-int mfma_rows_per_tile(int kind, int ksteps) { return 32 * (ksteps <= 16 ? 2 : 1); }
-int mfma_workgroups_per_cu(int kind, int ksteps, int waves) {
-  const int per = (kind == 0 && ksteps <= 4) ? 4 : ((kind == 0 ? ksteps <= 12 : ksteps <= 8) ? 2 : 1);
+int mfma_queries_per_block(int kind, int ksteps, int waves) { return waves * 32 * scan_groups(ksteps); }
+int mfma_rows_per_tile(int kind, int ksteps, bool resid) { return 32 * scan_row_blocks(kind, ksteps, resid); }
+int mfma_workgroups_per_cu(int kind, int ksteps, int waves, bool resid) {
+  const int per = (kind == 0 && ksteps <= 4) ? 4 : (scan_two_waves(kind, ksteps, resid) ? 2 : 1);
   return waves == 1 ? 4 * std::min(per, 2) : per;
 }
 bool mfma_one_wave_supported(int kind, int ksteps, int V, bool resid) {
@@ -511,9 +523,9 @@ bool mfma_one_wave_supported(int kind, int ksteps, int V, bool resid) {
 
 template <int KIND, int KS, bool RESID>
 static hipError_t launch_ks(const ScanMfmaArgs& a, int V, int nsplit, hipStream_t st) {
-  constexpr int A = KS <= 16 ? 2 : 1;
-  constexpr int R = KS <= 16 ? 2 : 1;
-  constexpr bool two_waves = KIND == 0 ? KS <= 12 : KS <= 8;
+  constexpr int A = scan_groups(KS);
+  constexpr int R = scan_row_blocks(KIND, KS, RESID);
+  constexpr bool two_waves = scan_two_waves(KIND, KS, RESID);
   if constexpr (KIND == 0 && RESID && KS <= 12) {
     if (a.waves == 1) {
       if (V != 4) return hipErrorInvalidValue;
@@ -545,9 +557,9 @@ static hipError_t launch_ks(const ScanMfmaArgs& a, int V, int nsplit, hipStream_
 
 template <int KIND, int KS, bool RESID>
 static hipError_t launch_batch_ks(const ScanBatchArgs& b, int V, int grid, hipStream_t st) {
-  constexpr int A = KS <= 16 ? 2 : 1;
-  constexpr int R = KS <= 16 ? 2 : 1;
-  constexpr bool two_waves = KIND == 0 ? KS <= 12 : KS <= 8;
+  constexpr int A = scan_groups(KS);
+  constexpr int R = scan_row_blocks(KIND, KS, RESID);
+  constexpr bool two_waves = scan_two_waves(KIND, KS, RESID);
   if constexpr (KIND == 0 && KS <= 4) {
     if (V == 4)
       hipLaunchKernelGGL((scan_mfma_batch_kernel_4wg<KIND, KS, A, R, 4, RESID>), dim3(grid), dim3(256), 0, st, b);
@@ -578,7 +590,7 @@ hipError_t launch_scan_mfma_batch(const ScanBatchArgs& b, int kind, int ks, int 
     return launch_batch_ks<0, K, true>(b, V, grid, st);
       KS_CASE(1) KS_CASE(2) KS_CASE(3) KS_CASE(4) KS_CASE(5) KS_CASE(6) KS_CASE(7) KS_CASE(8) KS_CASE(9)
       KS_CASE(10) KS_CASE(12) KS_CASE(14) KS_CASE(16) KS_CASE(20) KS_CASE(24) KS_CASE(28) KS_CASE(32)
-      KS_CASE(40) KS_CASE(48)
+      KS_CASE(40)
 #undef KS_CASE
       default:
         return hipErrorInvalidValue;
@@ -609,7 +621,7 @@ hipError_t launch_scan_mfma_batch(const ScanBatchArgs& b, int kind, int ks, int 
   }
 }
 
-int mfma_resid_max_ksteps() { return 48; }
+int mfma_resid_max_ksteps() { return 40; }
 
 hipError_t launch_scan_mfma(const ScanMfmaArgs& a, int kind, int ks, int V, int nsplit, hipStream_t st) {
   if (V != 4 && V != 8) return hipErrorInvalidValue;
@@ -620,7 +632,7 @@ hipError_t launch_scan_mfma(const ScanMfmaArgs& a, int kind, int ks, int V, int 
     return launch_ks<0, K, true>(a, V, nsplit, st);
       KS_CASE(1) KS_CASE(2) KS_CASE(3) KS_CASE(4) KS_CASE(5) KS_CASE(6) KS_CASE(7) KS_CASE(8) KS_CASE(9)
       KS_CASE(10) KS_CASE(12) KS_CASE(14) KS_CASE(16) KS_CASE(20) KS_CASE(24) KS_CASE(28) KS_CASE(32)
-      KS_CASE(40) KS_CASE(48)
+      KS_CASE(40)
 #undef KS_CASE
       default:
         return hipErrorInvalidValue;
