@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 PEAKS = {  # MI355X published dense peaks
     "bf16": (2516.6e12, "TFLOP/s", "mfma"),
+    "bf16r": (2516.6e12, "TFLOP/s", "mfma"),
     "fp32": (157.3e12, "TFLOP/s", "mfma"),
     "exact": (157.3e12, "TFLOP/s", "mfma"),
     "hbm": (8.0e12, "GB/s", "hbm"),
@@ -37,7 +38,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--mode", default="bf16", choices=["bf16", "fp32", "exact"])
+    p.add_argument("--mode", default="bf16", choices=["bf16", "bf16r", "fp32", "exact"])
     p.add_argument("--n", type=int, default=1_000_000)
     p.add_argument("--m", type=int, default=100_000)
     p.add_argument("--d", type=int, default=128)
@@ -553,7 +554,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"bf16": "bf16", "fp32": "f32", "exact": "f32"}[args.mode],
+        "dtype": {"bf16": "bf16", "bf16r": "bf16", "fp32": "f32", "exact": "f32"}[args.mode],
         "data": "synthetic",
         "config": {"workload": workload,
                    "mode": args.mode, "widen": args.widen, "splits": stats["splits"], "kp": stats["kp"],
