@@ -122,7 +122,9 @@ class SearchIndex:
 
     The rows are copied to HBM once; ``knn`` and ``radius_search`` then stream
     only the queries.  ``mode`` selects how candidates are generated ('auto',
-    'exact', 'fp32', 'bf16'); every mode returns the exact fuel answer.
+    'exact', 'fp32', 'bf16', or 'bf16r' -- bf16 on residuals against coarse
+    centroids, for clustered data whose norms are large next to the neighbour
+    distances); every mode returns the exact fuel answer.
 
     ``precompute=max_k`` fills fuel's precomputed kNN table (every point's
     neighbours with ties up to max_k) with one GPU call; self-queries with
