@@ -114,3 +114,41 @@ def test_edge_problems(gpu, oracle):
     # k beyond n pads with -1 / +inf
     assert np.all(res[0][0] == -1) and np.all(np.isinf(res[0][1]))
     b.close()
+
+
+def test_replay_after_the_data_changes(gpu, oracle):
+    # the graph holds pointers, not values: new contents in the same buffers are
+    # answered exactly on the next replay (partitions stay, operands are rebuilt)
+    import torch
+    from fuel_amd import BatchKnn
+    probs = _problems(53, 6, 300, 2500, [8, 64])
+    b = BatchKnn(probs, 7, mode="bf16r")
+    b.run()
+    _check(oracle, probs, b.results(), 7)
+    rng = np.random.default_rng(54)
+    new = []
+    for t in b._data:  # overwrite every problem's rows in place
+        X = rng.standard_normal((t.shape[0], t.shape[1])).astype(np.float32) * 3.0
+        t.copy_(torch.from_numpy(X).to(t.device))
+        new.append(X[:, :t.shape[1]])
+    b.run()
+    got = b.results()
+    for X, P, (gi, gd) in zip(new, probs, got):
+        d = P.shape[1]
+        oi, od = oracle.knn(X[:, :d].copy(), X[:, :d].copy(), 7, "euclidean", "for_each_knn")
+        assert np.array_equal(gi, oi) and gd.tobytes() == od.tobytes()
+    b.close()
+
+
+def test_cancel_and_resume(gpu, oracle, problems):
+    from fuel_amd import BatchKnn
+    b = BatchKnn(problems[:5], 6, mode="bf16")
+    gpu.request_cancel()
+    try:
+        with pytest.raises(KeyboardInterrupt):
+            b.run()
+    finally:
+        gpu.clear_cancel()
+    b.run()
+    _check(oracle, problems[:5], b.results(), 6)
+    b.close()

@@ -117,3 +117,56 @@ def test_unwaited_tickets_and_f64(gpu, oracle):
     oi, od = oracle.knn(X, X[51:68], 6, "euclidean", "none")
     assert np.array_equal(gi, oi) and gd.tobytes() == od.tobytes()
     srv.close()  # answers and releases the tickets never waited for
+
+
+def test_two_servers_and_direct_calls_on_one_index(gpu, oracle):
+    # per-caller workspaces: two servers and the synchronous entry point share
+    # one resident index from several threads at once
+    from fuel_amd import Server
+    rng = np.random.default_rng(8)
+    X = _mixture(30_000, 32, 20, 81)
+    Q = _mixture(600, 32, 20, 82)
+    ix = gpu.Index(X, "euclidean", mode="bf16r")
+    ref_i, ref_d = oracle.knn(X, Q, 10, "euclidean", "none")
+    out = {}
+    errs = []
+
+    def serve(srv, lo, hi, key):
+        try:
+            ts = [(i, srv.submit(Q[i:i + 25], 10)) for i in range(lo, hi, 25)]
+            out[key] = [(i, srv.wait(t)[:2]) for i, t in ts]
+        except Exception as e:  # surfaced below
+            errs.append(e)
+
+    def direct(lo, hi):
+        try:
+            out["direct"] = [(lo, ix.knn(Q[lo:hi], 10))]
+        except Exception as e:
+            errs.append(e)
+
+    with Server(ix, workers=2, max_batch=256) as s1, Server(ix, workers=1, max_batch=128) as s2:
+        ths = [threading.Thread(target=serve, args=(s1, 0, 300, "s1")),
+               threading.Thread(target=serve, args=(s2, 300, 600, "s2")),
+               threading.Thread(target=direct, args=(100, 400))]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+    assert not errs, errs
+    for key in ("s1", "s2", "direct"):
+        for lo, (gi, gd) in out[key]:
+            n = gi.shape[0]
+            assert np.array_equal(gi, ref_i[lo:lo + n]) and gd.tobytes() == ref_d[lo:lo + n].tobytes(), key
+
+
+@pytest.mark.parametrize("k", [127, 200])
+def test_large_k_through_the_server(gpu, oracle, k):
+    # k + 1 = 128 is the widest MFMA kept list; beyond it the exact scan answers
+    from fuel_amd import Server
+    X = _mixture(8000, 16, 10, 90)
+    Q = X[:40] + np.float32(0.01)
+    ix = gpu.Index(X, "sqeuclidean", mode="bf16r")
+    with Server(ix) as srv:
+        gi, gd, _, _ = srv.wait(srv.submit(Q, k))
+    oi, od = oracle.knn(X, Q, k, "sqeuclidean", "none")
+    assert np.array_equal(gi, oi) and gd.tobytes() == od.tobytes()
