@@ -101,6 +101,7 @@ struct fuel_amd_batch {
   double last_ms = 0.0;
   int64_t last_fallback = 0, last_exact = 0, runs = 0;
   std::string tuning;  // JSON array of the autotuner's measurements
+  std::mutex mu;       // one run (or describe) at a time: they share the stream and buffers
   ~fuel_amd_batch() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -743,6 +744,7 @@ int32_t fuel_amd_batch_create(const fuel_amd_problem* problems, int32_t count, i
 
 int32_t fuel_amd_batch_run(fuel_amd_batch* b) {
   if (!b) return fail(FUEL_AMD_ERR_INVALID, "batch must not be NULL");
+  std::lock_guard<std::mutex> lock(b->mu);
   int32_t rc = check_device(b->device);
   if (rc) return rc;
   return run_once(b);
@@ -750,6 +752,7 @@ int32_t fuel_amd_batch_run(fuel_amd_batch* b) {
 
 int32_t fuel_amd_batch_describe(fuel_amd_batch* b, char* text, int32_t len) {
   if (!b || !text || len <= 0) return fail(FUEL_AMD_ERR_INVALID, "NULL argument");
+  std::lock_guard<std::mutex> lock(b->mu);
   std::string s = "{\"graph\": ";
   s += (b->use_graph && b->exec) ? "true" : "false";
   s += ", \"groups\": [";

@@ -48,6 +48,7 @@ struct Request {
   std::string err;
   int64_t batch_queries = 0;
   bool done = false;
+  bool claimed = false;  // a wait call owns the ticket
 };
 
 // Page-locked host memory, grown on demand (allocated, never registered).
@@ -300,8 +301,12 @@ int32_t fuel_amd_server_wait(fuel_amd_server* s, uint64_t ticket, int64_t* out_i
     auto it = s->live.find(ticket);
     if (it == s->live.end()) return fail(FUEL_AMD_ERR_INVALID, "unknown ticket");
     r = it->second;
+    // one wait per ticket: a second caller would race the first for the request
+    if (r->claimed) return fail(FUEL_AMD_ERR_INVALID, "ticket is already being waited for");
+    r->claimed = true;
     s->cv_done.wait(lk, [&] { return r->done; });
-    s->live.erase(it);
+    // by key: submits during the wait may have rehashed the map
+    s->live.erase(ticket);
   }
   std::unique_ptr<Request> hold(r);
   if (r->rc) return fail(r->rc, r->err);

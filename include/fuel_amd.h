@@ -263,11 +263,13 @@ int32_t fuel_amd_server_submit(fuel_amd_server* server, const void* query, int64
 /* Blocks until the request is answered, writes int64 [m,k] indices and
  * [m,k] distances (index dtype; -1 / +inf padding as fuel_amd_knn), and
  * releases the ticket.  latency_ms and batch_queries (queries of the launch
- * that answered it) may be NULL. */
+ * that answered it) may be NULL.  One wait per ticket: a second concurrent
+ * wait on the same ticket fails with FUEL_AMD_ERR_INVALID. */
 int32_t fuel_amd_server_wait(fuel_amd_server* server, uint64_t ticket, int64_t* out_idx, void* out_dist,
                              double* latency_ms, int64_t* batch_queries);
 int32_t fuel_amd_server_stats(fuel_amd_server* server, int32_t reset, fuel_amd_serve_stats* out);
-/* Answers every submitted request, then stops the workers. */
+/* Answers every submitted request, then stops the workers.  No other call on
+ * the server may be in progress or follow. */
 int32_t fuel_amd_server_destroy(fuel_amd_server* server);
 
 /* ---- batched problems ---------------------------------------------------------
@@ -307,7 +309,8 @@ typedef struct fuel_amd_batch_opts {
 
 int32_t fuel_amd_batch_create(const fuel_amd_problem* problems, int32_t count, int32_t metric, int32_t exclude_self,
                               const fuel_amd_batch_opts* opts, fuel_amd_batch** out);
-/* One pass over every problem; synchronous. */
+/* One pass over every problem; synchronous.  Calls on one batch from several
+ * threads run one after another. */
 int32_t fuel_amd_batch_run(fuel_amd_batch* batch);
 /* The plan (per dimension: mode, widening, split size, work items, tuning times)
  * and the last run's statistics as JSON text (truncated to len - 1 bytes). */

@@ -152,3 +152,28 @@ def test_cancel_and_resume(gpu, oracle, problems):
     b.run()
     _check(oracle, problems[:5], b.results(), 6)
     b.close()
+
+
+def test_runs_from_two_threads(gpu, oracle, problems):
+    # calls on one batch from several threads run one after another
+    import threading
+    from fuel_amd import BatchKnn
+    b = BatchKnn(problems[:6], 8, mode="bf16r")
+    errs = []
+
+    def go():
+        try:
+            for _ in range(3):
+                b.run()
+        except Exception as e:  # surfaced below
+            errs.append(e)
+
+    ths = [threading.Thread(target=go) for _ in range(2)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    assert not errs, errs
+    _check(oracle, problems[:6], b.results(), 8)
+    assert b.describe()["runs"] == 6
+    b.close()

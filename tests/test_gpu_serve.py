@@ -170,3 +170,33 @@ def test_large_k_through_the_server(gpu, oracle, k):
         gi, gd, _, _ = srv.wait(srv.submit(Q, k))
     oi, od = oracle.knn(X, Q, k, "sqeuclidean", "none")
     assert np.array_equal(gi, oi) and gd.tobytes() == od.tobytes()
+
+
+def test_one_wait_per_ticket(gpu):
+    # a second wait on a ticket fails cleanly (never a double release), whether it
+    # comes while the first waits or after the ticket was released
+    from fuel_amd import Server, _lib
+    X = _mixture(200_000, 64, 20, 95)
+    Q = _mixture(2000, 64, 20, 96)
+    ix = gpu.Index(X, "euclidean", mode="bf16")
+    lib = _lib.load()
+    with Server(ix, workers=1, max_batch=4096) as srv:
+        for _ in range(3):
+            t = srv.submit(Q, 10)
+            rcs = []
+            outs = [(np.empty((2000, 10), np.int64), np.empty((2000, 10), np.float32)) for _ in range(2)]
+
+            def waiter(o):
+                rcs.append(lib.fuel_amd_server_wait(srv.handle, t, o[0].ctypes.data, o[1].ctypes.data, None, None))
+
+            ths = [threading.Thread(target=waiter, args=(o,)) for o in outs]
+            for th in ths:
+                th.start()
+            for th in ths:
+                th.join()
+            assert sorted(rcs) == sorted([_lib.OK, _lib.ERR_INVALID]), rcs
+            srv._k.pop(t, None)
+            # the waiter that succeeded holds the answer of the synchronous entry point
+            ai, ad = ix.knn(Q[:50], 10)
+            good = [o for o in outs if np.array_equal(o[0][:50], ai)]
+            assert good and good[0][1][:50].tobytes() == ad.tobytes()
