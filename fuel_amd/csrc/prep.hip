@@ -131,8 +131,10 @@ __global__ __launch_bounds__(256) void prep_query_kernel(PrepQArgs a) {
   const size_t es = a.bf16 ? 2 : 4;
   // no operand copy for the residual operand (the scan builds it per list)
   char* dst = a.Qop ? reinterpret_cast<char*>(a.Qop) + size_t(row) * a.Kp * es : nullptr;
-  if (lane == 0 && a.gthr)  // +inf, or the caller's threshold for this slot
-    a.gthr[row] = (a.theta && row < a.m) ? ord_f32(a.theta[row]) : ord_f32(INFINITY);
+  // +inf, or the caller's threshold for this slot; a padding slot starts below
+  // every key, so that it keeps no candidates (its results are never read)
+  if (lane == 0 && a.gthr)
+    a.gthr[row] = row >= a.m ? ord_f32(-MFMA_KEY_LIMIT) : (a.theta ? ord_f32(a.theta[row]) : ord_f32(INFINITY));
   if (row >= a.m) {
     if (!dst) return;
     for (int c = lane; c < a.Kp; c += 64) {

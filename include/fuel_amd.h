@@ -256,6 +256,8 @@ typedef struct fuel_amd_serve_stats {
   double busy_s;     /* seconds with at least one batch in flight */
 } fuel_amd_serve_stats;
 
+/* The index must outlive the server (destroy the server first); other calls
+ * on the index may run while the server does. */
 int32_t fuel_amd_server_create(fuel_amd_index* index, const fuel_amd_server_opts* opts, fuel_amd_server** out);
 /* Copies m query rows (index dtype, row-major, d per row); never blocks on the
  * GPU.  m must not exceed max_batch. */
