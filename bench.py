@@ -337,6 +337,8 @@ def secondary(args):
                                                   for m in {t["mode"] for t in info["tuning"] if t["d"] == dd}}
                                         for dd in sorted({t["d"] for t in info["tuning"]})},
                      "fallback_queries": info["last_fallback_queries"], "exact_queries": info["last_exact_queries"],
+                     "last_pass_split_ms": {"level0": info["last_level0_ms"],
+                                            "chain": round(info["last_ms"] - info["last_level0_ms"], 3)},
                      "roofline": {"bound": "mfma", "achieved": round(achieved / 1e12, 2),
                                   "peak": PEAKS["bf16"][0] / 1e12, "unit": "TFLOP/s",
                                   "frac": round(achieved / PEAKS["bf16"][0], 4),

@@ -98,7 +98,8 @@ struct fuel_amd_batch {
   hipGraphExec_t exec = nullptr;
   fuel_amd_index shell;  // borrows one problem's rows at a time (certification chain, exact path)
   std::vector<int> h_fail;
-  double last_ms = 0.0;
+  double last_ms = 0.0, last_graph_ms = 0.0;  // whole pass; level 0 up to its certification counts
+  std::chrono::steady_clock::time_point t_launch;
   int64_t last_fallback = 0, last_exact = 0, runs = 0;
   std::string tuning;  // JSON array of the autotuner's measurements
   std::mutex mu;       // one run (or describe) at a time: they share the stream and buffers
@@ -518,6 +519,7 @@ int32_t finish(fuel_amd_batch* b) {
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) return hip_fail(e, "certification counts");
   }
+  b->last_graph_ms = std::chrono::duration<double, std::milli>(Clock::now() - b->t_launch).count();
   int64_t fallback = 0, exact = 0;
   for (int i = 0; i < np; ++i) {
     const auto& pp = b->plan[i];
@@ -561,6 +563,7 @@ int32_t finish(fuel_amd_batch* b) {
 
 int32_t run_once(fuel_amd_batch* b) {
   const auto t0 = Clock::now();
+  b->t_launch = t0;
   hipError_t e;
   if (b->use_graph && b->exec)
     e = hipGraphLaunch(b->exec, b->stream);
@@ -770,11 +773,11 @@ int32_t fuel_amd_batch_describe(fuel_amd_batch* b, char* text, int32_t len) {
   }
   int64_t nexact = 0;
   for (const auto& pp : b->plan) nexact += pp.exact ? 1 : 0;
-  char tail[256];
+  char tail[320];
   std::snprintf(tail, sizeof(tail),
-                "], \"exact_problems\": %lld, \"runs\": %lld, \"last_ms\": %.3f, \"last_fallback_queries\": %lld, "
+                "], \"exact_problems\": %lld, \"runs\": %lld, \"last_ms\": %.3f, \"last_level0_ms\": %.3f, \"last_fallback_queries\": %lld, "
                 "\"last_exact_queries\": %lld, \"tuning\": ",
-                (long long)nexact, (long long)b->runs, b->last_ms, (long long)b->last_fallback,
+                (long long)nexact, (long long)b->runs, b->last_ms, b->last_graph_ms, (long long)b->last_fallback,
                 (long long)b->last_exact);
   s += tail;
   s += b->tuning;
