@@ -136,3 +136,21 @@ def test_huge_norms_take_the_exact_scan(gpu, oracle, mode):
     Q = (rng.random((50, 8)) * 2e18).astype(np.float32)
     st = _run(gpu, oracle, X, Q, 7, "euclidean", 0, mode)
     assert st["exact_queries"] == 50, st
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("m", [1, 63, 65, 191, 193, 255, 257])
+def test_partial_query_blocks(gpu, oracle, mode, m):
+    # a launch fills whole query blocks (64 slots for one-wave workgroups, 256
+    # otherwise); the padding slots start below every key and keep nothing, and
+    # the real queries of the last block are answered as if alone, at every level
+    # of the chain (clustered rows send some of them to the second pass)
+    from fuel_amd import workloads
+    rng = np.random.default_rng(1000 + m)
+    X = workloads.gaussian_mixture(rng, 30_000, 48, 12, -6.0, 6.0)
+    Q = workloads.gaussian_mixture(rng, m, 48, 12, -6.0, 6.0)
+    _run(gpu, oracle, X, Q, 10, "euclidean", 0, mode)
+    ix = gpu.Index(X, "euclidean", mode=mode)
+    li, ld = ix.knn(Q[-1:], 10, 0)  # the last query alone
+    gi, gd = ix.knn(Q, 10, 0)
+    assert np.array_equal(li[0], gi[-1]) and ld[0].tobytes() == gd[-1].tobytes()
