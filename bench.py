@@ -61,6 +61,7 @@ def parse():
     p.add_argument("--serve-p99-seconds", type=float, default=3.0,
                    help="seconds of serving at 10,000 queries/s measured for the headline line's p99 (0: skip)")
     p.add_argument("--serve-rows", type=int, default=10_000_000, help="rows of the served index (c5)")
+    p.add_argument("--serve-workers", type=int, default=2, help="serving worker threads (c5 and the headline p99)")
     p.add_argument("--host-path", action="store_true",
                    help="only time the host-pointer entry point (queries and results cross PCIe each call)")
     p.add_argument("--copy-probe", type=int, default=0,
@@ -231,7 +232,7 @@ def serve_headline(args):
     from fuel_amd import Server, _lib
     X, pool, rng = c5_data(args)
     ix = _lib.Index(X, "euclidean", mode=args.serve_mode)
-    with Server(ix, workers=2, max_batch=4096) as srv:
+    with Server(ix, workers=args.serve_workers, max_batch=4096) as srv:
         serve_run(srv, pool, 2000, 0.5, rng)  # warm-up
         r, _ = serve_run(srv, pool, 10000, args.serve_p99_seconds, rng)
     ix.close()
@@ -356,7 +357,7 @@ def secondary(args):
         ix = _lib.Index(X, "euclidean", mode=args.serve_mode)
         print(f"# c5 data + index in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
         rates, kept = [], []
-        with Server(ix, workers=2, max_batch=4096) as srv:
+        with Server(ix, workers=args.serve_workers, max_batch=4096) as srv:
             serve_run(srv, pool, 2000, 0.5, rng)  # warm-up (workspaces, operand copy)
             for lam in (2000, 5000, 10000, 20000):
                 r, kp = serve_run(srv, pool, lam, args.serve_seconds, rng, keep=2)
@@ -382,7 +383,7 @@ def secondary(args):
         t_c = time.perf_counter() - t0
         line.update({"config": f"serving: resident index {X.shape[0]:,} x {X.shape[1]} f32 (50-component mixture), "
                                "k=10, Poisson arrivals, geometric batch sizes (mean 64, cap 4096)",
-                     "mode": args.serve_mode, "workers": 2, "rates": rates,
+                     "mode": args.serve_mode, "workers": args.serve_workers, "rates": rates,
                      "saturated_qps": round(sat, 1), "saturated_p99_ms": round(sst["p99_ms"], 3),
                      "identical_sample": same, "checked_queries": nchk,
                      "cpu_baseline": {"value": round(nchk / t_c, 2), "unit": "queries/s", "cores": threads,
