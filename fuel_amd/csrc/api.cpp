@@ -71,14 +71,17 @@ int32_t check_device(int dev) {
 }
 
 int device_cu_count(int dev) {
-  static int cached[64] = {0};
-  if (dev >= 0 && dev < 64 && cached[dev]) return cached[dev];
+  static std::atomic<int> cached[64];  // zero-initialised; callers on several threads
+  if (dev >= 0 && dev < 64) {
+    const int c = cached[dev].load(std::memory_order_relaxed);
+    if (c) return c;
+  }
   int cu = 256;
   if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
     (void)hipGetLastError();
     cu = 256;
   }
-  if (dev >= 0 && dev < 64) cached[dev] = cu;
+  if (dev >= 0 && dev < 64) cached[dev].store(cu, std::memory_order_relaxed);
   return cu;
 }
 

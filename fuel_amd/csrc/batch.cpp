@@ -33,6 +33,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "api_internal.h"
@@ -42,6 +43,7 @@ namespace fa {
 namespace {
 
 constexpr int KP_LIMIT = 128;  // widest kept list of the MFMA path
+constexpr int kChainLanes = 4;  // certification chains of different problems run concurrently
 using Clock = std::chrono::steady_clock;
 
 struct Cfg {
@@ -96,7 +98,13 @@ struct fuel_amd_batch {
   fa::DevBuf pool, keep, fail_counts, counter;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  fuel_amd_index shell;  // borrows one problem's rows at a time (certification chain, exact path)
+  // Certification chain: shells that borrow one problem's rows at a time, each
+  // with its own stream, so that the chains of several problems overlap.
+  struct ChainLane {
+    fuel_amd_index shell;
+    hipStream_t stream = nullptr;
+  };
+  std::vector<std::unique_ptr<ChainLane>> lanes;
   std::vector<int> h_fail;
   double last_ms = 0.0, last_graph_ms = 0.0;  // whole pass; level 0 up to its certification counts
   std::chrono::steady_clock::time_point t_launch;
@@ -110,7 +118,13 @@ struct fuel_amd_batch {
       (void)hipStreamSynchronize(stream);
       (void)hipStreamDestroy(stream);
     }
-    shell.stream = nullptr;
+    for (auto& L : lanes) {
+      if (L->stream) {
+        (void)hipStreamSynchronize(L->stream);
+        (void)hipStreamDestroy(L->stream);
+      }
+      L->shell.stream = nullptr;
+    }
   }
 };
 
@@ -465,8 +479,7 @@ hipError_t enqueue_all(fuel_amd_batch* b, hipStream_t st) {
   return hipSuccess;
 }
 
-void point_shell(fuel_amd_batch* b, int i, int mode) {
-  auto& s = b->shell;
+void point_shell(fuel_amd_batch* b, fuel_amd_index& s, hipStream_t st, int i, int mode) {
   const auto& pr = b->probs[i];
   s.device = b->device;
   s.dtype = FUEL_AMD_F32;
@@ -480,7 +493,7 @@ void point_shell(fuel_amd_batch* b, int i, int mode) {
   s.X.borrow(pr.data, size_t(pr.n) * pr.ld * 4);
   s.mop[0].ready.store(false);
   s.mop[1].ready.store(false);
-  s.stream = b->stream;
+  s.stream = st;
 }
 
 // Norms beyond the MFMA scan's finite sentinels: such problems take the exact
@@ -520,44 +533,77 @@ int32_t finish(fuel_amd_batch* b) {
     if (e != hipSuccess) return hip_fail(e, "certification counts");
   }
   b->last_graph_ms = std::chrono::duration<double, std::milli>(Clock::now() - b->t_launch).count();
-  int64_t fallback = 0, exact = 0;
-  for (int i = 0; i < np; ++i) {
-    const auto& pp = b->plan[i];
-    const auto& pr = b->probs[i];
-    if (cancelled()) return fail(FUEL_AMD_ERR_INTERRUPTED, "interrupted");
-    if (pp.bucket >= 0 && b->h_fail[i] > 0) {
-      fallback += b->h_fail[i];
-      // bf16 / f32: the same kind from the certified threshold, then the chain;
-      // residual: the f32 form (its threshold is in other key units), then exact
-      point_shell(b, i, pp.kind == 0 ? FUEL_AMD_MODE_BF16 : FUEL_AMD_MODE_FP32);
-      const int64_t before = b->shell.ctx.stats.exact_queries;
-      const bool resid = pp.kind == 2;
-      int32_t rc;
-      if (resid && !mfma_supported(&b->shell, 1, pr.k, b->exclude)) {
-        rc = knn_exact_rows(&b->shell, &b->shell.ctx, static_cast<const float*>(pr.data), pr.ld,
-                            at<int64_t>(b->keep, pp.o_fl), b->h_fail[i], 0, pr.k, b->exclude, pr.out_idx,
-                            static_cast<float*>(pr.out_dist), b->stream);
-        b->shell.ctx.stats.exact_queries += b->h_fail[i];
-      } else {
-        rc = knn_mfma(&b->shell, &b->shell.ctx, static_cast<const float*>(pr.data), b->h_fail[i], pr.ld, pr.k,
-                      b->exclude, resid ? 1 : pp.kind, pr.out_idx, static_cast<float*>(pr.out_dist), b->stream,
-                      at<int64_t>(b->keep, pp.o_fl), 0, resid ? 2 : 1, KP_LIMIT,
-                      resid ? nullptr : at<float>(b->keep, pp.o_ft));
-      }
-      if (rc) return rc;
-      exact += b->shell.ctx.stats.exact_queries - before;
-    } else if (pp.exact) {
-      point_shell(b, i, FUEL_AMD_MODE_EXACT);
-      int32_t rc = knn_run(&b->shell, &b->shell.ctx, pr.data, pr.n, pr.ld, pr.k, b->exclude, pr.out_idx, pr.out_dist,
-                           b->stream);
-      if (rc) return rc;
-      exact += pr.n;
-    }
+  if (np > 0 && cancelled()) return fail(FUEL_AMD_ERR_INTERRUPTED, "interrupted");
+  std::vector<int> work;  // problems with queries left for the chain, or on the exact scan
+  for (int i = 0; i < np; ++i)
+    if ((b->plan[i].bucket >= 0 && b->h_fail[i] > 0) || (b->plan[i].bucket < 0 && b->plan[i].exact)) work.push_back(i);
+  const int nl = std::min<int>((int)work.size(), kChainLanes);
+  while ((int)b->lanes.size() < nl) {
+    auto L = std::make_unique<fuel_amd_batch::ChainLane>();
+    e = hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "chain stream");
+    b->lanes.push_back(std::move(L));
   }
-  e = hipStreamSynchronize(b->stream);
-  if (e != hipSuccess) return hip_fail(e, "batch fallback");
-  b->last_fallback = fallback;
-  b->last_exact = exact;
+  std::atomic<int> next{0};
+  std::atomic<int64_t> fallback{0}, exact{0};
+  std::vector<int32_t> lane_rc(nl, FUEL_AMD_OK);
+  std::vector<std::string> lane_err(nl);
+  // one lane: problems taken in turn from `work`, each through its whole chain
+  // on the lane's stream (the problems' outputs and fail lists are disjoint)
+  auto lane_main = [&](int l) {
+    (void)hipSetDevice(b->device);
+    auto& shell = b->lanes[l]->shell;
+    hipStream_t st = b->lanes[l]->stream;
+    int32_t rc = FUEL_AMD_OK;
+    for (;;) {
+      const int w = next.fetch_add(1);
+      if (w >= (int)work.size()) break;
+      if (cancelled()) {
+        rc = fail(FUEL_AMD_ERR_INTERRUPTED, "interrupted");
+        break;
+      }
+      const int i = work[w];
+      const auto& pp = b->plan[i];
+      const auto& pr = b->probs[i];
+      if (pp.bucket >= 0) {
+        fallback += b->h_fail[i];
+        // bf16 / f32: the same kind from the certified threshold, then the chain;
+        // residual: the f32 form (its threshold is in other key units), then exact
+        point_shell(b, shell, st, i, pp.kind == 0 ? FUEL_AMD_MODE_BF16 : FUEL_AMD_MODE_FP32);
+        const int64_t before = shell.ctx.stats.exact_queries;
+        const bool resid = pp.kind == 2;
+        if (resid && !mfma_supported(&shell, 1, pr.k, b->exclude)) {
+          rc = knn_exact_rows(&shell, &shell.ctx, static_cast<const float*>(pr.data), pr.ld,
+                              at<int64_t>(b->keep, pp.o_fl), b->h_fail[i], 0, pr.k, b->exclude, pr.out_idx,
+                              static_cast<float*>(pr.out_dist), st);
+          shell.ctx.stats.exact_queries += b->h_fail[i];
+        } else {
+          rc = knn_mfma(&shell, &shell.ctx, static_cast<const float*>(pr.data), b->h_fail[i], pr.ld, pr.k,
+                        b->exclude, resid ? 1 : pp.kind, pr.out_idx, static_cast<float*>(pr.out_dist), st,
+                        at<int64_t>(b->keep, pp.o_fl), 0, resid ? 2 : 1, KP_LIMIT,
+                        resid ? nullptr : at<float>(b->keep, pp.o_ft));
+        }
+        exact += shell.ctx.stats.exact_queries - before;
+      } else {
+        point_shell(b, shell, st, i, FUEL_AMD_MODE_EXACT);
+        rc = knn_run(&shell, &shell.ctx, pr.data, pr.n, pr.ld, pr.k, b->exclude, pr.out_idx, pr.out_dist, st);
+        exact += pr.n;
+      }
+      if (rc) break;
+    }
+    const hipError_t se = hipStreamSynchronize(st);
+    if (rc == FUEL_AMD_OK && se != hipSuccess) rc = hip_fail(se, "batch fallback");
+    lane_rc[l] = rc;
+    if (rc) lane_err[l] = fuel_amd_last_error();  // the message is per thread
+  };
+  std::vector<std::thread> threads;
+  for (int l = 1; l < nl; ++l) threads.emplace_back(lane_main, l);
+  if (nl > 0) lane_main(0);
+  for (auto& t : threads) t.join();
+  for (int l = 0; l < nl; ++l)
+    if (lane_rc[l]) return fail(lane_rc[l], lane_err[l]);
+  b->last_fallback = fallback.load();
+  b->last_exact = exact.load();
   return FUEL_AMD_OK;
 }
 
@@ -588,7 +634,6 @@ int32_t make_batch(const fuel_amd_problem* probs, const std::vector<int>& which,
   b->parts.resize(b->probs.size());
   hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
   if (e != hipSuccess) return hip_fail(e, "batch stream");
-  b->shell.stream = b->stream;
   int32_t rc = plan_batch(b.get(), cfg_by_d);
   if (rc) return rc;
   bool changed = false;
