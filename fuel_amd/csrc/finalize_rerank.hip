@@ -297,7 +297,11 @@ __global__ __launch_bounds__(64) void rerank_kernel(RerankArgs a) {
       }
     }
   }
-  wave_bitonic_sort<T, V>(y);
+  // at most 64 candidates sit in the first register: a one-register sort suffices
+  if (ncand <= 64)
+    wave_bitonic_sort<T, 1>(reinterpret_cast<T(&)[1]>(y[0]));
+  else
+    wave_bitonic_sort<T, V>(y);
 
   // 3. certification
   bool cert = true;
