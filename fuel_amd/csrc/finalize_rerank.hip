@@ -192,6 +192,22 @@ __device__ __forceinline__ float certified_threshold_resid(const RerankArgs& a, 
   return certify_resid(a, b, Tf, kk) ? Tf : INFINITY;
 }
 
+// Ascending sort of the first `count` of 64*V entries whose tail is C::MAX:
+// only the smallest register prefix that holds them is sorted (the rest is
+// already in place).
+template <typename T, int V>
+__device__ __forceinline__ void sort_prefix(T (&x)[V], int count) {
+  if (count <= 64 || V == 1) {
+    wave_bitonic_sort<T, 1>(reinterpret_cast<T(&)[1]>(x[0]));
+  } else if (V == 2 || count <= 128) {
+    wave_bitonic_sort<T, (V < 2 ? V : 2)>(reinterpret_cast<T(&)[(V < 2 ? V : 2)]>(x[0]));
+  } else if (V == 4 || count <= 256) {
+    wave_bitonic_sort<T, (V < 4 ? V : 4)>(reinterpret_cast<T(&)[(V < 4 ? V : 4)]>(x[0]));
+  } else {
+    wave_bitonic_sort<T, V>(x);
+  }
+}
+
 template <int V>
 __global__ __launch_bounds__(64) void rerank_kernel(RerankArgs a) {
   using C = Comp<float>;
@@ -273,7 +289,7 @@ __global__ __launch_bounds__(64) void rerank_kernel(RerankArgs a) {
   __syncthreads();
 #pragma unroll
   for (int v = 0; v < V; ++v) x[v] = v * 64 + lane < fill ? buf[v * 64 + lane] : C::MAX;
-  wave_bitonic_sort<T, V>(x);
+  sort_prefix<T, V>(x, fill);
   const int ncand = total < KP ? (int)total : KP;
   // Bound on the excluded rows: with a cut here, the KP-th kept key; otherwise
   // the scan's final shared threshold (every row a split dropped lies at or
@@ -297,11 +313,7 @@ __global__ __launch_bounds__(64) void rerank_kernel(RerankArgs a) {
       }
     }
   }
-  // at most 64 candidates sit in the first register: a one-register sort suffices
-  if (ncand <= 64)
-    wave_bitonic_sort<T, 1>(reinterpret_cast<T(&)[1]>(y[0]));
-  else
-    wave_bitonic_sort<T, V>(y);
+  sort_prefix<T, V>(y, ncand);
 
   // 3. certification
   bool cert = true;
