@@ -717,7 +717,7 @@ int32_t autotune(const fuel_amd_problem* probs, int count, int metric, int exclu
         }
       }
     }
-    // the fastest; among those within 10 % of it, the one with the fewest
+    // the fastest; among those within 5 % of it, the one with the fewest
     // certification fallbacks (each failing problem costs a host-synchronised
     // chain, and the whole group has many more problems than the sample)
     double fastest = 1e300;
@@ -726,7 +726,7 @@ int32_t autotune(const fuel_amd_problem* probs, int count, int metric, int exclu
     double best_ms = 1e300;
     int64_t best_fb = INT64_MAX;
     for (const auto& t : trials) {
-      if (t.ms > 1.10 * fastest) continue;
+      if (t.ms > 1.05 * fastest) continue;
       if (t.fallback < best_fb || (t.fallback == best_fb && t.ms < best_ms)) {
         best_fb = t.fallback;
         best_ms = t.ms;
