@@ -492,6 +492,10 @@ constexpr int scan_row_blocks(int kind, int ks, bool resid) {
   return (kind == 0 && !resid ? ks <= 12 : ks <= 16) ? 2 : 1;
 }
 constexpr bool scan_two_waves(int kind, int ks, bool resid) { return kind == 0 ? ks <= (resid ? 12 : 16) : ks <= 8; }
+// Shapes whose fragments fit 128 registers run four workgroups (four waves per
+// SIMD): at few K steps the selection, not the MFMA, bounds the scan, and more
+// waves hide its latency.
+constexpr bool scan_four_wg(int kind, int ks) { return kind == 0 ? ks <= 4 : ks <= 2; }
 
 // K step buckets: bf16 covers 16 coordinates per step, f32 covers 8.  Up to 16
 // steps a wave holds 2 query groups and the tile has 64 rows; beyond, 1 query
@@ -514,7 +518,7 @@ int mfma_ksteps_bucket(int kind, int d) {
 int mfma_queries_per_block(int kind, int ksteps, int waves) { return waves * 32 * scan_groups(ksteps); }
 int mfma_rows_per_tile(int kind, int ksteps, bool resid) { return 32 * scan_row_blocks(kind, ksteps, resid); }
 int mfma_workgroups_per_cu(int kind, int ksteps, int waves, bool resid) {
-  const int per = (kind == 0 && ksteps <= 4) ? 4 : (scan_two_waves(kind, ksteps, resid) ? 2 : 1);
+  const int per = scan_four_wg(kind, ksteps) ? 4 : (scan_two_waves(kind, ksteps, resid) ? 2 : 1);
   return waves == 1 ? 4 * std::min(per, 2) : per;
 }
 bool mfma_one_wave_supported(int kind, int ksteps, int V, bool resid) {
@@ -536,7 +540,7 @@ static hipError_t launch_ks(const ScanMfmaArgs& a, int V, int nsplit, hipStream_
   }
   if (a.waves == 1) return hipErrorInvalidValue;
   const dim3 grid((unsigned)(a.nqb * nsplit)), block(256);
-  if constexpr (KIND == 0 && KS <= 4) {
+  if constexpr (scan_four_wg(KIND, KS)) {
     if (V == 4)
       hipLaunchKernelGGL((scan_mfma_kernel_4wg<KIND, KS, A, R, 4, RESID>), grid, block, 0, st, a);
     else
@@ -560,7 +564,7 @@ static hipError_t launch_batch_ks(const ScanBatchArgs& b, int V, int grid, hipSt
   constexpr int A = scan_groups(KS);
   constexpr int R = scan_row_blocks(KIND, KS, RESID);
   constexpr bool two_waves = scan_two_waves(KIND, KS, RESID);
-  if constexpr (KIND == 0 && KS <= 4) {
+  if constexpr (scan_four_wg(KIND, KS)) {
     if (V == 4)
       hipLaunchKernelGGL((scan_mfma_batch_kernel_4wg<KIND, KS, A, R, 4, RESID>), dim3(grid), dim3(256), 0, st, b);
     else
