@@ -488,14 +488,15 @@ __global__ __launch_bounds__(256, 4) void scan_mfma_batch_kernel_4wg(ScanBatchAr
 // The residual variants keep 64-row tiles and one wave per SIMD from 14 steps
 // (their list-change rebuild of the fragments leaves no room for two).
 constexpr int scan_groups(int ks) { return ks <= 16 ? 2 : 1; }
-constexpr int scan_row_blocks(int kind, int ks, bool resid) {
-  return (kind == 0 && !resid ? ks <= 12 : ks <= 16) ? 2 : 1;
-}
 constexpr bool scan_two_waves(int kind, int ks, bool resid) { return kind == 0 ? ks <= (resid ? 12 : 16) : ks <= 8; }
 // Shapes whose fragments fit 128 registers run four workgroups (four waves per
 // SIMD): at few K steps the selection, not the MFMA, bounds the scan, and more
-// waves hide its latency.
+// waves hide its latency.  They take 32-row tiles (half the accumulators) so
+// that the 128 registers hold them without spills.
 constexpr bool scan_four_wg(int kind, int ks) { return kind == 0 ? ks <= 4 : ks <= 2; }
+constexpr int scan_row_blocks(int kind, int ks, bool resid) {
+  return scan_four_wg(kind, ks) ? 1 : ((kind == 0 && !resid ? ks <= 12 : ks <= 16) ? 2 : 1);
+}
 
 // K step buckets: bf16 covers 16 coordinates per step, f32 covers 8.  Up to 16
 // steps a wave holds 2 query groups and the tile has 64 rows; beyond, 1 query
