@@ -19,6 +19,13 @@ _lib = None
 
 METRICS = {"euclidean": 0, "l2": 0, "sqeuclidean": 1, "squared_euclidean": 1}
 EXCLUDE = {"none": 0, "for_each_knn": 1, "python_linear": 2}
+# distance arithmetic: "avx" = src/distance/squared_euclidean.rs:13-67 (x86 hosts with
+# AVX+FMA), "fallback" = :99-133 (every other host); bit 4 of the metric code
+ARITH = {"avx": 0, "fallback": 16}
+
+
+def _metric(metric, arith="avx"):
+    return METRICS[metric] | ARITH[arith]
 
 
 def build(force=False):
@@ -53,6 +60,8 @@ def _load():
     lib.oracle_sqdist_f32_fallback.argtypes = [P, P, I64]
     lib.oracle_sqdist_f64.restype = ctypes.c_double
     lib.oracle_sqdist_f64.argtypes = [P, P, I64]
+    lib.oracle_sqdist_f64_fallback.restype = ctypes.c_double
+    lib.oracle_sqdist_f64_fallback.argtypes = [P, P, I64]
     lib.oracle_sqdist_rows_f32.argtypes = [P, I64, I64, P, P]
     lib.oracle_sqdist_fp64_pairs.argtypes = [P, I64, P, P, P, I64, P]
     for t in ("f32", "f64"):
@@ -91,7 +100,8 @@ def sqdist(a, b, kind="avx"):
     lib = _load()
     if np.asarray(a).dtype == np.float64:
         a, b = _c(a, np.float64), _c(b, np.float64)
-        return lib.oracle_sqdist_f64(a.ctypes.data, b.ctypes.data, a.shape[0])
+        fn = lib.oracle_sqdist_f64_fallback if kind == "fallback" else lib.oracle_sqdist_f64
+        return np.float64(fn(a.ctypes.data, b.ctypes.data, a.shape[0]))
     a, b = _c(a, np.float32), _c(b, np.float32)
     fn = {"avx": lib.oracle_sqdist_f32, "lanes": lib.oracle_sqdist_f32_lanes,
           "fallback": lib.oracle_sqdist_f32_fallback}[kind]
@@ -117,7 +127,7 @@ def sqdist_fp64(X, Q, qi, xi):
     return out
 
 
-def knn(X, Q, k, metric="euclidean", exclude="none", threads=None):
+def knn(X, Q, k, metric="euclidean", exclude="none", threads=None, arith="avx"):
     """Python-packed kNN: (int64[m,k] padded -1, F[m,k] padded +inf)."""
     lib = _load()
     dt = np.float64 if np.asarray(X).dtype == np.float64 else np.float32
@@ -126,7 +136,7 @@ def knn(X, Q, k, metric="euclidean", exclude="none", threads=None):
     idx = np.empty((m, k), np.int64)
     dist = np.empty((m, k), dt)
     getattr(lib, f"oracle_knn_{_tag(dt)}")(
-        X.ctypes.data, X.shape[0], X.shape[1], Q.ctypes.data, m, k, METRICS[metric],
+        X.ctypes.data, X.shape[0], X.shape[1], Q.ctypes.data, m, k, _metric(metric, arith),
         EXCLUDE[exclude] if isinstance(exclude, str) else int(exclude),
         threads or default_threads(), idx.ctypes.data, dist.ctypes.data)
     return idx, dist
@@ -143,19 +153,19 @@ def _csr_to_lists(c, dt, m):
     return offs, idx, dist
 
 
-def knn_ties(X, Q, k, metric="euclidean", threads=None):
+def knn_ties(X, Q, k, metric="euclidean", threads=None, arith="avx"):
     """KNNHeap rows with all ties, as CSR (offsets, index, dist)."""
     lib = _load()
     dt = np.float64 if np.asarray(X).dtype == np.float64 else np.float32
     X, Q = _c(X, dt), _c(Q, dt)
     c = _Csr()
     getattr(lib, f"oracle_knn_ties_{_tag(dt)}")(
-        X.ctypes.data, X.shape[0], X.shape[1], Q.ctypes.data, Q.shape[0], k, METRICS[metric],
+        X.ctypes.data, X.shape[0], X.shape[1], Q.ctypes.data, Q.shape[0], k, _metric(metric, arith),
         threads or default_threads(), ctypes.byref(c))
     return _csr_to_lists(c, dt, Q.shape[0])
 
 
-def range_search(X, Q, radius, metric="euclidean", exclude_self=False, threads=None):
+def range_search(X, Q, radius, metric="euclidean", exclude_self=False, threads=None, arith="avx"):
     """Range rows as CSR (offsets, index, dist)."""
     lib = _load()
     dt = np.float64 if np.asarray(X).dtype == np.float64 else np.float32
@@ -163,28 +173,28 @@ def range_search(X, Q, radius, metric="euclidean", exclude_self=False, threads=N
     c = _Csr()
     getattr(lib, f"oracle_range_{_tag(dt)}")(
         X.ctypes.data, X.shape[0], X.shape[1], Q.ctypes.data, Q.shape[0], float(radius),
-        METRICS[metric], int(bool(exclude_self)), threads or default_threads(), ctypes.byref(c))
+        _metric(metric, arith), int(bool(exclude_self)), threads or default_threads(), ctypes.byref(c))
     return _csr_to_lists(c, dt, Q.shape[0])
 
 
-def pairwise(X, metric="euclidean", form="square", threads=None):
+def pairwise(X, metric="euclidean", form="square", threads=None, arith="avx"):
     lib = _load()
     dt = np.float64 if np.asarray(X).dtype == np.float64 else np.float32
     X = _c(X, dt)
     n = X.shape[0]
     out = np.empty((n, n) if form == "square" else (n * (n - 1) // 2,), dt)
     getattr(lib, f"oracle_pairwise_{_tag(dt)}")(
-        X.ctypes.data, n, X.shape[1], METRICS[metric], 0 if form == "square" else 1,
+        X.ctypes.data, n, X.shape[1], _metric(metric, arith), 0 if form == "square" else 1,
         threads or default_threads(), out.ctypes.data)
     return out
 
 
-def lof(X, k, metric="euclidean", threads=None):
+def lof(X, k, metric="euclidean", threads=None, arith="avx"):
     lib = _load()
     dt = np.float64 if np.asarray(X).dtype == np.float64 else np.float32
     X = _c(X, dt)
     out = np.empty(X.shape[0], dt)
-    getattr(lib, f"oracle_lof_{_tag(dt)}")(X.ctypes.data, X.shape[0], X.shape[1], k, METRICS[metric],
+    getattr(lib, f"oracle_lof_{_tag(dt)}")(X.ctypes.data, X.shape[0], X.shape[1], k, _metric(metric, arith),
                                            threads or default_threads(), out.ctypes.data)
     return out
 

@@ -9,10 +9,16 @@
 // Build: oracle/Makefile  (g++ -O2 -mavx2 -mfma -ffp-contract=off, never fast-math)
 //
 // Pinning (how each part is checked, see tests/test_oracle.py):
-//   sqdist_f32 / sqdist_f64 ......... pinned: the exact pairwise matrices of
-//                                     src/math/mod.rs:672-690; an exact-integer
-//                                     closed form; a scalar fmaf restatement;
-//                                     brute-force fp64 within the rounding bound.
+//   sqdist_f32 / sqdist_f64 / ....... pinned bit for bit by an exact-rational
+//   sqdist_fallback                   emulator of each IEEE operation
+//                                     (tests/test_oracle_arith.py) on scale-mixed
+//                                     inputs where unfused lanes, 4-vs-8 lanes, one
+//                                     sequential accumulator, another lane fold or
+//                                     another tail form each change the bits; also
+//                                     the exact pairwise matrices of
+//                                     src/math/mod.rs:672-690, an exact-integer
+//                                     closed form and brute-force fp64 within the
+//                                     rounding bound.
 //   KNNHeap / linear_scan_knn ....... pinned: src/search/linear_scan.rs:252-265
 //                                     (order [1,0,2] with a tie), the brute-force
 //                                     semantics of src/search/vptree/tests/shared.rs:20-42
@@ -147,11 +153,15 @@ inline double sqdist(const double* a, const double* b, int64_t d) { return sqdis
 
 // a4/a5: Euclidean = sqrt of a1 (src/distance/euclidean.rs:7-9, IEEE correctly
 // rounded); SquaredEuclidean = a1 (src/distance/squared_euclidean.rs:137-157).
-// metric 0 = euclidean / l2, 1 = sqeuclidean / squared_euclidean.
+// metric bit 0: 0 = euclidean / l2, 1 = sqeuclidean / squared_euclidean.
+// metric bit 4 (ARITH_FALLBACK): the distance takes a3's rounding, the branch of
+// squared_euclidean_distance (src/distance/squared_euclidean.rs:71-97) that fuel
+// runs on hosts without AVX+FMA (every non-x86 host), instead of a1/a2.
+constexpr int ARITH_FALLBACK = 16;
 template <typename F>
 inline F distance(const F* a, const F* b, int64_t d, int metric) {
-  F s = sqdist(a, b, d);
-  return metric == 0 ? std::sqrt(s) : s;
+  F s = (metric & ARITH_FALLBACK) ? sqdist_fallback<F>(a, b, d) : sqdist(a, b, d);
+  return (metric & 1) == 0 ? std::sqrt(s) : s;
 }
 
 // ---------------------------------------------------------------------------
@@ -792,6 +802,9 @@ float oracle_sqdist_f32_lanes(const float* a, const float* b, int64_t d) {
 double oracle_sqdist_f64(const double* a, const double* b, int64_t d) { return sqdist_f64_avx(a, b, d); }
 float oracle_sqdist_f32_fallback(const float* a, const float* b, int64_t d) {
   return sqdist_fallback<float>(a, b, d);
+}
+double oracle_sqdist_f64_fallback(const double* a, const double* b, int64_t d) {
+  return sqdist_fallback<double>(a, b, d);
 }
 
 // Row-wise restated values of one query against every point (no sqrt).
