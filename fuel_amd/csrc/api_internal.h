@@ -108,6 +108,11 @@ struct fuel_amd_index {
     fa::DevBuf cents, rperm, tile_list, lconsts, lcount;  // 64-row block, per-list bound constants, rows
   } mop[3];
   std::mutex prep_mu;
+  // FUEL_AMD_MODE_AUTO: the mode chosen by the first kNN call's probe (-1 until
+  // then), with the probe's outcome
+  std::atomic<int> auto_mode{-1};
+  std::mutex auto_mu;
+  int64_t auto_probe_rows = 0, auto_probe_fail = 0;
   fa::QueryCtx ctx;  // the synchronous entry points' query workspaces and stats
   // profiling: event pairs around the dominant kernel of each call
   bool timing = false;
@@ -168,10 +173,27 @@ int32_t outlier_run(fuel_amd_index* ix, int k, int knn_outlier, void* out_scores
 // position in the certification chain (which also selects the query
 // workspace); d_theta = optional per-slot starting thresholds (key units), the
 // certified second pass of queries that failed a previous level.
+// probe = true runs the first level only: the queries whose certificate fails
+// are counted (stats.fallback_queries) but not answered.
 int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, int64_t ldq, int k, int exclude,
                  int kind, int64_t* out_idx, float* out_dist, hipStream_t st, const int64_t* d_rows = nullptr,
-                 int64_t qbase = 0, int level = 0, int kp = 0, const float* d_theta = nullptr);
+                 int64_t qbase = 0, int level = 0, int kp = 0, const float* d_theta = nullptr, bool probe = false);
+// The mode FUEL_AMD_MODE_AUTO stands for on this index (decided once, by a
+// probe with this k on the first call): FUEL_AMD_MODE_BF16, _BF16R or _EXACT.
+int32_t resolve_auto_mode(fuel_amd_index* ix, QueryCtx* ctx, int k, hipStream_t st, int* mode);
 bool mfma_supported(const fuel_amd_index* ix, int kind, int k, int exclude);
+// Launch plan of the next chunk of an MFMA kNN call: `remaining` of m_total
+// queries, operand type mk (0 bf16, 1 f32), residual operand or not, K steps,
+// KP kept candidates, n_pad operand rows, cus compute units.
+struct MfmaChunkPlan {
+  int V, NS, csub, waves, qpb, rows_tile, wg_per_cu, nsplit;
+  int64_t chunk, nqb, nq_pad, rps;
+  size_t slots_bytes;  // candidate slots the chunk allocates
+};
+MfmaChunkPlan plan_mfma_chunk(int mk, bool resid, int ks, int64_t m_total, int64_t remaining, int KP, int64_t n_pad,
+                              int cus);
+// Padded residual layout of lists with cnt[j] rows (returns the operand rows).
+int64_t resid_layout(const std::vector<uint32_t>& cnt, std::vector<int64_t>& pad, std::vector<int>& tl);
 // Residual operand layout of n rows (f32, pitch ld): centroids (L x d), padded
 // position -> row (RPERM_NONE for padding) and the list of every 64-row block.
 int32_t build_partition(const float* X, int64_t ld, int64_t n, int d, hipStream_t st, DevBuf& cents,

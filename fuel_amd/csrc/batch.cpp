@@ -306,6 +306,30 @@ int32_t plan_batch(fuel_amd_batch* b, const std::map<int, Cfg>& cfg_by_d) {
       sa.slots = at<uint64_t>(b->pool, pp.o_slots);
       sa.lcnts = at<int>(b->pool, pp.o_lcnts);
       sa.gthr = at<uint32_t>(b->pool, pp.o_gthr);
+      {
+        // the problem's work items must stay inside its pool ranges
+        const int mk = pp.kind == 2 ? 0 : pp.kind;
+        const size_t es = mk == 0 ? 2 : 4;
+        ScanExtents ex{};
+        ex.xop = size_t(pp.n_pad) * pp.Kp * es;
+        ex.hx = size_t(pp.n_pad) * 4;
+        ex.qop = pp.kind == 2 ? 0 : size_t(pp.nq_pad) * pp.Kp * es;
+        ex.slots = size_t(pp.nsplit) * pp.nq_pad * 64 * B.V * 8;
+        ex.lcnts = size_t(pp.nsplit) * pp.nq_pad * 4;
+        ex.gthr = size_t(pp.nq_pad) * 4;
+        ex.q32 = size_t(pr.n) * pr.ld * 4;
+        int L = 0;
+        if (pp.kind == 2) {
+          const auto& part = *b->parts[i];
+          L = part.L;
+          ex.tile_list = part.tile_list.bytes;
+          ex.cents = part.cents.bytes;
+          ex.qtab = size_t(L) * pp.nq_pad * 4;
+        }
+        std::string why;
+        if (scan_check_launch(sa, mk, pp.ks, B.V, pp.nsplit, ex, L, &why))
+          return fail(FUEL_AMD_ERR_DEVICE, "internal error: batch scan plan outside its buffers: " + why);
+      }
       tab.push_back(sa);
     }
     e = hipMemcpyAsync(B.probs_dev.p, tab.data(), sizeof(ScanMfmaArgs) * tab.size(), hipMemcpyHostToDevice,

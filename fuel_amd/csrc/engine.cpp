@@ -297,10 +297,13 @@ int32_t knn_run(fuel_amd_index* ix, QueryCtx* ctx, const void* d_query, int64_t 
   if (m == 0 || k == 0) return FUEL_AMD_OK;
   if (!d_query || !d_out_idx || !d_out_dist) return fail(FUEL_AMD_ERR_INVALID, "NULL buffer");
   auto t0 = std::chrono::steady_clock::now();
+  int mode = ix->opts.mode;
+  if (mode == FUEL_AMD_MODE_AUTO) {
+    int32_t rc = resolve_auto_mode(ix, ctx, k, st, &mode);
+    if (rc) return rc;
+  }
   ctx->stats = fuel_amd_stats{};
   ctx->stats.queries = m;
-  int mode = ix->opts.mode;
-  if (mode == FUEL_AMD_MODE_AUTO) mode = FUEL_AMD_MODE_BF16;
   int kind = mode == FUEL_AMD_MODE_BF16 ? 0 : (mode == FUEL_AMD_MODE_FP32 ? 1 : (mode == FUEL_AMD_MODE_BF16R ? 2 : -1));
   if (kind == 2 && !mfma_supported(ix, 2, k, exclude)) kind = 0;  // no residual variant this wide
   if (kind >= 0 && !mfma_supported(ix, kind, k, exclude)) kind = -1;

@@ -12,7 +12,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "api_internal.h"
@@ -39,6 +41,23 @@ int kp_for(const fuel_amd_index* ix, int k, int exclude) {
 int resid_lists(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(256, n / 256)); }
 
 }  // namespace
+
+// Padded layout of the residual operand for lists of cnt[j] rows: list j
+// occupies the 64-row blocks [pad[j] / 64, pad[j + 1] / 64); the operand has
+// n_rpad rows (a multiple of 256, the value returned) and tl[b] is the list of
+// block b (the last list for the blocks past the end), so every entry of tl
+// lies in [0, L).
+int64_t resid_layout(const std::vector<uint32_t>& cnt, std::vector<int64_t>& pad, std::vector<int>& tl) {
+  const int L = (int)cnt.size();
+  pad.assign(L + 1, 0);
+  for (int j = 0; j < L; ++j) pad[j + 1] = pad[j] + ((int64_t(cnt[j]) + 63) & ~int64_t(63));
+  const int64_t n_rpad = std::max<int64_t>(256, (pad[L] + 255) & ~int64_t(255));
+  tl.assign(n_rpad / 64, 0);
+  for (int j = 0; j < L; ++j)
+    for (int64_t b = pad[j] / 64; b < pad[j + 1] / 64; ++b) tl[b] = j;
+  for (int64_t b = pad[L] / 64; b < n_rpad / 64; ++b) tl[b] = L - 1;
+  return n_rpad;
+}
 
 // Partition (Lloyd steps from evenly spaced rows) and the sorted padded layout
 // of the residual operand.
@@ -96,15 +115,9 @@ int32_t build_partition(const float* X, int64_t ld, int64_t n, int d, hipStream_
                             cents.as<float>(), st);
     if (e != hipSuccess) return hip_fail(e, "partition");
   }
-  // padded layout: every list starts on a 64-row block
-  std::vector<int64_t> hpad(L + 1);
-  hpad[0] = 0;
-  for (int j = 0; j < L; ++j) hpad[j + 1] = hpad[j] + ((int64_t(hcnt[j]) + 63) & ~int64_t(63));
-  const int64_t n_rpad = std::max<int64_t>(256, (hpad[L] + 255) & ~int64_t(255));
-  std::vector<int> htl(n_rpad / 64, 0);
-  for (int j = 0; j < L; ++j)
-    for (int64_t b = hpad[j] / 64; b < hpad[j + 1] / 64; ++b) htl[b] = j;
-  for (int64_t b = hpad[L] / 64; b < n_rpad / 64; ++b) htl[b] = L - 1;
+  std::vector<int64_t> hpad;
+  std::vector<int> htl;
+  const int64_t n_rpad = resid_layout(hcnt, hpad, htl);
   e = rperm.ensure(sizeof(uint32_t) * n_rpad);
   if (e == hipSuccess) e = tile_list.ensure(sizeof(int) * htl.size());
   if (e == hipSuccess) e = lcount.ensure(sizeof(uint32_t) * L);
@@ -254,6 +267,34 @@ int mfma_splits(int64_t nqb, int64_t n_pad, int cus, int64_t rows_tile, int wg_p
 
 }  // namespace
 
+MfmaChunkPlan plan_mfma_chunk(int mk, bool resid, int ks, int64_t m_total, int64_t remaining, int KP, int64_t n_pad,
+                              int cus) {
+  MfmaChunkPlan p{};
+  p.V = KP <= 64 ? 4 : 8;
+  p.NS = 64 * p.V;
+  p.csub = (p.NS - KP) / 2;
+  // small residual batches (serving): one-wave workgroups of 64 query slots
+  p.waves = (m_total <= 192 && mfma_one_wave_supported(mk, ks, p.V, resid)) ? 1 : 4;
+  p.qpb = mfma_queries_per_block(mk, ks, p.waves);
+  p.rows_tile = mfma_rows_per_tile(mk, ks, resid);
+  p.wg_per_cu = mfma_workgroups_per_cu(mk, ks, p.waves, resid);
+  p.chunk = remaining;
+  p.nqb = (p.chunk + p.qpb - 1) / p.qpb;
+  int nsplit = mfma_splits(p.nqb, n_pad, cus, p.rows_tile, p.wg_per_cu);
+  const size_t per_q = size_t(nsplit) * p.NS * 8;
+  const int64_t max_chunk = std::max<int64_t>(p.qpb, int64_t(MFMA_SLOT_BUDGET / per_q) / p.qpb * p.qpb);
+  if (p.chunk > max_chunk) {
+    p.chunk = max_chunk;
+    p.nqb = p.chunk / p.qpb;
+  }
+  p.nq_pad = p.nqb * p.qpb;
+  p.rps = (n_pad + nsplit - 1) / nsplit;
+  p.rps = (p.rps + p.rows_tile - 1) / p.rows_tile * p.rows_tile;
+  p.nsplit = (int)((n_pad + p.rps - 1) / p.rps);  // <= the count the slots are sized for
+  p.slots_bytes = per_q * size_t(p.nq_pad);
+  return p;
+}
+
 bool mfma_supported(const fuel_amd_index* ix, int kind, int k, int exclude) {
   if (ix->dtype != FUEL_AMD_F32) return false;
   const int ks = mfma_ksteps_bucket(kind == 2 ? 0 : kind, ix->d);
@@ -262,9 +303,80 @@ bool mfma_supported(const fuel_amd_index* ix, int kind, int k, int exclude) {
   return KK <= KP_MAX;
 }
 
+// Plain bf16 certifies when the neighbour gaps are wide next to ~2^-8 |q| |x|
+// (uniform data); on clustered data with large norms most queries fail it,
+// and the residual form certifies instead.  The probe runs the first level of
+// the plain bf16 form on PROBE_ROWS evenly spaced index rows as queries; if
+// more than a tenth of them fail and the residual form has a variant for this
+// width, AUTO means the residual form.  The probe only picks the route: every
+// mode returns the exact answer.
+int32_t resolve_auto_mode(fuel_amd_index* ix, QueryCtx* ctx, int k, hipStream_t st, int* mode) {
+  int m = ix->auto_mode.load(std::memory_order_acquire);
+  if (m >= 0) {
+    *mode = m;
+    return FUEL_AMD_OK;
+  }
+  std::lock_guard<std::mutex> lock(ix->auto_mu);
+  m = ix->auto_mode.load(std::memory_order_relaxed);
+  if (m >= 0) {
+    *mode = m;
+    return FUEL_AMD_OK;
+  }
+  constexpr int64_t PROBE_ROWS = 256;
+  constexpr int64_t PROBE_MIN_N = 4096;
+  m = FUEL_AMD_MODE_BF16;
+  if (ix->dtype != FUEL_AMD_F32 || mfma_ksteps_bucket(0, ix->d) < 0) {
+    m = FUEL_AMD_MODE_EXACT;
+  } else if (ix->n >= PROBE_MIN_N && k >= 1 && mfma_supported(ix, 0, k, EXCL_NONE) &&
+             mfma_supported(ix, 2, k, EXCL_NONE)) {
+    const int64_t S = PROBE_ROWS;
+    DevBuf q, oi, od;
+    hipError_t e = q.ensure(sizeof(float) * size_t(S) * ix->ld);
+    if (e == hipSuccess) e = oi.ensure(sizeof(int64_t) * size_t(S) * k);
+    if (e == hipSuccess) e = od.ensure(sizeof(float) * size_t(S) * k);
+    // rows i * n / S, gathered with one strided copy when n is a multiple of S,
+    // else row by row (256 small copies, once per index)
+    const float* X = ix->X.as<float>();
+    if (e == hipSuccess && ix->n % S == 0)
+      e = hipMemcpy2DAsync(q.p, sizeof(float) * ix->ld, X, sizeof(float) * ix->ld * (ix->n / S),
+                           sizeof(float) * ix->ld, S, hipMemcpyDeviceToDevice, st);
+    for (int64_t i = 0; e == hipSuccess && ix->n % S != 0 && i < S; ++i)
+      e = hipMemcpyAsync(q.as<float>() + i * ix->ld, X + (i * ix->n / S) * ix->ld, sizeof(float) * ix->ld,
+                         hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "mode probe");
+    const fuel_amd_stats saved = ctx->stats;
+    ctx->stats = fuel_amd_stats{};
+    int32_t rc = knn_mfma(ix, ctx, q.as<float>(), S, ix->ld, k, EXCL_NONE, 0, oi.as<int64_t>(), od.as<float>(), st,
+                          nullptr, 0, 0, 0, nullptr, true);
+    const int64_t nfail = ctx->stats.fallback_queries + ctx->stats.exact_queries;
+    ctx->stats = saved;
+    if (e == hipSuccess) e = hipStreamSynchronize(st);  // before the probe buffers are released
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "mode probe");
+    ix->auto_probe_rows = S;
+    ix->auto_probe_fail = nfail;
+    if (nfail * 10 > S) {
+      m = FUEL_AMD_MODE_BF16R;
+      // the residual chain continues with f32, never with the plain bf16 copy
+      std::lock_guard<std::mutex> plock(ix->prep_mu);
+      auto& op = ix->mop[0];
+      op.ready.store(false, std::memory_order_release);
+      op.Xop.release();
+      op.hx.release();
+      op.consts.release();
+    }
+  }
+  ix->auto_mode.store(m, std::memory_order_release);
+  *mode = m;
+  return FUEL_AMD_OK;
+}
+
 int32_t prepare_mode(fuel_amd_index* ix, hipStream_t st) {
   int mode = ix->opts.mode;
-  if (mode == FUEL_AMD_MODE_AUTO) mode = FUEL_AMD_MODE_BF16;
+  if (mode == FUEL_AMD_MODE_AUTO) {
+    int32_t rc = resolve_auto_mode(ix, &ix->ctx, 10, st, &mode);
+    if (rc) return rc;
+  }
   if (mode == FUEL_AMD_MODE_EXACT || ix->dtype != FUEL_AMD_F32) return FUEL_AMD_OK;
   int kind = mode == FUEL_AMD_MODE_BF16 ? 0 : (mode == FUEL_AMD_MODE_BF16R ? 2 : 1);
   if (kind == 2 && !mfma_supported(ix, 2, 1, 0)) kind = 0;  // as knn_run
@@ -277,7 +389,7 @@ int32_t prepare_mode(fuel_amd_index* ix, hipStream_t st) {
 
 int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, int64_t ldq, int k, int exclude,
                  int kind, int64_t* out_idx, float* out_dist, hipStream_t st, const int64_t* d_rows, int64_t qbase,
-                 int level, int kp, const float* d_theta) {
+                 int level, int kp, const float* d_theta, bool probe) {
   int32_t rc = ensure_prepared(ix, kind, st);
   if (rc) return rc;
   auto& op = ix->mop[kind];
@@ -289,18 +401,10 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
   auto& ws = ctx->lv[level];
   const int KK = k + (exclude != EXCL_NONE ? 1 : 0);
   const int KP = kp > 0 ? std::max(kp, k + (exclude != EXCL_NONE ? 1 : 0)) : kp_for(ix, k, exclude);
-  const int V = KP <= 64 ? 4 : 8;
-  const int NS = 64 * V;
-  const int csub = (NS - KP) / 2;
   const int ks = op.ks;
   const int Kp = op.kp;
   const bool resid = kind == 2;
   const int mk = resid ? 0 : kind;  // MFMA operand type
-  // small residual batches (serving): one-wave workgroups of 64 query slots
-  const int waves = (m <= 192 && mfma_one_wave_supported(mk, ks, V, resid)) ? 1 : 4;
-  const int qpb = mfma_queries_per_block(mk, ks, waves);
-  const int rows_tile = mfma_rows_per_tile(mk, ks, resid);
-  const int wg_per_cu = mfma_workgroups_per_cu(mk, ks, waves, resid);
   const int cus = device_cu_count(ix->device);
   const size_t es = mk == 0 ? 2 : 4;
   const int64_t n_pad = op.n_pad;
@@ -310,26 +414,16 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
   int64_t done = 0;
   while (done < m) {
     if (cancelled()) return fail(FUEL_AMD_ERR_INTERRUPTED, "interrupted");
-    int64_t chunk = m - done;
-    int64_t nqb = (chunk + qpb - 1) / qpb;
-    int nsplit = mfma_splits(nqb, n_pad, cus, rows_tile, wg_per_cu);
-    const size_t per_q = size_t(nsplit) * NS * 8;
-    const int64_t max_chunk = std::max<int64_t>(qpb, int64_t(MFMA_SLOT_BUDGET / per_q) / qpb * qpb);
-    if (chunk > max_chunk) {
-      chunk = max_chunk;
-      nqb = chunk / qpb;
-    }
-    const int64_t nq_pad = nqb * qpb;
-    int64_t rps = (n_pad + nsplit - 1) / nsplit;
-    rps = (rps + rows_tile - 1) / rows_tile * rows_tile;
-    nsplit = (int)((n_pad + rps - 1) / rps);
+    const MfmaChunkPlan pl = plan_mfma_chunk(mk, resid, ks, m, m - done, KP, n_pad, cus);
+    const int V = pl.V, NS = pl.NS, csub = pl.csub, waves = pl.waves, nsplit = pl.nsplit;
+    const int64_t chunk = pl.chunk, nqb = pl.nqb, nq_pad = pl.nq_pad, rps = pl.rps;
 
     hipError_t e = resid ? ws.qtab.ensure(sizeof(float) * size_t(op.L) * nq_pad) : ws.Qop.ensure(size_t(nq_pad) * Kp * es);
     if (resid && e == hipSuccess) e = ws.qs2.ensure(sizeof(float) * size_t(op.L) * chunk);
     if (e == hipSuccess) e = ws.Q32.ensure(size_t(chunk) * ld32 * 4);
     if (e == hipSuccess) e = ws.qnorm.ensure(sizeof(double) * 3 * size_t(chunk));
     if (e == hipSuccess) e = ws.qint.ensure(sizeof(int) * size_t(chunk));
-    if (e == hipSuccess) e = ws.slots.ensure(per_q * nq_pad);
+    if (e == hipSuccess) e = ws.slots.ensure(pl.slots_bytes);
     if (e == hipSuccess) e = ws.lcnts.ensure(sizeof(int) * size_t(nsplit) * nq_pad);
     if (e == hipSuccess) e = ws.fail_count.ensure(64);
     if (e == hipSuccess) e = ws.fail_list.ensure(sizeof(int64_t) * size_t(chunk));
@@ -415,6 +509,23 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
       sa.d = ix->d;
       sa.qtab = ws.qtab.as<float>();
     }
+    {
+      // refuse a launch whose addresses would leave the buffers it was given
+      ScanExtents ex{};
+      ex.xop = op.Xop.bytes;
+      ex.hx = op.hx.bytes;
+      ex.qop = resid ? 0 : ws.Qop.bytes;
+      ex.slots = ws.slots.bytes;
+      ex.lcnts = ws.lcnts.bytes;
+      ex.gthr = ws.gthr.bytes;
+      ex.tile_list = resid ? op.tile_list.bytes : 0;
+      ex.cents = resid ? op.cents.bytes : 0;
+      ex.q32 = ws.Q32.bytes;
+      ex.qtab = resid ? ws.qtab.bytes : 0;
+      std::string why;
+      if (scan_check_launch(sa, mk, ks, V, nsplit, ex, resid ? op.L : 0, &why))
+        return fail(FUEL_AMD_ERR_DEVICE, "internal error: MFMA scan plan outside its buffers: " + why);
+    }
     if (timed)
       timing_mark(ix, st, true,
                   kind == 0 ? "scan_mfma_kernel<bf16>" : (kind == 1 ? "scan_mfma_kernel<f32>" : "scan_mfma_kernel<bf16r>"));
@@ -463,7 +574,7 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
     e = hipMemcpyAsync(&nfail, ws.fail_count.p, sizeof(int), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return hip_fail(e, "certification count");
-    if (nfail > 0) {
+    if (nfail > 0 && !probe) {
       // the fail list holds query rows relative to Qb and to this chunk's outputs
       int next_kind = -1;
       if (level == 0)
@@ -493,4 +604,129 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
   return FUEL_AMD_OK;
 }
 
+// Every launch plan plan_mfma_chunk makes for a matrix of shapes and sizes,
+// checked by scan_check_launch against the buffers knn_mfma allocates for it
+// (host only: nothing is allocated or launched).  perturb > 0 plants one
+// mistake in every plan (a buffer one element short of what the kernel reads,
+// one split too many, an append segment one entry too long, ...), so that the
+// check can be seen to catch it.  Returns the number of plans flagged.
+int64_t check_scan_plans(int perturb, int64_t* launches, std::string* first) {
+  int64_t bad = 0, count = 0;
+  const int KPs[] = {10, 11, 16, 20, 40, 64, 65, 128};
+  const int64_t ms[] = {1, 63, 64, 65, 192, 193, 257, 4097, 100000};
+  const int64_t ns[] = {300, 65536 + 13, 1000000, 10000000};
+  const int cus_list[] = {256, 1};
+  for (int variant = 0; variant < 3; ++variant) {
+    const int mk = variant == 1 ? 1 : 0;
+    const bool resid = variant == 2;
+    for (int dd = 1; dd <= (mk == 0 ? 1024 : 256) + 1; ++dd) {
+      // the widest and the narrowest dimension of every K step bucket
+      const int ks = mfma_ksteps_bucket(mk, dd);
+      if (ks < 0 || (resid && ks > mfma_resid_max_ksteps())) continue;
+      if (mfma_ksteps_bucket(mk, dd - 1) == ks && mfma_ksteps_bucket(mk, dd + 1) == ks) continue;
+      const int d = dd;
+      const int Kp = ks * (mk == 0 ? 16 : 8);
+      const size_t es = mk == 0 ? 2 : 4;
+      const int64_t ld32 = (d + 7) & ~int64_t(7);
+      for (int64_t n : ns) {
+        int L = 0;
+        int64_t n_pad = std::max<int64_t>(256, (n + 255) & ~int64_t(255));
+        size_t tile_list_bytes = 0;
+        if (resid) {
+          L = (int)std::max<int64_t>(1, std::min<int64_t>(256, n / 256));
+          std::vector<uint32_t> cnt(L, uint32_t(n / L));
+          for (int64_t j = 0; j < n % L; ++j) cnt[j] += 1;
+          std::vector<int64_t> pad;
+          std::vector<int> tl;
+          n_pad = resid_layout(cnt, pad, tl);
+          tile_list_bytes = tl.size() * sizeof(int);
+        }
+        for (int KP : KPs)
+          for (int64_t m : ms)
+            for (int cus : cus_list) {
+              if (m * n > int64_t(1) << 40 && cus == 1) continue;
+              int64_t done = 0;
+              while (done < m) {
+                const MfmaChunkPlan pl = plan_mfma_chunk(mk, resid, ks, m, m - done, KP, n_pad, cus);
+                ScanMfmaArgs sa{};
+                sa.Kp = Kp;
+                sa.n_pad = n_pad;
+                sa.nq_pad = pl.nq_pad;
+                sa.nqb = (int)pl.nqb;
+                sa.rows_per_split = pl.rps;
+                sa.KP = KP;
+                sa.Csub = pl.csub;
+                sa.waves = pl.waves;
+                ScanExtents ex{};
+                ex.xop = size_t(n_pad) * Kp * es;
+                ex.hx = size_t(n_pad) * 4;
+                ex.qop = resid ? 0 : size_t(pl.nq_pad) * Kp * es;
+                ex.slots = pl.slots_bytes;
+                ex.lcnts = size_t(pl.nsplit) * pl.nq_pad * 4;
+                ex.gthr = size_t(pl.nq_pad) * 4;
+                ex.q32 = size_t(pl.chunk) * ld32 * 4;
+                if (resid) {
+                  sa.resid = 1;
+                  sa.ldc = d;
+                  sa.ld32 = ld32;
+                  sa.m = pl.chunk;
+                  sa.d = d;
+                  ex.tile_list = tile_list_bytes;
+                  ex.cents = size_t(L) * d * 4;
+                  ex.qtab = size_t(L) * pl.nq_pad * 4;
+                }
+                int nsplit = pl.nsplit;
+                const size_t need_slots = size_t(pl.nsplit) * pl.nq_pad * pl.NS * 8;
+                switch (perturb) {
+                  case 1: ex.slots = need_slots - 8; break;
+                  case 2: ex.lcnts -= 4; break;
+                  case 3: ex.gthr -= 4; break;
+                  case 4: nsplit += 1; break;
+                  case 5: sa.Csub += 1; break;
+                  case 6: ex.xop -= 16; break;
+                  case 7: sa.rows_per_split += 1; break;
+                  case 8: sa.waves = sa.waves == 1 ? 4 : 1; break;
+                  case 9: ex.hx -= 4; break;
+                  case 10:
+                    if (resid) ex.qtab -= 4; else ex.qop -= 16;
+                    break;
+                  case 11:
+                    if (resid) ex.q32 = size_t(pl.chunk - 1) * ld32 * 4 + size_t(d) * 4 - 4; else ex.gthr -= 4;
+                    break;
+                  case 12:
+                    if (resid) ex.cents -= 4; else ex.lcnts -= 4;
+                    break;
+                  default: break;
+                }
+                std::string why;
+                const int64_t v = scan_check_launch(sa, mk, ks, pl.V, nsplit, ex, L, &why) ? 1 : 0;
+                if (v && bad == 0 && first) {
+                  char buf[160];
+                  std::snprintf(buf, sizeof(buf), "d %d, n %lld, m %lld (chunk at %lld), KP %d, %d CUs: ", d,
+                                (long long)n, (long long)m, (long long)done, KP, cus);
+                  *first = buf + why;
+                }
+                bad += v;
+                ++count;
+                done += pl.chunk;
+              }
+            }
+      }
+    }
+  }
+  if (launches) *launches = count;
+  return bad;
+}
+
 }  // namespace fa
+
+// Test hook (not part of the ABI in include/fuel_amd.h): runs check_scan_plans
+// and returns its violation count; `text` receives the first violation.
+extern "C" int64_t fuel_amd_internal_check_scan_plans(int32_t perturb, int64_t* launches, char* text, int32_t len) {
+  std::string first;
+  const int64_t bad = fa::check_scan_plans(perturb, launches, &first);
+  if (text && len > 0) {
+    std::snprintf(text, size_t(len), "%s", first.c_str());
+  }
+  return bad;
+}

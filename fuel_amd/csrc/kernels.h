@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "common.h"
 
 namespace fa {
@@ -405,6 +407,21 @@ int mfma_queries_per_block(int kind, int ksteps, int waves = 4);
 int mfma_rows_per_tile(int kind, int ksteps, bool resid);
 int mfma_workgroups_per_cu(int kind, int ksteps, int waves, bool resid);
 bool mfma_one_wave_supported(int kind, int ksteps, int V, bool resid);
+
+// Bytes addressable from each pointer of a ScanMfmaArgs.
+struct ScanExtents {
+  size_t xop, hx, qop, slots, lcnts, gthr, tile_list, cents, q32, qtab;
+};
+// Host-side check of one scan launch with the arguments launch_scan_mfma (or
+// one problem of the persistent form) receives: every global address the
+// kernel derives from `a` stays inside `ex`, every tile copy lands inside its
+// LDS buffer and stages the row chunk the matrix-core reads of that LDS
+// position expect, and a tile's appends fit the slot segments.  The residual
+// operand's block lists are taken to lie in [0, L) (checked when the
+// partition is built).  Returns the number of violations; *first (if not
+// null) receives a description of the first.
+int64_t scan_check_launch(const ScanMfmaArgs& a, int kind, int ksteps, int V, int nsplit, const ScanExtents& ex,
+                          int L, std::string* first);
 
 template <typename F>
 hipError_t launch_scan_exact(const ScanExactArgs<F>& a, int qw, int nsplit, hipStream_t st);

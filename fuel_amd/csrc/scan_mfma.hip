@@ -35,6 +35,11 @@
 // query through a per-query atomicMin; every tile's copy brings the values the
 // other splits published.
 #include <algorithm>
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
 
 #include "common.h"
 #include "kernels.h"
@@ -171,7 +176,7 @@ __device__ __forceinline__ int merge_select(uint64_t* __restrict__ slot, int KP,
       const int c0_ = __builtin_amdgcn_readlane(cnt[g], j_);                              \
       const int c1_ = __builtin_amdgcn_readlane(cnt[g], j_ + 32);                         \
       const int lc_ = __builtin_amdgcn_readlane(lcnt[g], j_);                             \
-      uint64_t* sl_ = a.slots + (size_t(split) * a.nq_pad + q0 + (g) * 32 + j_) * NS;     \
+      uint64_t* sl_ = a.slots + scan_slot_index(split, a.nq_pad, q0 + (g) * 32 + j_, NS, 0); \
       uint32_t kt_;                                                                       \
       const int nl_ = merge_select<V>(sl_, KP, lc_, csub, c0_, c1_, kt_);                 \
       const float nthr_ = nl_ == KP ? unord_f32(kt_) : MFMA_KEY_LIMIT;                    \
@@ -185,6 +190,51 @@ __device__ __forceinline__ int merge_select(uint64_t* __restrict__ slot, int KP,
     }                                                                                     \
   } while (0)
 
+// Tile geometry of a shape (KS K steps, R 32-row blocks per tile, W waves):
+// each row takes 2 KS 16-byte chunks plus one pad chunk (bank spread); a tile
+// is copied by whole 64-lane loads of every wave, so its LDS size is rounded
+// up to them.  The scan body and the launch planning both use these.
+constexpr int scan_wave_loads(int ks, int R, int W) { return (32 * R * (2 * ks + 1) + 64 * W - 1) / (64 * W); }
+constexpr int scan_tile_bytes(int ks, int R, int W) { return scan_wave_loads(ks, R, W) * 64 * W * 16; }
+// LDS of one workgroup: two tile buffers, their C inputs (64 floats each) and
+// the published thresholds (64 words per wave and buffer), plus the persistent
+// form's work item.
+constexpr int scan_lds_bytes(int ks, int R, int W, bool batch) {
+  return 2 * scan_tile_bytes(ks, R, W) + 2 * 64 * 4 + 2 * W * 64 * 4 + (batch ? 16 : 0);
+}
+constexpr int kLdsPerCu = 160 * 1024;  // gfx950
+constexpr int kLdsGranule = 512;       // allocation granularity per workgroup
+
+// Tile staging: lane i of copy j of wave w fills LDS chunk p = (j W + w) 64 + i
+// = row (2 KS + 1) + c of the tile; the pad chunk (c == 2 KS) and the round-up
+// tail re-read a valid chunk.  Returns the lane's source byte offset from the
+// tile's first operand row (rows of KS * 32 bytes).
+__host__ __device__ __forceinline__ uint32_t scan_stage_src(int ks, int R, int W, int j, int wave, int lane) {
+  const int CH = 2 * ks, ROWC = CH + 1, BN = 32 * R;
+  const int p = (j * W + wave) * 64 + lane;
+  int row = p / ROWC, c = p - row * ROWC;
+  if (c >= CH) c = CH - 1;
+  if (row >= BN) {
+    row = BN - 1;
+    c = CH - 1;
+  }
+  return (uint32_t)(row * (ks * 32) + c * 16);
+}
+// First query slot of a wave, and the row range [r0, r1) of a split.
+__host__ __device__ __forceinline__ int64_t scan_wave_q0(int qb, int wave, int A, int W) {
+  return int64_t(qb) * (W * 32 * A) + wave * (32 * A);
+}
+__host__ __device__ __forceinline__ void scan_split_rows(int split, int64_t rps, int64_t n_pad, int64_t& r0,
+                                                         int64_t& r1) {
+  r0 = int64_t(split) * rps;
+  r1 = r0 + rps;
+  if (r1 > n_pad) r1 = n_pad;
+}
+// Element index of entry e of query slot q's list in split `split`.
+__host__ __device__ __forceinline__ size_t scan_slot_index(int split, int64_t nq_pad, int64_t q, int NS, int e) {
+  return (size_t(split) * nq_pad + q) * NS + e;
+}
+
 template <int KIND, int KS, int A, int R, int V, bool RESID = false, int W = 4>
 __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int qb, const int split) {
   using C = Comp<float>;
@@ -195,9 +245,9 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
   constexpr int CH = KS * 2;                  // 16-byte chunks of a row
   constexpr int ROWC = CH + 1;                // + one pad chunk (bank spread)
   constexpr int ROWB = ROWC * 16;             // LDS row stride in bytes
-  constexpr int TILE_CHUNKS = BN * ROWC;
-  constexpr int WAVE_LOADS = (TILE_CHUNKS + 64 * W - 1) / (64 * W);  // 64-chunk loads per wave per tile
-  constexpr int TILE_BYTES = WAVE_LOADS * 64 * W * 16;                // rounded up to whole wave loads
+  constexpr int WAVE_LOADS = scan_wave_loads(KS, R, W);  // 64-chunk loads per wave per tile
+  constexpr int TILE_BYTES = scan_tile_bytes(KS, R, W);  // rounded up to whole wave loads
+  static_assert(WAVE_LOADS * 64 * W >= BN * ROWC, "tile copy covers the tile");
   __shared__ __attribute__((aligned(16))) char tile[2][TILE_BYTES];
   __shared__ __attribute__((aligned(16))) float tile_hx[2][64];      // one dword per lane of wave 0
   __shared__ __attribute__((aligned(16))) uint32_t thr_lds[2][W][64];  // published thresholds, per wave
@@ -207,10 +257,9 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
   const int half = lane >> 5;
   const int col = lane & 31;
   const int wave = tid >> 6;
-  const int64_t q0 = int64_t(qb) * QPB + wave * QPW;
-  const int64_t r0 = int64_t(split) * a.rows_per_split;
-  int64_t r1 = r0 + a.rows_per_split;
-  if (r1 > a.n_pad) r1 = a.n_pad;
+  const int64_t q0 = scan_wave_q0(qb, wave, A, W);
+  int64_t r0, r1;
+  scan_split_rows(split, a.rows_per_split, a.n_pad, r0, r1);
   const int KP = a.KP;
   const int csub = a.Csub;
   const size_t kp_bytes = size_t(KS) * 32;
@@ -241,21 +290,11 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
     lcnt[g] = 0;
   }
 
-  // Tile staging: lane i of DMA j of wave w fills LDS chunk p = (j*W + w)*64 + i
-  // = row*ROWC + c; the pad chunk (c == CH) and the round-up tail re-read a
-  // valid chunk.  The per-lane source offsets do not depend on the tile.
+  // Tile staging (scan_stage_src): the per-lane source offsets do not depend on
+  // the tile.
   uint32_t soff[WAVE_LOADS];
 #pragma unroll
-  for (int j = 0; j < WAVE_LOADS; ++j) {
-    const int p = (j * W + wave) * 64 + lane;
-    int row = p / ROWC, c = p - row * ROWC;
-    if (c >= CH) c = CH - 1;
-    if (row >= BN) {
-      row = BN - 1;
-      c = CH - 1;
-    }
-    soff[j] = (uint32_t)(row * (KS * 32) + c * 16);
-  }
+  for (int j = 0; j < WAVE_LOADS; ++j) soff[j] = scan_stage_src(KS, R, W, j, wave, lane);
   const uint32_t lds_tile = (uint32_t)(uintptr_t)&tile[0][0];
   const uint32_t lds_hx = (uint32_t)(uintptr_t)&tile_hx[0][0];
   const uint32_t lds_thr = (uint32_t)(uintptr_t)&thr_lds[0][0][0];
@@ -366,7 +405,7 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
         for (int r = 1; r < 16; ++r) mx = fmaxf(mx, acc[g][rb][r]);
         if (__ballot(mx > nt)) {
           appended = true;
-          uint64_t* sl = a.slots + (size_t(split) * a.nq_pad + q0 + g * 32 + col) * NS + KP + half * csub;
+          uint64_t* sl = a.slots + scan_slot_index(split, a.nq_pad, q0 + g * 32 + col, NS, KP + half * csub);
           float w[16];
 #pragma unroll
           for (int r = 0; r < 16; ++r)
@@ -518,9 +557,15 @@ int mfma_ksteps_bucket(int kind, int d) {
 
 int mfma_queries_per_block(int kind, int ksteps, int waves) { return waves * 32 * scan_groups(ksteps); }
 int mfma_rows_per_tile(int kind, int ksteps, bool resid) { return 32 * scan_row_blocks(kind, ksteps, resid); }
+// Resident workgroups per CU: the waves per SIMD the launch bounds allow (one
+// wave of a one-wave workgroup per SIMD and bound), capped by what the
+// workgroup's LDS leaves room for.
 int mfma_workgroups_per_cu(int kind, int ksteps, int waves, bool resid) {
   const int per = scan_four_wg(kind, ksteps) ? 4 : (scan_two_waves(kind, ksteps, resid) ? 2 : 1);
-  return waves == 1 ? 4 * std::min(per, 2) : per;
+  const int by_waves = waves == 1 ? 4 * std::min(per, 2) : per;
+  const int lds = scan_lds_bytes(ksteps, scan_row_blocks(kind, ksteps, resid), waves == 1 ? 1 : 4, false);
+  const int by_lds = kLdsPerCu / ((lds + kLdsGranule - 1) / kLdsGranule * kLdsGranule);
+  return std::max(1, std::min(by_waves, by_lds));
 }
 bool mfma_one_wave_supported(int kind, int ksteps, int V, bool resid) {
   return kind == 0 && resid && V == 4 && ksteps <= 12;
@@ -627,6 +672,146 @@ hipError_t launch_scan_mfma_batch(const ScanBatchArgs& b, int kind, int ks, int 
 }
 
 int mfma_resid_max_ksteps() { return 40; }
+
+namespace {
+
+bool in_bucket(int kind, int ks) {
+  const int* tab = kind == 0 ? kBf16Steps : kF32Steps;
+  const int nt = kind == 0 ? (int)(sizeof(kBf16Steps) / sizeof(int)) : (int)(sizeof(kF32Steps) / sizeof(int));
+  for (int i = 0; i < nt; ++i)
+    if (tab[i] == ks) return true;
+  return false;
+}
+
+struct Violations {
+  int64_t count = 0;
+  std::string first;
+  void add(const char* what, long long v, long long lim) {
+    if (count++ == 0) {
+      char buf[200];
+      std::snprintf(buf, sizeof(buf), "%s: %lld against a limit of %lld", what, v, lim);
+      first = buf;
+    }
+  }
+};
+
+// The launch-independent part of the check: LDS staging of one shape.
+void check_shape(int kind, int ks, int W, bool resid, Violations& v) {
+  const int R = scan_row_blocks(kind, ks, resid);
+  const int BN = 32 * R, CH = 2 * ks, ROWB = (CH + 1) * 16;
+  const int WL = scan_wave_loads(ks, R, W), TB = scan_tile_bytes(ks, R, W);
+  const uint32_t kp_bytes = uint32_t(ks) * 32;
+  std::vector<uint32_t> src(size_t(WL) * 64 * W, 0xFFFFFFFFu);
+  for (int j = 0; j < WL; ++j)
+    for (int w = 0; w < W; ++w)
+      for (int lane = 0; lane < 64; ++lane) {
+        const long long dst = (long long)(j * W + w) * 1024 + lane * 16;  // inside one tile buffer
+        if (dst + 16 > TB) v.add("LDS tile copy destination", dst + 16, TB);
+        const uint32_t so = scan_stage_src(ks, R, W, j, w, lane);
+        if (so + 16 > uint32_t(BN) * kp_bytes) v.add("tile copy source", so + 16, (long long)BN * kp_bytes);
+        src[size_t(j * W + w) * 64 + lane] = so;
+      }
+  // every 16-byte fragment the matrix cores read holds (row, K step, lane half)
+  for (int row = 0; row < BN; ++row)
+    for (int s = 0; s < ks; ++s)
+      for (int h = 0; h < 2; ++h) {
+        const int byte = row * ROWB + s * 32 + h * 16;
+        if (byte + 16 > TB || size_t(byte / 16) >= src.size()) {
+          v.add("LDS fragment read", byte + 16, TB);
+        } else if (src[byte / 16] != uint32_t(row) * kp_bytes + uint32_t(s * 32 + h * 16)) {
+          v.add("staged chunk differs from the chunk read", src[byte / 16], (long long)row * kp_bytes + s * 32 + h * 16);
+        }
+      }
+  // C inputs: lane r < BN of wave 0 copies row t + r; reads of float4 at
+  // rb * 32 + q4 * 8 + half * 4 stay inside the 64 floats of a buffer
+  if ((R - 1) * 32 + 3 * 8 + 4 + 4 > 64) v.add("C-input read", (R - 1) * 32 + 32, 64);
+  // the thresholds ride in 64 words per wave and buffer; a wave reads g * 32 + col
+  if (32 * scan_groups(ks) > 64) v.add("threshold read", 32 * scan_groups(ks), 64);
+}
+
+}  // namespace
+
+int64_t scan_check_launch(const ScanMfmaArgs& a, int kind, int ks, int V, int nsplit, const ScanExtents& ex, int L,
+                          std::string* first) {
+  Violations v;
+  const bool resid = a.resid != 0;
+  const int W = a.waves == 1 ? 1 : 4;
+  if (kind != 0 && kind != 1) v.add("operand kind", kind, 1);
+  if (!in_bucket(kind, ks) || (resid && (kind != 0 || ks > mfma_resid_max_ksteps()))) v.add("K steps", ks, 0);
+  if (V != 4 && V != 8) v.add("slot width V", V, 8);
+  if (W == 1 && !mfma_one_wave_supported(kind, ks, V, resid)) v.add("one-wave shape", ks, 12);
+  if (v.count == 0) {
+    // the shape part does not depend on the launch: once per shape
+    static std::mutex mu;
+    static std::map<int, std::pair<int64_t, std::string>> done;
+    const int key = ((kind * 2 + (resid ? 1 : 0)) * 2 + (W == 1 ? 1 : 0)) * 128 + ks;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = done.find(key);
+    if (it == done.end()) {
+      Violations sv;
+      check_shape(kind, ks, W, resid, sv);
+      it = done.emplace(key, std::make_pair(sv.count, sv.first)).first;
+    }
+    if (it->second.first) {
+      if (v.count == 0) v.first = it->second.second;
+      v.count += it->second.first;
+    }
+  }
+  if (v.count == 0) {
+    const int A = scan_groups(ks), R = scan_row_blocks(kind, ks, resid);
+    const int NS = 64 * V, BN = 32 * R, QPB = W * 32 * A;
+    const size_t kp_bytes = size_t(ks) * 32;
+    const size_t es = kind == 0 ? 2 : 4;
+    if (size_t(a.Kp) * es != kp_bytes) v.add("operand row bytes", (long long)a.Kp * es, (long long)kp_bytes);
+    if (a.KP < 1 || 2 * a.KP > NS) v.add("kept candidates KP", a.KP, NS / 2);
+    // a merge leaves cnt <= Csub - 16 R before a tile, whose lanes then append at most 16 R
+    if (a.Csub < 16 * R) v.add("append segment shorter than one tile's appends", a.Csub, 16 * R);
+    if (a.KP + 2 * a.Csub > NS) v.add("kept list + append segments", a.KP + 2 * a.Csub, NS);
+    if (a.rows_per_split <= 0 || a.rows_per_split % BN) v.add("rows per split, whole tiles", a.rows_per_split, BN);
+    if (a.n_pad <= 0 || a.n_pad % BN) v.add("operand rows, whole tiles", a.n_pad, BN);
+    if (a.nqb < 1 || a.nq_pad != int64_t(a.nqb) * QPB) v.add("query slots", a.nq_pad, (long long)a.nqb * QPB);
+    if (nsplit < 1) v.add("splits", nsplit, 1);
+    // per-query accesses grow with the slot: the last slot of the last split bounds them
+    const int64_t qlast = int64_t(a.nqb) * QPB - 1;
+    if (!resid && size_t(qlast + 1) * kp_bytes > ex.qop)
+      v.add("query operand", (long long)((qlast + 1) * kp_bytes), (long long)ex.qop);
+    if (size_t(qlast + 1) * 4 > ex.gthr) v.add("shared thresholds", (long long)(qlast + 1) * 4, (long long)ex.gthr);
+    if (size_t(nsplit) * a.nq_pad * 4 > ex.lcnts)
+      v.add("list counts", (long long)(size_t(nsplit) * a.nq_pad * 4), (long long)ex.lcnts);
+    const size_t slot_end = scan_slot_index(nsplit - 1, a.nq_pad, qlast, NS, NS) * 8;
+    if (slot_end > ex.slots) v.add("candidate slots", (long long)slot_end, (long long)ex.slots);
+    // rows: the last tile of every split (every tile reads BN operand rows and C inputs)
+    for (int s = 0; s < nsplit && v.count == 0; ++s) {
+      int64_t r0, r1;
+      scan_split_rows(s, a.rows_per_split, a.n_pad, r0, r1);
+      if (r0 >= r1) continue;
+      const int64_t tl = r0 + (r1 - 1 - r0) / BN * BN;
+      if (size_t(tl + BN) * kp_bytes > ex.xop)
+        v.add("operand rows", (long long)(size_t(tl + BN) * kp_bytes), (long long)ex.xop);
+      if (size_t(tl + BN) * 4 > ex.hx) v.add("C inputs", (long long)(size_t(tl + BN) * 4), (long long)ex.hx);
+      if (resid && size_t((tl >> 6) + 1) * 4 > ex.tile_list)
+        v.add("block lists", (long long)((tl >> 6) + 1) * 4, (long long)ex.tile_list);
+    }
+    if (resid) {
+      if (L < 1) v.add("lists", L, 1);
+      if (size_t(L) * a.nq_pad * 4 > ex.qtab)
+        v.add("per-list query terms", (long long)(size_t(L) * a.nq_pad * 4), (long long)ex.qtab);
+      if (size_t(L - 1) * a.ldc * 4 + size_t(a.d) * 4 > ex.cents)
+        v.add("centroids", (long long)(size_t(L - 1) * a.ldc * 4 + size_t(a.d) * 4), (long long)ex.cents);
+      // padding slots repeat query m - 1
+      if (a.m < 1 || size_t(a.m - 1) * a.ld32 * 4 + size_t(a.d) * 4 > ex.q32)
+        v.add("f32 queries", (long long)(size_t(a.m - 1) * a.ld32 * 4 + size_t(a.d) * 4), (long long)ex.q32);
+      if (a.d > 16 * ks || (a.ld32 & 7)) v.add("f32 query pitch", a.ld32, 8);
+    }
+  }
+  if (v.count && first) {
+    char buf[160];
+    std::snprintf(buf, sizeof(buf), " (kind %d, %d K steps, V %d, %d waves, %d splits, %s)", kind, ks, V, W, nsplit,
+                  resid ? "residual" : "plain");
+    *first = v.first + buf;
+  }
+  return v.count;
+}
 
 hipError_t launch_scan_mfma(const ScanMfmaArgs& a, int kind, int ks, int V, int nsplit, hipStream_t st) {
   if (V != 4 && V != 8) return hipErrorInvalidValue;

@@ -96,8 +96,12 @@ struct fuel_amd_server {
   int64_t reserved = 0;  // estimated bytes of the in-flight batches
   int inflight = 0;
   bool stop = false;
-  // statistics since the last reset
+  // statistics since the last reset: p50/p99 over the latencies of the last
+  // LAT_WINDOW requests (a ring), max and mean over all of them
+  static constexpr size_t LAT_WINDOW = size_t(1) << 20;
   std::vector<double> lat_ms;
+  size_t lat_next = 0;  // ring position once the window is full
+  double lat_max = 0.0, lat_sum = 0.0;
   int64_t n_req = 0, n_q = 0, n_batches = 0, max_bq = 0, n_failed = 0;
   double busy_s = 0.0;
   fa::Clock::time_point busy_since;
@@ -162,7 +166,15 @@ void finish(fuel_amd_server* s, Request* r, int32_t rc, const std::string& err, 
   s->n_req += 1;
   s->n_q += r->m;
   if (rc) s->n_failed += 1;
-  s->lat_ms.push_back(std::chrono::duration<double, std::milli>(t - r->t_submit).count());
+  const double ms = std::chrono::duration<double, std::milli>(t - r->t_submit).count();
+  if (s->lat_ms.size() < fuel_amd_server::LAT_WINDOW) {
+    s->lat_ms.push_back(ms);
+  } else {
+    s->lat_ms[s->lat_next] = ms;
+    s->lat_next = (s->lat_next + 1) % fuel_amd_server::LAT_WINDOW;
+  }
+  s->lat_max = std::max(s->lat_max, ms);
+  s->lat_sum += ms;
 }
 
 void worker_main(fuel_amd_server* s, Worker* w) {
@@ -344,14 +356,14 @@ int32_t fuel_amd_server_stats(fuel_amd_server* s, int32_t reset, fuel_amd_serve_
     };
     st.p50_ms = pct(0.50);
     st.p99_ms = pct(0.99);
-    st.max_ms = *std::max_element(v.begin(), v.end());
-    double sum = 0.0;
-    for (double x : v) sum += x;
-    st.mean_ms = sum / double(n);
+    st.max_ms = s->lat_max;
+    st.mean_ms = s->n_req > 0 ? s->lat_sum / double(s->n_req) : 0.0;
   }
   *out = st;
   if (reset) {
     s->lat_ms.clear();
+    s->lat_next = 0;
+    s->lat_max = s->lat_sum = 0.0;
     s->n_req = s->n_q = s->n_batches = s->max_bq = s->n_failed = 0;
     s->busy_s = 0.0;
     if (s->inflight > 0) s->busy_since = Clock::now();

@@ -17,7 +17,9 @@
  *     (src/distance/euclidean.rs:7-9), and neighbours are ordered by
  *     (distance, index) (src/api/search.rs:23-30).
  *   - host-pointer entry points are synchronous; *_device entry points take
- *     device pointers and a hipStream_t (as void*) and are asynchronous.
+ *     device pointers and a hipStream_t (as void*) and enqueue their work on
+ *     it; fuel_amd_knn_device in an MFMA mode also waits for the stream once
+ *     per query chunk (see there).
  *   - an index is immutable after creation; calls on one index from several
  *     threads are serialized internally.
  */
@@ -135,8 +137,16 @@ int32_t fuel_amd_index_destroy(fuel_amd_index* index);
  * (src/outlier/common.rs:12-49). */
 int32_t fuel_amd_knn(fuel_amd_index* index, const void* query, int64_t m, int32_t k,
                      int32_t exclude_self, int64_t* out_idx, void* out_dist);
-/* Device-pointer form (query rows with stride ldq elements); asynchronous on
- * `stream` except for the certification fallback, which synchronizes. */
+/* Device-pointer form (query rows with stride ldq elements), on `stream`.
+ * In the exact mode the call only enqueues work and returns.  In an MFMA mode
+ * (FP32 / BF16 / BF16R, and AUTO, which picks one of them) the call also waits
+ * for `stream` once per query chunk of up to ~100k queries, after that chunk's
+ * exact rerank: it reads the number of queries whose certificate failed,
+ * because the launches of the next certification level are sized from it.  On
+ * return the results of every chunk but the last are complete; the last
+ * chunk's results are complete when it certified (the common case) and are
+ * otherwise finished by kernels still queued on `stream`.  Synchronize
+ * `stream` before reading the outputs from another stream or the host. */
 int32_t fuel_amd_knn_device(fuel_amd_index* index, const void* d_query, int64_t m, int64_t ldq,
                             int32_t k, int32_t exclude_self, int64_t* d_out_idx, void* d_out_dist,
                             void* stream);
@@ -249,8 +259,8 @@ typedef struct fuel_amd_serve_stats {
   int64_t batches;   /* launches */
   int64_t max_batch_queries;
   int64_t failed;    /* requests answered with an error */
-  double p50_ms;     /* request latency, submit -> results ready on the host */
-  double p99_ms;
+  double p50_ms;     /* request latency, submit -> results ready on the host; p50/p99 over the */
+  double p99_ms;     /* last 2^20 requests (a ring), max and mean over every request since the reset */
   double max_ms;
   double mean_ms;
   double busy_s;     /* seconds with at least one batch in flight */

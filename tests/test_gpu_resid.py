@@ -85,3 +85,26 @@ def test_result_independent_of_batch_composition(gpu):
     for lo, hi in [(0, 1), (5, 70), (300, 700)]:
         bi, bd = ix.knn(Q[lo:hi], 10)
         assert np.array_equal(ai[lo:hi], bi) and ad[lo:hi].tobytes() == bd.tobytes()
+
+
+@pytest.mark.parametrize("kind", ["clustered", "uniform"])
+def test_auto_mode_chooses_by_probe(gpu, oracle, kind):
+    # AUTO probes the plain bf16 certificate on 256 index rows: on the serving
+    # mixture (means U[-10, 10]^96) it fails, and the index takes the residual
+    # form; on uniform data it certifies and the index stays on plain bf16.
+    # Either way the answer is the oracle's.
+    rng = np.random.default_rng(31)
+    if kind == "clustered":
+        X = _mixture(rng, 60000, 96, 50, -10.0, 10.0)
+        Q = X[::301] + np.float32(0.25)
+    else:
+        X = rng.random((60000, 96), dtype=np.float32)
+        Q = rng.random((200, 96), dtype=np.float32)
+    st = _check(gpu, oracle, X, Q, 10, mode="auto")
+    assert st["mode_used"] == (4 if kind == "clustered" else 3), st
+    # decided once: a second call keeps the choice
+    ix = gpu.Index(X, "euclidean", mode="auto")
+    ix.knn(Q[:5], 10)
+    first = ix.stats()["mode_used"]
+    ix.knn(Q, 3)
+    assert ix.stats()["mode_used"] == first
