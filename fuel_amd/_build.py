@@ -28,6 +28,7 @@ SOURCES = [
     "cluster.cpp",
     "serve.cpp",
     "batch.cpp",
+    "priority.cpp",
     "resid.hip",
     "scan_exact.hip",
     "scan_mfma.hip",

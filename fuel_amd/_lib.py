@@ -34,6 +34,11 @@ FORM_SQUARE = 0
 FORM_CONDENSED = 1
 
 MODES = {"auto": MODE_AUTO, "exact": MODE_EXACT, "fp32": MODE_FP32, "bf16": MODE_BF16, "bf16r": MODE_BF16R}
+FLAG_FALLBACK_ARITH = 1
+# rounding of the restated distance: "avx" = fuel's AVX+FMA kernels
+# (src/distance/squared_euclidean.rs:13-67, x86 hosts with AVX2+FMA),
+# "fallback" = its portable kernel (:99-133, every other host)
+ARITH = {"avx": 0, "fallback": FLAG_FALLBACK_ARITH}
 
 # every symbol include/fuel_amd.h declares
 EXPORTED = [
@@ -41,13 +46,17 @@ EXPORTED = [
     "fuel_amd_request_cancel", "fuel_amd_clear_cancel",
     "fuel_amd_index_create", "fuel_amd_index_create_device", "fuel_amd_index_destroy",
     "fuel_amd_knn", "fuel_amd_knn_device", "fuel_amd_range", "fuel_amd_knn_ties", "fuel_amd_csr_free",
-    "fuel_amd_pairwise", "fuel_amd_lof", "fuel_amd_knn_outlier", "fuel_amd_last_stats",
+    "fuel_amd_pairwise", "fuel_amd_index_pairwise", "fuel_amd_lof", "fuel_amd_knn_outlier", "fuel_amd_last_stats",
     "fuel_amd_dbscan", "fuel_amd_optics", "fuel_amd_pairwise_sqdist", "fuel_amd_pairwise_sqdist_device",
     "fuel_amd_timing_enable", "fuel_amd_timing_read",
     "fuel_amd_server_create", "fuel_amd_server_submit", "fuel_amd_server_wait", "fuel_amd_server_stats",
     "fuel_amd_server_destroy",
     "fuel_amd_batch_create", "fuel_amd_batch_run", "fuel_amd_batch_describe", "fuel_amd_batch_destroy",
+    "fuel_amd_priority_create", "fuel_amd_priority_destroy", "fuel_amd_priority_reset",
+    "fuel_amd_priority_decrease_cutoff", "fuel_amd_priority_next", "fuel_amd_priority_all_lower_bound",
 ]
+
+SKIP_POINT = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64)
 
 
 class Opts(ctypes.Structure):
@@ -147,6 +156,8 @@ def load():
     lib.fuel_amd_csr_free.argtypes = [ctypes.POINTER(Csr)]
     lib.fuel_amd_pairwise.argtypes = [P, I64, I32, I32, I32, I32, P]
     lib.fuel_amd_pairwise.restype = I32
+    lib.fuel_amd_index_pairwise.argtypes = [P, I32, P]
+    lib.fuel_amd_index_pairwise.restype = I32
     lib.fuel_amd_lof.argtypes = [P, I32, P]
     lib.fuel_amd_lof.restype = I32
     lib.fuel_amd_knn_outlier.argtypes = [P, I32, P]
@@ -178,6 +189,18 @@ def load():
     lib.fuel_amd_batch_describe.restype = I32
     lib.fuel_amd_batch_destroy.argtypes = [P]
     lib.fuel_amd_batch_destroy.restype = I32
+    lib.fuel_amd_priority_create.argtypes = [P, P, I64, ctypes.POINTER(P)]
+    lib.fuel_amd_priority_create.restype = I32
+    lib.fuel_amd_priority_destroy.argtypes = [P]
+    lib.fuel_amd_priority_destroy.restype = I32
+    lib.fuel_amd_priority_reset.argtypes = [P, I64, ctypes.c_double, ctypes.c_double]
+    lib.fuel_amd_priority_reset.restype = I32
+    lib.fuel_amd_priority_decrease_cutoff.argtypes = [P, I64, ctypes.c_double]
+    lib.fuel_amd_priority_decrease_cutoff.restype = I32
+    lib.fuel_amd_priority_next.argtypes = [P, I64, SKIP_POINT, P, ctypes.POINTER(I64), P]
+    lib.fuel_amd_priority_next.restype = I32
+    lib.fuel_amd_priority_all_lower_bound.argtypes = [P, I64, P]
+    lib.fuel_amd_priority_all_lower_bound.restype = I32
     _lib = lib
     return lib
 
@@ -226,26 +249,26 @@ def metric_code(name):
 class Index:
     """Owner of a fuel_amd_index handle."""
 
-    def __init__(self, data, metric="euclidean", mode="auto", widen=4, device=0):
+    def __init__(self, data, metric="euclidean", mode="auto", widen=4, device=0, arith="avx"):
         lib = load()
         data = np.ascontiguousarray(data)
         if data.ndim != 2:
             raise ValueError("data must be a 2D array")
         self.dtype = data.dtype
         self.n, self.d = data.shape
-        opts = Opts(MODES[mode] if isinstance(mode, str) else int(mode), int(widen), int(device), 0)
+        opts = Opts(MODES[mode] if isinstance(mode, str) else int(mode), int(widen), int(device), ARITH[arith])
         h = ctypes.c_void_p()
         check(lib.fuel_amd_index_create(data.ctypes.data, self.n, self.d, dtype_code(data.dtype),
                                         metric_code(metric), ctypes.byref(opts), ctypes.byref(h)))
         self.handle = h
 
     @classmethod
-    def from_device(cls, ptr, n, d, ld, dtype, metric="euclidean", mode="auto", widen=4, device=0):
+    def from_device(cls, ptr, n, d, ld, dtype, metric="euclidean", mode="auto", widen=4, device=0, arith="avx"):
         lib = load()
         self = cls.__new__(cls)
         self.dtype = np.dtype(dtype)
         self.n, self.d = n, d
-        opts = Opts(MODES[mode] if isinstance(mode, str) else int(mode), int(widen), int(device), 0)
+        opts = Opts(MODES[mode] if isinstance(mode, str) else int(mode), int(widen), int(device), ARITH[arith])
         h = ctypes.c_void_p()
         check(lib.fuel_amd_index_create_device(ctypes.c_void_p(ptr), n, d, ld, dtype_code(dtype),
                                                metric_code(metric), ctypes.byref(opts), ctypes.byref(h)))
@@ -329,6 +352,18 @@ class Index:
             lib.fuel_amd_csr_free(ctypes.byref(c))
         return offs, idx, dist
 
+    def priority(self, query):
+        """fuel's LinearScanPrioritySearcher for every query row (slot q = row q)."""
+        return PrioritySearchers(self, query)
+
+    def pairwise(self, form="square"):
+        """Distance matrix of the index rows (square, or fuel's condensed order)."""
+        n = self.n
+        out = np.empty((n, n) if form == "square" else (n * (n - 1) // 2,), self.dtype)
+        check(load().fuel_amd_index_pairwise(self.handle, FORM_SQUARE if form == "square" else FORM_CONDENSED,
+                                             out.ctypes.data))
+        return out
+
     def lof(self, k):
         out = np.empty(self.n, self.dtype)
         check(load().fuel_amd_lof(self.handle, int(k), out.ctypes.data))
@@ -390,3 +425,56 @@ def pairwise_sqdist(points1, points2):
     check(load().fuel_amd_pairwise_sqdist(p1.ctypes.data, p1.shape[0], p2.ctypes.data, p2.shape[0], p1.shape[1],
                                           dtype_code(dt), out.ctypes.data))
     return out
+
+
+class PrioritySearchers:
+    """One LinearScanPrioritySearcher (src/search/linear_scan.rs:70-235) per
+    query row; slot q answers for query row q.  Methods mirror the
+    PrioritySearcher trait (src/api/search.rs:216-229), with the slot first."""
+
+    def __init__(self, index, query):
+        query = np.ascontiguousarray(query, dtype=index.dtype)
+        if query.ndim != 2 or query.shape[1] != index.d:
+            raise ValueError("query shape mismatch")
+        self.index = index  # keeps the index alive
+        self.m = query.shape[0]
+        self._cty = ctypes.c_float if index.dtype == np.float32 else ctypes.c_double
+        h = ctypes.c_void_p()
+        check(load().fuel_amd_priority_create(index.handle, query.ctypes.data, self.m, ctypes.byref(h)))
+        self.handle = h
+        self._no_filter = ctypes.cast(None, SKIP_POINT)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            load().fuel_amd_priority_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def reset(self, q):
+        check(load().fuel_amd_priority_reset(self.handle, int(q), float("inf"), 0.0))
+
+    def reset_with_limits(self, q, cutoff, skip):
+        check(load().fuel_amd_priority_reset(self.handle, int(q), float(cutoff), float(skip)))
+
+    def decrease_cutoff(self, q, threshold):
+        check(load().fuel_amd_priority_decrease_cutoff(self.handle, int(q), float(threshold)))
+
+    def next(self, q, skip_point=None):
+        """(index, distance) of the next neighbour of slot q, or None."""
+        idx = ctypes.c_int64()
+        dist = self._cty()
+        cb = self._no_filter if skip_point is None else SKIP_POINT(lambda _c, i: 1 if skip_point(int(i)) else 0)
+        check(load().fuel_amd_priority_next(self.handle, int(q), cb, None, ctypes.byref(idx), ctypes.byref(dist)))
+        if idx.value < 0:
+            return None
+        return int(idx.value), np.array(dist.value, dtype=self.index.dtype)[()]
+
+    def all_lower_bound(self, q):
+        v = self._cty()
+        check(load().fuel_amd_priority_all_lower_bound(self.handle, int(q), ctypes.byref(v)))
+        return np.array(v.value, dtype=self.index.dtype)[()]

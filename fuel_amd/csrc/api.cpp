@@ -187,6 +187,7 @@ static int32_t index_create_common(const void* data, bool on_device, int64_t n, 
   if (opts) o = *opts;
   if (o.widen <= 0) o.widen = 4;
   if (o.mode < FUEL_AMD_MODE_AUTO || o.mode > FUEL_AMD_MODE_BF16R) return fail(FUEL_AMD_ERR_INVALID, "bad mode");
+  if (o.flags & ~FUEL_AMD_FLAG_FALLBACK_ARITH) return fail(FUEL_AMD_ERR_INVALID, "unknown option flags");
   int32_t rc = check_device(o.device);
   if (rc) return rc;
   auto* ix = new (std::nothrow) fuel_amd_index();
@@ -197,6 +198,7 @@ static int32_t index_create_common(const void* data, bool on_device, int64_t n, 
   ix->n = n;
   ix->d = d;
   ix->opts = o;
+  ix->arith = (o.flags & FUEL_AMD_FLAG_FALLBACK_ARITH) ? fa::ARITH_FALLBACK : fa::ARITH_AVX;
   ix->ctx.timed = true;
   const size_t es = dtype == FUEL_AMD_F32 ? 4 : 8;
   // rows padded to a 32-byte multiple so the exact kernels can use 16-byte loads
@@ -497,6 +499,17 @@ int32_t fuel_amd_pairwise(const void* data, int64_t n, int32_t d, int32_t dtype,
   }
   fuel_amd_index_destroy(ix);
   return rc;
+}
+
+int32_t fuel_amd_index_pairwise(fuel_amd_index* ix, int32_t form, void* out) {
+  g_last_error.clear();
+  if (!ix || (!out && ix->n > 1)) return fail(FUEL_AMD_ERR_INVALID, "NULL argument");
+  if (form != FUEL_AMD_FORM_SQUARE && form != FUEL_AMD_FORM_CONDENSED)
+    return fail(FUEL_AMD_ERR_INVALID, "form must be SQUARE or CONDENSED");
+  std::lock_guard<std::mutex> lock(ix->mu);
+  int32_t rc = check_device(ix->device);
+  if (rc) return rc;
+  return pairwise_run(ix, form, out);
 }
 
 // ---- outlier scores -----------------------------------------------------------

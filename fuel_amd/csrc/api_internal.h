@@ -56,6 +56,7 @@ struct QueryCtx {
     DevBuf Qop, Q32, qnorm, qint, slots, lcnts, fail_count, fail_list, fail_theta, gthr, qtab, qs2;
   } lv[3];
   DevBuf slots, lcnts;  // exact scan
+  DevBuf raw, lo, rcnt;  // exact kNN in rounds (k beyond one slot)
   fuel_amd_stats stats{};
   bool timed = false;  // record kernel timing events (the index's own context)
   void release() {
@@ -75,6 +76,9 @@ struct QueryCtx {
     }
     slots.release();
     lcnts.release();
+    raw.release();
+    lo.release();
+    rcnt.release();
   }
 };
 
@@ -87,6 +91,7 @@ struct fuel_amd_index {
   int64_t n = 0;
   int d = 0;
   int64_t ld = 0;  // padded row stride of the resident copy (elements)
+  int arith = 0;   // rounding of the restated distance: fa::ARITH_AVX or fa::ARITH_FALLBACK
   fuel_amd_opts opts{};
   hipStream_t stream = nullptr;
   std::mutex mu;
@@ -156,6 +161,11 @@ int32_t prepare_mode(fuel_amd_index* ix, hipStream_t st);
 int32_t range_run(fuel_amd_index* ix, const void* query, int64_t m, double radius, int exclude_self,
                   fuel_amd_csr* out);
 int32_t knn_ties_run(fuel_amd_index* ix, const void* query, int64_t m, int k, fuel_amd_csr* out);
+// Every row with skip[i] <= distance <= cutoff[i] of m device query rows (pitch
+// ix->ld; cutoff and skip: m device values of the index dtype), sorted by
+// (distance, index), as a host CSR.
+int32_t range_window_run(fuel_amd_index* ix, const void* d_q, int64_t m, const void* d_cutoff, const void* d_skip,
+                         fuel_amd_csr* out);
 // Exact range of every indexed row against the index (self included), host CSR.
 int32_t range_self_run(fuel_amd_index* ix, double radius, fuel_amd_csr* out);
 // Density clustering over range_self_run (cluster.cpp).  variant 0: dbscan,

@@ -789,7 +789,6 @@ int32_t fuel_amd_batch_create(const fuel_amd_problem* problems, int32_t count, i
       if (!p.data || !p.out_idx || !p.out_dist) return fail(FUEL_AMD_ERR_INVALID, "NULL problem buffer");
       if (p.ld < std::max(p.d, 1) || p.ld % 8 != 0) return fail(FUEL_AMD_ERR_INVALID, "ld must be >= d and a multiple of 8");
       if (reinterpret_cast<uintptr_t>(p.data) % 32 != 0) return fail(FUEL_AMD_ERR_INVALID, "rows must be 32-byte aligned");
-      if (p.k + (exclude_self ? 1 : 0) > 255) return fail(FUEL_AMD_ERR_UNSUPPORTED, "k above 254 is not supported");
     }
   }
   int32_t rc = check_device(o.device);

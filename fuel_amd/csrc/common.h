@@ -29,35 +29,15 @@ struct Lanes<double> {
   static constexpr int N = 4;
 };
 
+// Rounding forms of the restated distance (the index option
+// FUEL_AMD_FLAG_FALLBACK_ARITH selects the second).
+constexpr int ARITH_AVX = 0;
+constexpr int ARITH_FALLBACK = 1;
+
 __device__ __forceinline__ float fma_rn(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma_rn(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float sqrt_rn(float x) { return __builtin_sqrtf(x); }
 __device__ __forceinline__ double sqrt_rn(double x) { return __builtin_sqrt(x); }
-
-// Generic restated distance between two rows held in memory (any address space).
-template <typename F>
-__device__ __forceinline__ F restated_sqdist(const F* __restrict__ a, const F* __restrict__ b, int d) {
-  constexpr int L = Lanes<F>::N;
-  const int sd = d & ~(L - 1);
-  F acc[L];
-#pragma unroll
-  for (int j = 0; j < L; ++j) acc[j] = F(0);
-  for (int i = 0; i < sd; i += L) {
-#pragma unroll
-    for (int j = 0; j < L; ++j) {
-      F t = a[i + j] - b[i + j];
-      acc[j] = fma_rn(t, t, acc[j]);
-    }
-  }
-  F s = F(-0.0);
-#pragma unroll
-  for (int j = 0; j < L; ++j) s = s + acc[j];
-  for (int i = sd; i < d; ++i) {
-    F t = a[i] - b[i];
-    s = fma_rn(t, t, s);
-  }
-  return s;
-}
 
 // ---------------------------------------------------------------------------
 // Composite keys.  A candidate is (key, index) ordered lexicographically, the

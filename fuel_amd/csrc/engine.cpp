@@ -21,6 +21,98 @@ constexpr size_t SLOT_BUDGET = size_t(1) << 30;  // bytes of candidate slots per
 
 int exact_qw(int64_t nq) { return nq >= 64 ? 16 : (nq >= 8 ? 8 : 4); }
 
+// kNN with k beyond one slot's KK = EXACT_NS / 2 (fuel's KNNHeap takes any k,
+// src/api/search.rs:80-151): the top-KK composites are found in rounds of RK =
+// EXACT_NS / 2, each an exact scan that keeps only the candidates ordering
+// after the previous round's last composite; the rounds' sorted lists then
+// concatenate to the sorted top KK, which the row writer packs like
+// write_knn_row.
+template <typename F>
+int32_t knn_exact_rounds(fuel_amd_index* ix, QueryCtx* ctx, const F* Q, int64_t m, int64_t ldq, int k, int mode,
+                         int64_t* out_idx, F* out_dist, hipStream_t st, const int64_t* qrows, int64_t qbase) {
+  using T = typename Comp<F>::T;
+  const int KK = k + (mode != EXCL_NONE ? 1 : 0);
+  constexpr int RK = EXACT_NS / 2;
+  const int rounds = (KK + RK - 1) / RK;
+  const int64_t stride = int64_t(rounds) * RK;
+  const int cus = device_cu_count(ix->device);
+  int64_t done = 0;
+  while (done < m) {
+    if (cancelled()) return fail(FUEL_AMD_ERR_INTERRUPTED, "interrupted");
+    int64_t chunk = m - done;
+    const int qw = exact_qw(chunk);
+    int nsplit = choose_splits((chunk + qw - 1) / qw, ix->n, cus, 512);
+    const size_t per_q = size_t(nsplit) * EXACT_NS * sizeof(T) + size_t(stride) * sizeof(T);
+    chunk = std::min<int64_t>(chunk, std::max<int64_t>(qw, int64_t(SLOT_BUDGET / per_q)));
+    hipError_t e = ctx->slots.ensure(size_t(nsplit) * EXACT_NS * sizeof(T) * chunk);
+    if (e == hipSuccess) e = ctx->lcnts.ensure(sizeof(int) * size_t(nsplit) * chunk);
+    if (e == hipSuccess) e = ctx->raw.ensure(sizeof(T) * size_t(stride) * chunk);
+    if (e == hipSuccess) e = ctx->lo.ensure(sizeof(T) * size_t(chunk));
+    if (e == hipSuccess) e = ctx->rcnt.ensure(sizeof(int) * size_t(chunk));
+    if (e == hipSuccess) e = launch_fill_u32(ctx->rcnt.as<uint32_t>(), chunk, 0u, st);
+    if (e != hipSuccess) return hip_fail(e, "round workspaces");
+    int64_t rps = (ix->n + nsplit - 1) / nsplit;
+    rps = std::max<int64_t>(256, (rps + 255) & ~int64_t(255));
+    nsplit = (int)std::max<int64_t>(1, (ix->n + rps - 1) / rps);
+    for (int r = 0; r < rounds; ++r) {
+      if (cancelled()) return fail(FUEL_AMD_ERR_INTERRUPTED, "interrupted");
+      ScanExactArgs<F> sa{};
+      sa.X = ix->X.as<F>();
+      sa.ldx = ix->ld;
+      sa.n = ix->n;
+      sa.d = ix->d;
+      sa.Q = qrows ? Q : Q + done * ldq;
+      sa.ldq = ldq;
+      sa.qrows = qrows ? qrows + done : nullptr;
+      sa.nq = chunk;
+      sa.metric = ix->metric;
+      sa.arith = ix->arith;
+      sa.KP = RK;
+      sa.rows_per_split = rps;
+      sa.slots = ctx->slots.as<T>();
+      sa.lcnts = ctx->lcnts.as<int>();
+      sa.lo = r > 0 ? ctx->lo.as<T>() : nullptr;
+      if (ctx->timed) timing_mark(ix, st, true, "scan_exact_kernel");
+      e = launch_scan_exact<F>(sa, qw, nsplit, st);
+      if (ctx->timed) timing_mark(ix, st, false, nullptr);
+      if (e != hipSuccess) return hip_fail(e, "scan_exact launch");
+      FinalizeArgs<F> fa_{};
+      fa_.slots = sa.slots;
+      fa_.lcnts = sa.lcnts;
+      fa_.slot_stride = EXACT_NS;
+      fa_.nsplit = nsplit;
+      fa_.nq = chunk;
+      fa_.KK = RK;
+      fa_.k = RK;
+      fa_.mode = EXCL_NONE;
+      fa_.raw_out = ctx->raw.as<T>();
+      fa_.raw_stride = stride;
+      fa_.raw_off = int64_t(r) * RK;
+      fa_.lo_out = ctx->lo.as<T>();
+      fa_.cnt_out = ctx->rcnt.as<int>();
+      e = launch_finalize_exact<F>(fa_, st);
+      if (e != hipSuccess) return hip_fail(e, "finalize launch");
+    }
+    LongRowArgs<F> la{};
+    la.raw = ctx->raw.as<T>();
+    la.stride = stride;
+    la.cnt = ctx->rcnt.as<int>();
+    la.nq = chunk;
+    la.k = k;
+    la.mode = mode;
+    la.qrows = qrows ? qrows + done : nullptr;
+    la.qbase = qrows ? qbase : qbase + done;
+    la.out_idx = qrows ? out_idx : out_idx + done * k;
+    la.out_dist = qrows ? out_dist : out_dist + done * k;
+    e = launch_write_long_rows<F>(la, st);
+    if (e != hipSuccess) return hip_fail(e, "kNN rows");
+    ctx->stats.splits = nsplit;
+    ctx->stats.kp = KK;
+    done += chunk;
+  }
+  return FUEL_AMD_OK;
+}
+
 // Exact kNN of m query slots.  With qrows, slot i is query row qrows[i] of Q
 // (a device array) and its result goes to output row qrows[i].
 template <typename F>
@@ -28,8 +120,7 @@ int32_t knn_exact(fuel_amd_index* ix, QueryCtx* ctx, const F* Q, int64_t m, int6
                   int64_t* out_idx, F* out_dist, hipStream_t st, const int64_t* qrows = nullptr, int64_t qbase = 0) {
   using T = typename Comp<F>::T;
   const int KK = k + (mode != EXCL_NONE ? 1 : 0);
-  if (KK > EXACT_NS / 2)
-    return fail(FUEL_AMD_ERR_UNSUPPORTED, "k above " + std::to_string(EXACT_NS / 2 - 1) + " is not supported");
+  if (KK > EXACT_NS / 2) return knn_exact_rounds<F>(ix, ctx, Q, m, ldq, k, mode, out_idx, out_dist, st, qrows, qbase);
   const int cus = device_cu_count(ix->device);
   int64_t done = 0;
   while (done < m) {
@@ -52,6 +143,7 @@ int32_t knn_exact(fuel_amd_index* ix, QueryCtx* ctx, const F* Q, int64_t m, int6
     sa.qrows = qrows ? qrows + done : nullptr;
     sa.nq = chunk;
     sa.metric = ix->metric;
+    sa.arith = ix->arith;
     sa.KP = KK;
     int64_t rps = (ix->n + nsplit - 1) / nsplit;
     rps = std::max<int64_t>(256, (rps + 255) & ~int64_t(255));
@@ -86,10 +178,11 @@ int32_t knn_exact(fuel_amd_index* ix, QueryCtx* ctx, const F* Q, int64_t m, int6
 }
 
 // Range search of m query rows already in device memory (pitch ix->ld), with a
-// common radius or per-query radii (d_radii[i * radii_stride]), into a host CSR.
+// common radius or per-query radii (d_radii[i * radii_stride]) and optional
+// per-query lower limits (d_skips[i]: keep distances >= it), into a host CSR.
 template <typename F>
 int32_t range_core(fuel_amd_index* ix, const F* d_q, int64_t m, double radius, const F* d_radii,
-                   int64_t radii_stride, int exclude_self, fuel_amd_csr* out) {
+                   int64_t radii_stride, int exclude_self, fuel_amd_csr* out, const F* d_skips = nullptr) {
   using K = typename OrdKey<F>::T;
   hipStream_t st = ix->stream;
   const size_t es = sizeof(F);
@@ -121,9 +214,11 @@ int32_t range_core(fuel_amd_index* ix, const F* d_q, int64_t m, double radius, c
   ra.nq = m;
   ra.qbase = 0;
   ra.metric = ix->metric;
+  ra.arith = ix->arith;
   ra.radius = (F)radius;  // python/search.rs:785 (`radius as f32`)
   ra.radii = d_radii;
   ra.radii_stride = radii_stride;
+  ra.skips = d_skips;
   ra.exclude_self = exclude_self ? 1 : 0;
   ra.nsplit = nsplit;
   ra.rows_per_split = rps;
@@ -246,6 +341,7 @@ int32_t pairwise_impl(fuel_amd_index* ix, int form, void* out) {
   pa.n = n;
   pa.d = ix->d;
   pa.metric = ix->metric;
+  pa.arith = ix->arith;
   pa.rows_per_split = rps;
   pa.out = ix->ws_a.as<F>();
   e = launch_pairwise<F>(pa, form, nsplit, st);
@@ -336,6 +432,17 @@ int32_t range_self_run(fuel_amd_index* ix, double radius, fuel_amd_csr* out) {
   int32_t rc = ix->dtype == FUEL_AMD_F32
                    ? range_core<float>(ix, ix->X.as<float>(), ix->n, radius, nullptr, 0, 0, out)
                    : range_core<double>(ix, ix->X.as<double>(), ix->n, radius, nullptr, 0, 0, out);
+  if (rc) fuel_amd_csr_free(out);
+  return rc;
+}
+
+int32_t range_window_run(fuel_amd_index* ix, const void* d_q, int64_t m, const void* d_cutoff, const void* d_skip,
+                         fuel_amd_csr* out) {
+  int32_t rc = ix->dtype == FUEL_AMD_F32
+                   ? range_core<float>(ix, (const float*)d_q, m, 0.0, (const float*)d_cutoff, 1, 0, out,
+                                       (const float*)d_skip)
+                   : range_core<double>(ix, (const double*)d_q, m, 0.0, (const double*)d_cutoff, 1, 0, out,
+                                        (const double*)d_skip);
   if (rc) fuel_amd_csr_free(out);
   return rc;
 }

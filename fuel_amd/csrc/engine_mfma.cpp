@@ -547,6 +547,7 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
     ra.ld32 = ld32;
     ra.d = ix->d;
     ra.metric = ix->metric;
+    ra.arith = ix->arith;
     ra.bf16 = mk == 0;
     ra.resid = resid ? 1 : 0;
     ra.rperm = resid ? op.rperm.as<uint32_t>() : nullptr;

@@ -27,12 +27,29 @@ __device__ __forceinline__ void stage_queries(F (*qbuf)[DC], const QRow& qrow, i
   }
 }
 
+// Lanes and rounding of the restated distance.  ARITH_AVX: fuel's AVX+FMA
+// kernels (src/distance/squared_euclidean.rs:13-67), LANES = 8 (f32) or 4 (f64)
+// fused lanes, a left fold from -0.0 (Rust's Sum), a fused tail.  ARITH_FALLBACK:
+// the kernel of hosts without AVX+FMA (:99-133), 8 lanes of fl(t * t) + acc for
+// both dtypes, a left fold from +0, an unfused tail.
+template <typename F, int ARITH>
+struct Arith {
+  static constexpr int L = ARITH == ARITH_FALLBACK ? 8 : Lanes<F>::N;
+  __device__ static __forceinline__ F step(F t, F acc) {
+    if constexpr (ARITH == ARITH_FALLBACK) return t * t + acc;  // two roundings (-ffp-contract=off)
+    else return fma_rn(t, t, acc);
+  }
+  __device__ static __forceinline__ F fold_start() { return ARITH == ARITH_FALLBACK ? F(0) : F(-0.0); }
+};
+
 // s[qq] = restated squared distance between query qq and the lane's row xr.
-// `single` means all d coordinates are already staged (d <= DC).
-template <typename F, int QW, int DC, typename QRow>
+// `single` means all d coordinates are already staged (d <= DC, a multiple of 8).
+template <typename F, int QW, int DC, int ARITH = ARITH_AVX, typename QRow>
 __device__ __forceinline__ void exact_step(F (*qbuf)[DC], const QRow& qrow, int nqb, bool single,
                                            const F* __restrict__ xr, int d, F (&s)[QW]) {
-  constexpr int L = Lanes<F>::N;
+  using AR = Arith<F, ARITH>;
+  constexpr int L = AR::L;
+  static_assert(DC % L == 0, "lane groups do not straddle staged chunks");
   const int sd = d & ~(L - 1);
   F acc[QW][L];
 #pragma unroll
@@ -57,25 +74,29 @@ __device__ __forceinline__ void exact_step(F (*qbuf)[DC], const QRow& qrow, int 
         xv[4] = u1.x; xv[5] = u1.y; xv[6] = u1.z; xv[7] = u1.w;
       } else {
         const double2* p = reinterpret_cast<const double2*>(xr + i);
-        const double2 u0 = p[0], u1 = p[1];
-        xv[0] = u0.x; xv[1] = u0.y; xv[2] = u1.x; xv[3] = u1.y;
+#pragma unroll
+        for (int h = 0; h < L / 2; ++h) {
+          const double2 u = p[h];
+          xv[2 * h] = u.x;
+          xv[2 * h + 1] = u.y;
+        }
       }
 #pragma unroll
       for (int qq = 0; qq < QW; ++qq) {
 #pragma unroll
         for (int j = 0; j < L; ++j) {
           const F t = qbuf[qq][i - c0 + j] - xv[j];
-          acc[qq][j] = fma_rn(t, t, acc[qq][j]);
+          acc[qq][j] = AR::step(t, acc[qq][j]);
         }
       }
     }
   }
-  // Left-fold lane sum (start value -0.0, as Rust's f32/f64 Sum), then the fused
-  // tail over d mod L.  The tail coordinates are in the last staged chunk.
+  // Left-fold lane sum, then the tail over d mod L.  The tail coordinates are
+  // in the last staged chunk.
   const int c0_last = d > 0 ? ((d - 1) / DC) * DC : 0;
 #pragma unroll
   for (int qq = 0; qq < QW; ++qq) {
-    F t = F(-0.0);
+    F t = AR::fold_start();
 #pragma unroll
     for (int j = 0; j < L; ++j) t = t + acc[qq][j];
     s[qq] = t;
@@ -85,7 +106,7 @@ __device__ __forceinline__ void exact_step(F (*qbuf)[DC], const QRow& qrow, int 
 #pragma unroll
     for (int qq = 0; qq < QW; ++qq) {
       const F t = qbuf[qq][i - c0_last] - xv;
-      s[qq] = fma_rn(t, t, s[qq]);
+      s[qq] = AR::step(t, s[qq]);
     }
   }
 }

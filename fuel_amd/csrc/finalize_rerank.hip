@@ -36,30 +36,34 @@
 
 namespace fa {
 
-// Restated f32 distance of two 32-byte-aligned rows (fuel's AVX2 order).
+// Restated f32 distance of two 32-byte-aligned rows in fuel's order: 8 lanes
+// interleaved over the coordinates, a left fold, then the tail; ARITH_AVX fuses
+// every lane and tail update (src/distance/squared_euclidean.rs:13-39),
+// ARITH_FALLBACK rounds the square first (:99-133).
+template <int ARITH>
 __device__ __forceinline__ float restated_aligned(const float* __restrict__ a, const float* __restrict__ b, int d) {
+  auto step = [](float t, float acc) {
+    if constexpr (ARITH == ARITH_FALLBACK) return t * t + acc;  // two roundings (-ffp-contract=off)
+    else return fma_rn(t, t, acc);
+  };
   const int sd = d & ~7;
   float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int i = 0; i < sd; i += 8) {
     const float4 a0 = *reinterpret_cast<const float4*>(a + i), a1 = *reinterpret_cast<const float4*>(a + i + 4);
     const float4 b0 = *reinterpret_cast<const float4*>(b + i), b1 = *reinterpret_cast<const float4*>(b + i + 4);
-    float t;
-    t = a0.x - b0.x; acc[0] = fma_rn(t, t, acc[0]);
-    t = a0.y - b0.y; acc[1] = fma_rn(t, t, acc[1]);
-    t = a0.z - b0.z; acc[2] = fma_rn(t, t, acc[2]);
-    t = a0.w - b0.w; acc[3] = fma_rn(t, t, acc[3]);
-    t = a1.x - b1.x; acc[4] = fma_rn(t, t, acc[4]);
-    t = a1.y - b1.y; acc[5] = fma_rn(t, t, acc[5]);
-    t = a1.z - b1.z; acc[6] = fma_rn(t, t, acc[6]);
-    t = a1.w - b1.w; acc[7] = fma_rn(t, t, acc[7]);
+    acc[0] = step(a0.x - b0.x, acc[0]);
+    acc[1] = step(a0.y - b0.y, acc[1]);
+    acc[2] = step(a0.z - b0.z, acc[2]);
+    acc[3] = step(a0.w - b0.w, acc[3]);
+    acc[4] = step(a1.x - b1.x, acc[4]);
+    acc[5] = step(a1.y - b1.y, acc[5]);
+    acc[6] = step(a1.z - b1.z, acc[6]);
+    acc[7] = step(a1.w - b1.w, acc[7]);
   }
-  float s = -0.0f;
+  float s = ARITH == ARITH_FALLBACK ? 0.0f : -0.0f;
 #pragma unroll
   for (int j = 0; j < 8; ++j) s = s + acc[j];
-  for (int i = sd; i < d; ++i) {
-    const float t = a[i] - b[i];
-    s = fma_rn(t, t, s);
-  }
+  for (int i = sd; i < d; ++i) s = step(a[i] - b[i], s);
   return s;
 }
 
@@ -100,6 +104,10 @@ __device__ __forceinline__ BoundTerms bound_terms(const RerankArgs& a, int64_t q
   const double e_tag = 0x1p-19 * (dot_mag + H) * (1 + 0x1p-10) + 0x1p-145;
   double e_s = e_round + e_hx + gacc * (dot_mag + H) + e_tag + (mfma_exact ? 0.0 : 2.0 * (d + 2) * 0x1p-126);
   b.e_s = e_s * (1 + 0x1p-40);
+  // relative error of the restated sum of non-negative terms: a lane term sees
+  // the subtraction (2u), its square's rounding (fused or not: u), its lane's
+  // remaining updates (d/8 - 1), the 7 fold additions and the d%8 tail
+  // updates; a tail term at most 2u + u + d%8 u.  Both rounding forms fit.
   const int n_r = d / 8 + 8 + (d % 8) + 2;
   b.gr = n_r * u * (1 + 0x1p-10);
   return b;
@@ -208,7 +216,7 @@ __device__ __forceinline__ void sort_prefix(T (&x)[V], int count) {
   }
 }
 
-template <int V>
+template <int V, int ARITH>
 __global__ __launch_bounds__(64) void rerank_kernel(RerankArgs a) {
   using C = Comp<float>;
   using T = uint64_t;
@@ -307,7 +315,7 @@ __global__ __launch_bounds__(64) void rerank_kernel(RerankArgs a) {
       uint32_t idx = C::idx(x[v]);
       if (a.resid) idx = a.rperm[idx];  // padded position -> row (padding rows never survive)
       if (idx != RPERM_NONE) {
-        const float s = restated_aligned(qrow, a.X + size_t(idx) * a.ldx, a.d);
+        const float s = restated_aligned<ARITH>(qrow, a.X + size_t(idx) * a.ldx, a.d);
         const float key = a.metric == 0 ? sqrt_rn(s) : s;
         y[v] = C::make(key, idx);
       }
@@ -341,10 +349,15 @@ __global__ __launch_bounds__(64) void rerank_kernel(RerankArgs a) {
 
 hipError_t launch_rerank(const RerankArgs& a, int V, hipStream_t st) {
   if (a.nq == 0) return hipSuccess;
-  if (V == 4)
-    hipLaunchKernelGGL((rerank_kernel<4>), dim3((unsigned)a.nq), dim3(64), 0, st, a);
+  const bool fb = a.arith == ARITH_FALLBACK;
+  if (V == 4 && !fb)
+    hipLaunchKernelGGL((rerank_kernel<4, ARITH_AVX>), dim3((unsigned)a.nq), dim3(64), 0, st, a);
+  else if (V == 8 && !fb)
+    hipLaunchKernelGGL((rerank_kernel<8, ARITH_AVX>), dim3((unsigned)a.nq), dim3(64), 0, st, a);
+  else if (V == 4)
+    hipLaunchKernelGGL((rerank_kernel<4, ARITH_FALLBACK>), dim3((unsigned)a.nq), dim3(64), 0, st, a);
   else if (V == 8)
-    hipLaunchKernelGGL((rerank_kernel<8>), dim3((unsigned)a.nq), dim3(64), 0, st, a);
+    hipLaunchKernelGGL((rerank_kernel<8, ARITH_FALLBACK>), dim3((unsigned)a.nq), dim3(64), 0, st, a);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

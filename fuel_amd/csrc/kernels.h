@@ -26,10 +26,12 @@ struct ScanExactArgs {
   const int64_t* qrows;  // slot -> query row (nullptr: identity)
   int64_t nq;            // number of query slots
   int metric;            // 0 euclidean, 1 sqeuclidean
+  int arith;             // ARITH_AVX or ARITH_FALLBACK
   int KP;
   int64_t rows_per_split;
   typename Comp<F>::T* slots;  // [nsplit][nq][NS]
   int* lcnts;                  // [nsplit][nq]
+  const typename Comp<F>::T* lo;  // optional [nq]: keep only candidates ordering after lo[slot]
 };
 
 // Order-preserving unsigned keys of a distance (u32 for f32, u64 for f64).
@@ -63,9 +65,11 @@ struct RangeArgs {
   int64_t nq;
   int64_t qbase;  // data row of query slot 0 (self-exclusion)
   int metric;
+  int arith;      // ARITH_AVX or ARITH_FALLBACK
   F radius;
   const F* radii;        // optional per-query radius (query slot i: radii[i * radii_stride])
   int64_t radii_stride;
+  const F* skips;        // optional per-query lower limit (keep distances >= skips[i])
   int exclude_self;
   int nsplit;
   int64_t rows_per_split;
@@ -82,6 +86,7 @@ struct PairwiseArgs {
   int64_t n;
   int d;
   int metric;
+  int arith;  // ARITH_AVX or ARITH_FALLBACK
   int64_t rows_per_split;
   F* out;
 };
@@ -111,7 +116,34 @@ struct FinalizeArgs {
   int64_t qbase;         // data row of batch query 0 (self-exclusion)
   int64_t* out_idx;      // rows indexed by query row
   F* out_dist;
+  // Instead of output rows: the slot's sorted top-KK composites go to
+  // raw_out[slot * raw_stride + raw_off + e] (MAX-padded), their count is added
+  // to cnt_out[slot], and lo_out[slot] receives the KK-th (MAX when fewer were
+  // found): the round of a kNN whose k exceeds one slot.
+  typename Comp<F>::T* raw_out;
+  int64_t raw_stride;
+  int64_t raw_off;
+  typename Comp<F>::T* lo_out;
+  int* cnt_out;
 };
+
+// Output rows of a kNN found in rounds: slot s holds cnt[s] sorted composites
+// raw[s * stride ...]; fuel's exclusion / truncation / padding as write_knn_row.
+template <typename F>
+struct LongRowArgs {
+  const typename Comp<F>::T* raw;
+  int64_t stride;
+  const int* cnt;
+  int64_t nq;
+  int k;
+  int mode;
+  const int64_t* qrows;
+  int64_t qbase;
+  int64_t* out_idx;
+  F* out_dist;
+};
+template <typename F>
+hipError_t launch_write_long_rows(const LongRowArgs<F>& a, hipStream_t st);
 
 // Write one output row from a wave-resident, ascending, lexicographically sorted
 // candidate set x (count valid entries) with fuel's exclusion/truncation/padding
@@ -346,6 +378,7 @@ struct RerankArgs {
   int64_t ld32;
   int d;
   int metric;
+  int arith;  // rounding of the exact distances: ARITH_AVX or ARITH_FALLBACK
   int bf16;
   const double* qn2;
   const double* eq2;

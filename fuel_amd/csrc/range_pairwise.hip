@@ -15,7 +15,7 @@
 
 namespace fa {
 
-template <typename F, int QW, int PASS>
+template <typename F, int QW, int PASS, int ARITH>
 __global__ __launch_bounds__(256) void range_kernel(RangeArgs<F> a) {
   constexpr int DC = ExactCfg<F, QW>::DC;
   __shared__ F qbuf[QW][DC];
@@ -36,22 +36,24 @@ __global__ __launch_bounds__(256) void range_kernel(RangeArgs<F> a) {
   const bool single = d <= DC;
   if (single) stage_queries<F, QW, DC>(qbuf, qrow, nqb, 0, d);
   __syncthreads();
-  F rad[QW], rs[QW];
+  F rad[QW], rs[QW], lo[QW];
 #pragma unroll
   for (int qq = 0; qq < QW; ++qq) {
     rad[qq] = (a.radii && qq < nqb) ? a.radii[(qs0 + qq) * a.radii_stride] : a.radius;
     rs[qq] = a.metric == 0 ? sq_upper(rad[qq]) : rad[qq];
+    lo[qq] = (a.skips && qq < nqb) ? a.skips[qs0 + qq] : F(-INFINITY);
   }
   for (int64_t rb = r0; rb < r1; rb += 256) {
     const int64_t row = rb + tid;
     const bool valid = row < r1;
     F s[QW];
-    exact_step<F, QW, DC>(qbuf, qrow, nqb, single, a.X + (valid ? row : r0) * a.ldx, d, s);
+    exact_step<F, QW, DC, ARITH>(qbuf, qrow, nqb, single, a.X + (valid ? row : r0) * a.ldx, d, s);
     uint64_t hits[QW];
 #pragma unroll
     for (int qq = 0; qq < QW; ++qq) {
       bool hit = qq < nqb && valid && s[qq] <= rs[qq] && !(a.exclude_self && row == a.qbase + qs0 + qq);
       if (hit && a.metric == 0) hit = sqrt_rn(s[qq]) <= rad[qq];  // exact predicate
+      if (hit && a.skips) hit = (a.metric == 0 ? sqrt_rn(s[qq]) : s[qq]) >= lo[qq];
       hits[qq] = __ballot(hit);
       if (__lane_id() == 0) wcnt[qq][wave] = __popcll(hits[qq]);
     }
@@ -79,7 +81,7 @@ __global__ __launch_bounds__(256) void range_kernel(RangeArgs<F> a) {
   }
 }
 
-template <typename F, int QW, int FORM>
+template <typename F, int QW, int FORM, int ARITH>
 __global__ __launch_bounds__(256) void pairwise_kernel(PairwiseArgs<F> a) {
   constexpr int DC = ExactCfg<F, QW>::DC;
   __shared__ F qbuf[QW][DC];
@@ -101,7 +103,7 @@ __global__ __launch_bounds__(256) void pairwise_kernel(PairwiseArgs<F> a) {
     const int64_t j = rb + tid;
     const bool valid = j < r1;
     F s[QW];
-    exact_step<F, QW, DC>(qbuf, qrow, nqb, single, a.X + (valid ? j : r0) * a.ldx, d, s);
+    exact_step<F, QW, DC, ARITH>(qbuf, qrow, nqb, single, a.X + (valid ? j : r0) * a.ldx, d, s);
 #pragma unroll
     for (int qq = 0; qq < QW; ++qq) {
       const int64_t i = qs0 + qq;
@@ -117,25 +119,61 @@ __global__ __launch_bounds__(256) void pairwise_kernel(PairwiseArgs<F> a) {
   }
 }
 
+// Queries per workgroup: 16, and 8 for the f64 fallback form, whose 8 double
+// lanes per query would not stay in registers for 16 (the grid follows).
 template <typename F>
 hipError_t launch_range(const RangeArgs<F>& a, int pass, int nsplit, hipStream_t st) {
   constexpr int QW = 16;
+  constexpr int QW8 = 8;
+  if (a.arith == ARITH_FALLBACK) {
+    if constexpr (sizeof(F) == 8) {
+      dim3 grid((unsigned)((a.nq + QW8 - 1) / QW8), (unsigned)nsplit);
+      if (pass == 0)
+        hipLaunchKernelGGL((range_kernel<F, QW8, 0, ARITH_FALLBACK>), grid, dim3(256), 0, st, a);
+      else
+        hipLaunchKernelGGL((range_kernel<F, QW8, 1, ARITH_FALLBACK>), grid, dim3(256), 0, st, a);
+    } else {
+      dim3 grid((unsigned)((a.nq + QW - 1) / QW), (unsigned)nsplit);
+      if (pass == 0)
+        hipLaunchKernelGGL((range_kernel<F, QW, 0, ARITH_FALLBACK>), grid, dim3(256), 0, st, a);
+      else
+        hipLaunchKernelGGL((range_kernel<F, QW, 1, ARITH_FALLBACK>), grid, dim3(256), 0, st, a);
+    }
+    return hipGetLastError();
+  }
   dim3 grid((unsigned)((a.nq + QW - 1) / QW), (unsigned)nsplit);
   if (pass == 0)
-    hipLaunchKernelGGL((range_kernel<F, QW, 0>), grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL((range_kernel<F, QW, 0, ARITH_AVX>), grid, dim3(256), 0, st, a);
   else
-    hipLaunchKernelGGL((range_kernel<F, QW, 1>), grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL((range_kernel<F, QW, 1, ARITH_AVX>), grid, dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
 template <typename F>
 hipError_t launch_pairwise(const PairwiseArgs<F>& a, int form, int nsplit, hipStream_t st) {
   constexpr int QW = 16;
+  constexpr int QW8 = 8;
+  if (a.arith == ARITH_FALLBACK) {
+    if constexpr (sizeof(F) == 8) {
+      dim3 grid((unsigned)((a.n + QW8 - 1) / QW8), (unsigned)nsplit);
+      if (form == 0)
+        hipLaunchKernelGGL((pairwise_kernel<F, QW8, 0, ARITH_FALLBACK>), grid, dim3(256), 0, st, a);
+      else
+        hipLaunchKernelGGL((pairwise_kernel<F, QW8, 1, ARITH_FALLBACK>), grid, dim3(256), 0, st, a);
+    } else {
+      dim3 grid((unsigned)((a.n + QW - 1) / QW), (unsigned)nsplit);
+      if (form == 0)
+        hipLaunchKernelGGL((pairwise_kernel<F, QW, 0, ARITH_FALLBACK>), grid, dim3(256), 0, st, a);
+      else
+        hipLaunchKernelGGL((pairwise_kernel<F, QW, 1, ARITH_FALLBACK>), grid, dim3(256), 0, st, a);
+    }
+    return hipGetLastError();
+  }
   dim3 grid((unsigned)((a.n + QW - 1) / QW), (unsigned)nsplit);
   if (form == 0)
-    hipLaunchKernelGGL((pairwise_kernel<F, QW, 0>), grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL((pairwise_kernel<F, QW, 0, ARITH_AVX>), grid, dim3(256), 0, st, a);
   else
-    hipLaunchKernelGGL((pairwise_kernel<F, QW, 1>), grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL((pairwise_kernel<F, QW, 1, ARITH_AVX>), grid, dim3(256), 0, st, a);
   return hipGetLastError();
 }
 

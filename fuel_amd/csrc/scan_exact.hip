@@ -16,7 +16,7 @@
 
 namespace fa {
 
-template <typename F, int QW, int V>
+template <typename F, int QW, int V, int ARITH>
 __global__ __launch_bounds__(256) void scan_exact_kernel(ScanExactArgs<F> a) {
   using C = Comp<F>;
   using T = typename C::T;
@@ -27,6 +27,7 @@ __global__ __launch_bounds__(256) void scan_exact_kernel(ScanExactArgs<F> a) {
   __shared__ F thr_s[QW];
   __shared__ int bcnt[QW];
   __shared__ int lcnt[QW];
+  __shared__ T lo_s[QW];
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
@@ -47,7 +48,9 @@ __global__ __launch_bounds__(256) void scan_exact_kernel(ScanExactArgs<F> a) {
     thr_s[tid] = F(INFINITY);
     bcnt[tid] = 0;
     lcnt[tid] = 0;
+    lo_s[tid] = (a.lo && tid < nqb) ? a.lo[qs0 + tid] : T(0);
   }
+  const bool has_lo = a.lo != nullptr;
   auto qrow = [&](int qq) -> const F* {
     const int64_t r = a.qrows ? a.qrows[qs0 + qq] : qs0 + qq;
     return a.Q + r * a.ldq;
@@ -60,14 +63,14 @@ __global__ __launch_bounds__(256) void scan_exact_kernel(ScanExactArgs<F> a) {
     const int64_t row = rb + tid;
     const bool valid = row < r1;
     F s[QW];
-    exact_step<F, QW, DC>(qbuf, qrow, nqb, single, a.X + (valid ? row : r0) * a.ldx, d, s);
+    exact_step<F, QW, DC, ARITH>(qbuf, qrow, nqb, single, a.X + (valid ? row : r0) * a.ldx, d, s);
 
 #pragma unroll
     for (int qq = 0; qq < QW; ++qq) {
       if (qq < nqb && valid && s[qq] <= thr_s[qq]) {
         const F key = a.metric == 0 ? sqrt_rn(s[qq]) : s[qq];
         const T c = C::make(key, (uint32_t)row);
-        if (c < thr[qq]) {
+        if (c < thr[qq] && (!has_lo || c > lo_s[qq])) {
           const int pos = atomicAdd(&bcnt[qq], 1);
           slotbase[size_t(qq) * NS + KP + pos] = c;
         }
@@ -138,20 +141,91 @@ __global__ __launch_bounds__(64) void finalize_exact_kernel(FinalizeArgs<F> a) {
   }
   wave_bitonic_sort<T, V>(x);
   const int count = fill < KK ? fill : KK;
+  if (a.raw_out) {
+    T* row = a.raw_out + qs * a.raw_stride + a.raw_off;
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const int e = v * 64 + lane;
+      if (e < KK) row[e] = e < count ? x[v] : C::MAX;
+    }
+    const T last = count == KK ? wave_get<T, V>(x, KK - 1) : C::MAX;
+    if (lane == 0) {
+      a.lo_out[qs] = last;
+      a.cnt_out[qs] += count;
+    }
+    return;
+  }
   const int64_t qr = a.qrows ? a.qrows[qs] : qs;
   write_knn_row<F, V>(x, count, a.qbase + qr, a.k, a.mode, a.out_idx + qr * a.k, a.out_dist + qr * a.k);
 }
 
+// One thread per query slot: the concatenated rounds are sorted, so the row is
+// written in one pass (self position, the k-th key for EXCL_PY_LINEAR, then
+// every kept entry up to k, then -1 / +inf padding).
+template <typename F>
+__global__ __launch_bounds__(64) void write_long_rows_kernel(LongRowArgs<F> a) {
+  using C = Comp<F>;
+  const int64_t qs = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (qs >= a.nq) return;
+  const typename C::T* row = a.raw + qs * a.stride;
+  const int cnt = a.cnt[qs];
+  const int64_t qr = a.qrows ? a.qrows[qs] : qs;
+  const int64_t self = a.qbase + qr;
+  int64_t* oidx = a.out_idx + qr * a.k;
+  F* odist = a.out_dist + qr * a.k;
+  int sp = -1;
+  if (a.mode != EXCL_NONE)
+    for (int e = 0; e < cnt; ++e)
+      if ((int64_t)C::idx(row[e]) == self) {
+        sp = e;
+        break;
+      }
+  const F kth = (a.mode == EXCL_PY_LINEAR && a.k >= 1 && cnt >= a.k) ? C::key(row[a.k - 1]) : F(INFINITY);
+  int j = 0;
+  for (int e = 0; e < cnt && j < a.k; ++e) {
+    if (e == sp) continue;
+    const F key = C::key(row[e]);
+    if (a.mode == EXCL_PY_LINEAR && key > kth) break;
+    oidx[j] = (int64_t)C::idx(row[e]);
+    odist[j] = key;
+    ++j;
+  }
+  for (; j < a.k; ++j) {
+    oidx[j] = -1;
+    odist[j] = F(INFINITY);
+  }
+}
+
+template <typename F>
+hipError_t launch_write_long_rows(const LongRowArgs<F>& a, hipStream_t st) {
+  if (a.nq == 0) return hipSuccess;
+  hipLaunchKernelGGL((write_long_rows_kernel<F>), dim3((unsigned)((a.nq + 63) / 64)), dim3(64), 0, st, a);
+  return hipGetLastError();
+}
+template hipError_t launch_write_long_rows<float>(const LongRowArgs<float>&, hipStream_t);
+template hipError_t launch_write_long_rows<double>(const LongRowArgs<double>&, hipStream_t);
+
 template <typename F>
 hipError_t launch_scan_exact(const ScanExactArgs<F>& a, int qw, int nsplit, hipStream_t st) {
   dim3 block(256);
+  // the f64 fallback form keeps 8 double lanes per query: at most 8 queries per
+  // workgroup keep its accumulators in registers
+  if (a.arith == ARITH_FALLBACK && sizeof(F) == 8 && qw > 8) qw = 8;
   dim3 grid((unsigned)((a.nq + qw - 1) / qw), (unsigned)nsplit);
-  if (qw == 16)
-    hipLaunchKernelGGL((scan_exact_kernel<F, 16, 8>), grid, block, 0, st, a);
-  else if (qw == 8)
-    hipLaunchKernelGGL((scan_exact_kernel<F, 8, 8>), grid, block, 0, st, a);
-  else
-    hipLaunchKernelGGL((scan_exact_kernel<F, 4, 8>), grid, block, 0, st, a);
+  if (a.arith == ARITH_FALLBACK) {
+    if (qw >= 16)
+      hipLaunchKernelGGL((scan_exact_kernel<F, 16, 8, ARITH_FALLBACK>), grid, block, 0, st, a);
+    else if (qw == 8)
+      hipLaunchKernelGGL((scan_exact_kernel<F, 8, 8, ARITH_FALLBACK>), grid, block, 0, st, a);
+    else
+      hipLaunchKernelGGL((scan_exact_kernel<F, 4, 8, ARITH_FALLBACK>), grid, block, 0, st, a);
+  } else if (qw == 16) {
+    hipLaunchKernelGGL((scan_exact_kernel<F, 16, 8, ARITH_AVX>), grid, block, 0, st, a);
+  } else if (qw == 8) {
+    hipLaunchKernelGGL((scan_exact_kernel<F, 8, 8, ARITH_AVX>), grid, block, 0, st, a);
+  } else {
+    hipLaunchKernelGGL((scan_exact_kernel<F, 4, 8, ARITH_AVX>), grid, block, 0, st, a);
+  }
   return hipGetLastError();
 }
 

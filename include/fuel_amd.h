@@ -71,11 +71,22 @@ extern "C" {
 
 typedef struct fuel_amd_index fuel_amd_index;
 
+/* Index option flags.
+ * FUEL_AMD_FLAG_FALLBACK_ARITH: distances take the rounding of fuel's
+ * squared_euclidean_distance_fallback (src/distance/squared_euclidean.rs:99-133),
+ * which fuel runs on every host without AVX+FMA (the dispatch at :71-97): 8
+ * lanes of fl(t * t) + acc for f32 and f64, a left fold from +0, an unfused
+ * tail.  Without the flag the engine matches the AVX+FMA kernels (:13-67), the
+ * ones fuel runs on x86 hosts with AVX2 and FMA.  Results are then identical to
+ * fuel on such a host; every mode and entry point of the index follows the flag
+ * (the batched problems of fuel_amd_batch_* always use the AVX form). */
+#define FUEL_AMD_FLAG_FALLBACK_ARITH 1
+
 typedef struct fuel_amd_opts {
   int32_t mode;   /* FUEL_AMD_MODE_* */
   int32_t widen;  /* candidate widening factor for the MFMA modes (default 4) */
   int32_t device; /* HIP device ordinal (default 0) */
-  int32_t flags;  /* reserved, 0 */
+  int32_t flags;  /* FUEL_AMD_FLAG_* (0: none) */
   int64_t reserved[6];
 } fuel_amd_opts;
 
@@ -169,6 +180,33 @@ void fuel_amd_csr_free(fuel_amd_csr* csr);
  * subset_with_ties (:60-67) then cuts per request. */
 int32_t fuel_amd_knn_ties(fuel_amd_index* index, const void* query, int64_t m, int32_t k, fuel_amd_csr* out);
 
+/* ---- priority searchers ------------------------------------------------------
+ * fuel's LinearScanPrioritySearcher (src/search/linear_scan.rs:70-235, the
+ * PrioritySearcher trait of src/api/search.rs:216-229) for m queries at once:
+ * slot q is the searcher of query row q (index dtype, d per row).  Each slot
+ * keeps the reference's state -- cutoff (initially +inf), skip (initially 0),
+ * an initialized flag and a position -- and on its first next() the list of
+ * every row with skip <= distance <= cutoff sorted by (distance, index), which
+ * the engine computes on the GPU for a group of slots at once.  The answers
+ * equal the reference searcher's call for call.  Calls on one searcher are
+ * serialized; the index must outlive it. */
+typedef struct fuel_amd_priority fuel_amd_priority;
+int32_t fuel_amd_priority_create(fuel_amd_index* index, const void* query, int64_t m, fuel_amd_priority** out);
+int32_t fuel_amd_priority_destroy(fuel_amd_priority* searcher);
+/* reset_with_limits(cutoff, skip) (:200-209); reset() is cutoff = +inf, skip = 0.
+ * Limits are cast to the index dtype; 0 <= skip <= cutoff is required. */
+int32_t fuel_amd_priority_reset(fuel_amd_priority* searcher, int64_t q, double cutoff, double skip);
+/* decrease_cutoff (:189-194); a larger threshold is FUEL_AMD_ERR_INVALID. */
+int32_t fuel_amd_priority_decrease_cutoff(fuel_amd_priority* searcher, int64_t q, double threshold);
+/* next / next_with_filter (:132-155): the next (index, distance) of slot q, or
+ * index -1 (distance +inf) when none is left.  skip_point (may be NULL) is the
+ * SearchFilter: a nonzero return skips that row. */
+int32_t fuel_amd_priority_next(fuel_amd_priority* searcher, int64_t q, int32_t (*skip_point)(void* ctx, int64_t index),
+                               void* ctx, int64_t* out_index, void* out_distance);
+/* all_lower_bound (:157-175): 0 before the first next(), else the distance of
+ * the next candidate within the limits (+inf when none). */
+int32_t fuel_amd_priority_all_lower_bound(fuel_amd_priority* searcher, int64_t q, void* out_distance);
+
 /* DBSCAN over the index rows with the exact range search (eps inclusive, cast
  * to the index dtype like fuel, python/dbscan.rs:45).  variant 0 = dbscan
  * (src/cluster/dbscan.rs:14-83), 1 = parallel_dbscan
@@ -209,6 +247,10 @@ int32_t fuel_amd_pairwise_sqdist_device(const void* d_points1, int64_t nrows, in
  * src/api/condensed_distance_matrix.rs:22-54). */
 int32_t fuel_amd_pairwise(const void* data, int64_t n, int32_t d, int32_t dtype, int32_t metric,
                           int32_t form, void* out);
+/* Same over the rows of an index (its metric and option flags), e.g. a resident
+ * dataset or one built with FUEL_AMD_FLAG_FALLBACK_ARITH.  Synchronous; out is
+ * a caller-allocated host array of n x n or n(n-1)/2 entries. */
+int32_t fuel_amd_index_pairwise(fuel_amd_index* index, int32_t form, void* out);
 
 /* Local outlier factor over the self-kNN of the index rows (f32 or f64 scores,
  * n entries).  Replaces local_outlier_factor over a LinearScanSearcher

@@ -273,3 +273,63 @@ def kmeans_pairwise(A, B, threads=None):
     getattr(lib, f"oracle_kmeans_pairwise_{_tag(dt)}")(A.ctypes.data, A.shape[0], B.ctypes.data, B.shape[0],
                                                        A.shape[1], threads or default_threads(), out.ctypes.data)
     return out
+
+
+_SKIP_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int64)
+
+
+class PrioritySearcher:
+    """LinearScanPrioritySearcher (src/search/linear_scan.rs:70-235) of one
+    query over X, restated in oracle/fuel_oracle.cpp."""
+
+    def __init__(self, X, q, metric="euclidean", arith="avx"):
+        lib = _load()
+        dt = np.float64 if np.asarray(X).dtype == np.float64 else np.float32
+        self._t = _tag(dt)
+        self._dt = dt
+        self._X = _c(X, dt)  # kept alive: the searcher reads it
+        self._q = _c(q, dt)
+        cty = ctypes.c_float if dt == np.float32 else ctypes.c_double
+        P = ctypes.c_void_p
+        f = getattr(lib, f"oracle_priority_create_{self._t}")
+        f.restype = P
+        f.argtypes = [P, ctypes.c_int64, ctypes.c_int64, P, ctypes.c_int]
+        for name, res, args in [("destroy", None, [P]), ("reset", None, [P, ctypes.c_double, ctypes.c_double]),
+                                ("decrease_cutoff", None, [P, ctypes.c_double]),
+                                ("next", ctypes.c_int, [P, _SKIP_FN, P, ctypes.POINTER(ctypes.c_int64),
+                                                        ctypes.POINTER(cty)]),
+                                ("all_lower_bound", cty, [P])]:
+            g = getattr(lib, f"oracle_priority_{name}_{self._t}")
+            g.restype = res
+            g.argtypes = args
+        self._cty = cty
+        self._h = f(self._X.ctypes.data, self._X.shape[0], self._X.shape[1], self._q.ctypes.data,
+                    _metric(metric, arith))
+
+    def __del__(self):
+        try:
+            getattr(_load(), f"oracle_priority_destroy_{self._t}")(self._h)
+        except Exception:
+            pass
+
+    def reset(self):
+        getattr(_load(), f"oracle_priority_reset_{self._t}")(self._h, float("inf"), 0.0)
+
+    def reset_with_limits(self, cutoff, skip):
+        getattr(_load(), f"oracle_priority_reset_{self._t}")(self._h, float(cutoff), float(skip))
+
+    def decrease_cutoff(self, t):
+        getattr(_load(), f"oracle_priority_decrease_cutoff_{self._t}")(self._h, float(t))
+
+    def next(self, skip_point=None):
+        idx = ctypes.c_int64()
+        dist = self._cty()
+        cb = ctypes.cast(None, _SKIP_FN) if skip_point is None else _SKIP_FN(lambda _c, i: 1 if skip_point(int(i)) else 0)
+        found = getattr(_load(), f"oracle_priority_next_{self._t}")(self._h, cb, None, ctypes.byref(idx),
+                                                                    ctypes.byref(dist))
+        if not found:
+            return None
+        return int(idx.value), np.array(dist.value, dtype=self._dt)[()]
+
+    def all_lower_bound(self):
+        return np.array(getattr(_load(), f"oracle_priority_all_lower_bound_{self._t}")(self._h), dtype=self._dt)[()]

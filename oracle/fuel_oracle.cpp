@@ -789,6 +789,68 @@ void extract_xi(const int64_t* ordering, const F* reach, const int64_t* pred, in
   }
 }
 
+// a10: LinearScanPrioritySearcher, src/search/linear_scan.rs:70-235: one
+// query's searcher.  ensure_initialized (:105-130) materialises every point with
+// skip <= d <= cutoff, sorted by DistPair order; find_next (:132-155),
+// all_lower_bound_internal (:157-175), decrease_cutoff (:177-180) and
+// reset_with_limits (:200-209) follow the reference line by line.
+template <typename F>
+struct PrioritySearcher {
+  const F* X;
+  int64_t n, d;
+  std::vector<F> q;
+  int metric;
+  std::vector<DistPair<F>> candidates;
+  size_t position = 0;
+  F cutoff = (F)INFINITY;
+  F skip = 0;
+  bool initialized = false;
+
+  void reset_state() {
+    position = 0;
+    cutoff = (F)INFINITY;
+    skip = 0;
+    initialized = false;
+  }
+  void ensure_initialized() {
+    if (initialized) return;
+    std::vector<DistPair<F>> c;
+    c.reserve(n);
+    for (int64_t i = 0; i < n; ++i) {
+      const F dd = distance(q.data(), X + i * d, d, metric);
+      if (dd < skip || dd > cutoff) continue;
+      c.push_back({dd, i});
+    }
+    std::sort(c.begin(), c.end(), PairLess<F>());
+    candidates.swap(c);
+    position = 0;
+    initialized = true;
+  }
+  template <typename Filter>
+  bool find_next(Filter skip_point, DistPair<F>& out) {
+    ensure_initialized();
+    while (position < candidates.size()) {
+      const DistPair<F> cand = candidates[position];
+      position += 1;
+      if (cand.distance > cutoff) return false;
+      if (cand.distance < skip) continue;
+      if (skip_point(cand.index)) continue;
+      out = cand;
+      return true;
+    }
+    return false;
+  }
+  F all_lower_bound() const {
+    if (!initialized) return 0;
+    for (size_t i = position; i < candidates.size(); ++i) {
+      const F dd = candidates[i].distance;
+      if (dd > cutoff) break;
+      if (dd >= skip) return dd;
+    }
+    return (F)INFINITY;
+  }
+};
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -943,5 +1005,38 @@ void oracle_extract_xi_f64(const int64_t* ordering, const double* reach, const i
                            double xi, int64_t min_points, int64_t* labels) {
   extract_xi<double>(ordering, reach, pred, n, xi, min_points, labels);
 }
+
+
+typedef int (*oracle_skip_fn)(void* ctx, int64_t index);
+#define ORACLE_PRIORITY(T, F)                                                                                \
+  void* oracle_priority_create_##T(const F* X, int64_t n, int64_t d, const F* q, int metric) {            \
+    auto* s = new PrioritySearcher<F>();                                                                   \
+    s->X = X;                                                                                              \
+    s->n = n;                                                                                              \
+    s->d = d;                                                                                              \
+    s->q.assign(q, q + d);                                                                                 \
+    s->metric = metric;                                                                                    \
+    return s;                                                                                              \
+  }                                                                                                        \
+  void oracle_priority_destroy_##T(void* h) { delete static_cast<PrioritySearcher<F>*>(h); }               \
+  void oracle_priority_reset_##T(void* h, double cutoff, double skip) {                                    \
+    auto* s = static_cast<PrioritySearcher<F>*>(h);                                                        \
+    s->reset_state();                                                                                      \
+    s->cutoff = (F)cutoff;                                                                                 \
+    s->skip = (F)skip;                                                                                     \
+  }                                                                                                        \
+  void oracle_priority_decrease_cutoff_##T(void* h, double t) { static_cast<PrioritySearcher<F>*>(h)->cutoff = (F)t; } \
+  int oracle_priority_next_##T(void* h, oracle_skip_fn fn, void* ctx, int64_t* idx, F* dist) {             \
+    DistPair<F> out{};                                                                                     \
+    const bool found = static_cast<PrioritySearcher<F>*>(h)->find_next(                                   \
+        [&](int64_t i) { return fn ? fn(ctx, i) != 0 : false; }, out);                                     \
+    *idx = found ? out.index : -1;                                                                         \
+    *dist = found ? out.distance : (F)INFINITY;                                                            \
+    return found ? 1 : 0;                                                                                  \
+  }                                                                                                        \
+  F oracle_priority_all_lower_bound_##T(void* h) { return static_cast<PrioritySearcher<F>*>(h)->all_lower_bound(); }
+ORACLE_PRIORITY(f32, float)
+ORACLE_PRIORITY(f64, double)
+#undef ORACLE_PRIORITY
 
 }  // extern "C"

@@ -353,3 +353,108 @@ def test_kmeans_pairwise_rounding_order(oracle, d):
     if d >= 8:  # the search kernel's 8-lane order differs on these inputs
         lanes = np.array([[oracle.sqdist(A[i], B[j]) for j in range(5)] for i in range(6)], np.float32)
         assert np.any(lanes.view(np.uint32) != got.view(np.uint32))
+
+
+# ---- LinearScanPrioritySearcher (src/search/linear_scan.rs:70-235) ----
+# The reference's priority-searcher tests (src/search/vptree/tests/priority.rs)
+# exercise the PrioritySearcher trait that the linear scan implements too; they
+# are transcribed here for the linear-scan restatement (increase_skip, a VP-tree
+# method, is reset_with_limits with that skip).
+
+def _all(searcher, skip_point=None):
+    out = []
+    while True:
+        r = searcher.next(skip_point)
+        if r is None:
+            return out
+        out.append(r)
+
+
+def test_priority_first_three(oracle):
+    # priority.rs:72-97: the query's own point first, then points 1 and 2
+    pts = np.array([[0, 0], [1, 0], [0, 1], [1, 1], [2, 2]], np.float64)
+    s = oracle.PrioritySearcher(pts, pts[0])
+    got = [s.next() for _ in range(3)]
+    assert [g[0] for g in got] == [0, 1, 2]
+    assert [float(g[1]) for g in got] == [0.0, 1.0, 1.0]
+
+
+def test_priority_reuse_after_reset(oracle):
+    # priority.rs:99-137
+    pts = np.array([[0, 0], [1, 0], [0, 1], [1, 1], [2, 2], [3, 0.5], [-1, 1.5]], np.float64)
+    reusable = oracle.PrioritySearcher(pts, pts[0])
+    reusable.decrease_cutoff(2.5)
+    first = _all(reusable)
+    fresh = oracle.PrioritySearcher(pts, pts[0])
+    fresh.decrease_cutoff(2.5)
+    assert first == _all(fresh)
+    assert [i for i, _ in first] == [0, 1, 2, 3, 6]  # distances 0, 1, 1, sqrt 2, sqrt 3.25
+    reusable.reset()
+    reusable.reset_with_limits(3.0, 0.5)
+    second = _all(reusable)
+    fresh2 = oracle.PrioritySearcher(pts, pts[0])
+    fresh2.reset_with_limits(3.0, 0.5)
+    assert second == _all(fresh2)
+    assert all(0.5 <= d <= 3.0 for _, d in second) and 0 not in [i for i, _ in second]
+
+
+def test_priority_cutoff_and_skip(oracle):
+    # priority.rs:139-175: 5 x 5 grid, query point 12
+    pts = np.array([[x, y] for y in range(5) for x in range(5)], np.float64)
+    s = oracle.PrioritySearcher(pts, pts[12])
+    s.decrease_cutoff(1.5)
+    res = _all(s)
+    assert res and all(d <= 1.5 + 1e-12 for _, d in res)
+    assert [i for i, _ in res] == [12, 7, 11, 13, 17, 6, 8, 16, 18]  # 0, 1 (4x), sqrt 2 (4x)
+    s2 = oracle.PrioritySearcher(pts, pts[12])
+    s2.reset_with_limits(2.0, 1.0 + 1e-12)
+    res2 = _all(s2)
+    assert res2 and all(1.0 + 1e-12 <= d <= 2.0 + 1e-12 for _, d in res2)
+    assert 12 not in [i for i, _ in res2]
+    assert [i for i, _ in res2] == [6, 8, 16, 18, 2, 10, 14, 22]  # sqrt 2 (4x), then 2 (4x)
+
+
+def test_priority_matches_knn_and_range(oracle):
+    # priority.rs:177-215: 10 x 10 grid, query 45, k 10
+    pts = np.array([[x, y] for y in range(10) for x in range(10)], np.float64)
+    offs, idx, dist = oracle.knn_ties(pts, pts[45:46], 10)
+    radius = float(dist[-1])
+    s = oracle.PrioritySearcher(pts, pts[45])
+    s.decrease_cutoff(radius)
+    pr = _all(s)
+    ro, ri, rd = oracle.range_search(pts, pts[45:46], radius)
+    assert [i for i, _ in pr] == ri.tolist() == idx.tolist()
+    assert len(pr) >= 10
+
+
+def test_priority_external_query_sorted(oracle):
+    # priority.rs:217-247: every point, sorted by (distance, index)
+    pts = np.array([[0, 0], [1, 0], [0, 1], [1, 1], [2, 2], [3, 1]], np.float64)
+    q = np.array([0.5, 0.25])
+    got = _all(oracle.PrioritySearcher(pts, q))
+    exp = sorted((float(np.sqrt(((p - q) ** 2).sum())), i) for i, p in enumerate(pts))
+    assert [i for i, _ in got] == [i for _, i in exp]
+    assert np.allclose([d for _, d in got], [d for d, _ in exp], atol=1e-10)
+
+
+def test_priority_lower_bound_filter_and_brute_force(oracle):
+    rng = np.random.default_rng(12)
+    X = np.round(rng.random((300, 3)) * 5).astype(np.float32)  # many ties
+    q = X[7]
+    s = oracle.PrioritySearcher(X, q, "sqeuclidean")
+    assert float(s.all_lower_bound()) == 0.0  # not initialised yet (linear_scan.rs:158-160)
+    first = s.next()
+    lb = s.all_lower_bound()
+    second = s.next()
+    assert lb == second[1]
+    rest = _all(s)
+    full = [first, second] + rest
+    d = oracle.pairwise(np.concatenate([q[None], X]), "sqeuclidean")[0, 1:]
+    exp = sorted((float(d[i]), i) for i in range(300))
+    assert [(float(dd), i) for i, dd in full] == exp
+    assert np.isinf(s.all_lower_bound())
+    # a filter skips points without ending the search; limits bound it
+    s2 = oracle.PrioritySearcher(X, q, "sqeuclidean")
+    s2.reset_with_limits(4.0, 1.0)
+    got = _all(s2, lambda i: i % 3 == 0)
+    assert [(float(dd), i) for i, dd in got] == [(v, i) for v, i in exp if 1.0 <= v <= 4.0 and i % 3]
