@@ -44,7 +44,17 @@ def parse():
     p.add_argument("--d", type=int, default=128)
     p.add_argument("--k", type=int, default=10)
     p.add_argument("--widen", type=int, default=4, help="candidates kept per query = widen * k (<= 128)")
-    p.add_argument("--verify", type=int, default=64, help="queries checked against the oracle")
+    p.add_argument("--verify", type=int, default=64, help="random queries checked against the oracle")
+    p.add_argument("--verify-per-block", type=int, default=6,
+                   help="queries checked from every block of 256 queries (beside every query whose first "
+                        "certificate failed)")
+    p.add_argument("--c3-full", type=int, default=1,
+                   help="headline line: GPU LOF of the full 70,000 x 784 workload against the oracle (0: skip)")
+    p.add_argument("--c4-line", type=int, default=1,
+                   help="headline line: the 512 batched problems (plan, passes, parity on the smallest third)")
+    p.add_argument("--c4-steps", type=int, default=3, help="timed passes of the batched problems")
+    p.add_argument("--calibrate", type=int, default=1,
+                   help="headline line: achievable bf16 rate of a large torch matmul beside the nominal peak")
     p.add_argument("--cpu-sample", type=int, default=1000, help="queries timed on the CPU baseline (0: skip)")
     p.add_argument("--fp32-steps", type=int, default=2,
                    help="timed steps of the plain FP32 MFMA form run beside the mixed one (0: skip)")
@@ -62,6 +72,8 @@ def parse():
                    help="seconds of serving at 10,000 queries/s measured for the headline line's p99 (0: skip)")
     p.add_argument("--serve-rows", type=int, default=10_000_000, help="rows of the served index (c5)")
     p.add_argument("--serve-workers", type=int, default=2, help="serving worker threads (c5 and the headline p99)")
+    p.add_argument("--serve-verify", type=int, default=240,
+                   help="served queries of the headline serving run checked against the oracle")
     p.add_argument("--host-path", action="store_true",
                    help="only time the host-pointer entry point (queries and results cross PCIe each call)")
     p.add_argument("--copy-probe", type=int, default=0,
@@ -132,17 +144,36 @@ def run_mode(_lib, Xd, Qd, n, m, d, k, mode, widen, steps, warmup, stream, out_i
     kernel_ms, launches, kname = ix.timing_read()
     ix.timing_enable(False)
     res = {"ms_per_step": (t1 - t0) / steps * 1e3, "qps": m * steps / (t1 - t0),
-           "kernel_s": (kernel_ms / max(launches, 1)) / 1e3, "kernel": kname, "stats": ix.stats()}
+           "kernel_s": (kernel_ms / max(launches, 1)) / 1e3, "kernel": kname, "stats": ix.stats(),
+           "fail_rows": ix.fallback_rows()}
     ix.close()
     return res
 
 
-def verify(oracle, X, Q, gi, gd, k, nverify):
-    """Sample of queries checked against the oracle: identical indices and f32
-    bits, recall, and the largest relative error against an f64 oracle."""
+def stratified_sample(m, per_block, block=256, seed=0):
+    """per_block random queries from every block of `block` consecutive query
+    slots (the scan's query blocks), so that every workgroup's queries are seen."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for b0 in range(0, m, block):
+        b1 = min(m, b0 + block)
+        out.append(rng.choice(np.arange(b0, b1), size=min(per_block, b1 - b0), replace=False))
+    return np.sort(np.concatenate(out)) if out else np.zeros(0, np.int64)
+
+
+def verify(oracle, X, Q, gi, gd, k, nverify, extra=None, per_block=0):
+    """Queries checked against the oracle: identical indices and f32 bits,
+    recall, and the largest relative error against an f64 oracle.  The set is
+    a random sample of nverify queries, per_block queries of every 256-query
+    block, and `extra` (e.g. every query whose first certificate failed)."""
     m = Q.shape[0]
     rng = np.random.default_rng(0)
-    sample = np.sort(rng.choice(m, size=min(nverify, m), replace=False))
+    parts = [rng.choice(m, size=min(nverify, m), replace=False)]
+    if per_block:
+        parts.append(stratified_sample(m, per_block))
+    if extra is not None and len(extra):
+        parts.append(np.asarray(extra, np.int64))
+    sample = np.unique(np.concatenate(parts))
     oi, od = oracle.knn(X, Q[sample], k)
     exact_rows = int(np.sum(np.all(gi[sample] == oi, axis=1) & np.all(gd[sample] == od, axis=1)))
     recall = float(np.mean([len(set(gi[s]) & set(oi[j])) / k for j, s in enumerate(sample)]))
@@ -150,8 +181,17 @@ def verify(oracle, X, Q, gi, gd, k, nverify):
     xi = gi[sample].ravel()
     ref64 = np.sqrt(oracle.sqdist_fp64(X, Q, qi, xi))
     max_rel_err = float(np.max(np.abs(gd[sample].ravel().astype(np.float64) - ref64) / np.maximum(ref64, 1e-300)))
-    return {"queries": int(len(sample)), "exact_rows": exact_rows, "recall": recall,
-            "max_rel_err_vs_fp64": max_rel_err}
+    out = {"queries": int(len(sample)), "exact_rows": exact_rows, "recall": recall,
+           "max_rel_err_vs_fp64": max_rel_err}
+    if extra is not None:
+        ex = np.asarray(extra, np.int64)
+        pos = np.searchsorted(sample, ex)
+        ok = np.all(gi[ex] == oi[pos], axis=1) & np.all(gd[ex] == od[pos], axis=1) if len(ex) else np.zeros(0, bool)
+        out["second_pass_queries"] = int(len(ex))
+        out["second_pass_exact_rows"] = int(ok.sum())
+    if per_block:
+        out["query_blocks_covered"] = int(len(np.unique(sample // 256)))
+    return out
 
 
 VALU_F32_PEAK = 256 * 4 * 16 * 2 * 2.4e9  # FMA lanes/s x 2 flop, no packed FP32 (78.6 TFLOP/s)
@@ -167,11 +207,12 @@ def timed(fn, reps):
     return float(np.median(ts))
 
 
-def serve_run(srv, pool, lam, seconds, rng, k=10, mean_batch=64, cap=4096, keep=0):
+def serve_run(srv, pool, lam, seconds, rng, k=10, mean_batch=64, cap=4096, keep=0, keep_queries=0):
     """Open-loop arrivals: requests arrive as a Poisson process of rate
     lam / mean_batch per second with geometric sizes (mean mean_batch, capped),
     so queries arrive at lam per second on average.  Returns the server's
-    latency statistics over the run and the first `keep` answers."""
+    latency statistics over the run and the first answers: `keep` requests,
+    or as many as hold `keep_queries` queries."""
     import queue
     import threading
     srv.stats(reset=True)
@@ -185,7 +226,7 @@ def serve_run(srv, pool, lam, seconds, rng, k=10, mean_batch=64, cap=4096, keep=
                 return
             t, off, m = item
             idx, dist, _, _ = srv.wait(t)
-            if len(kept) < keep:
+            if len(kept) < keep or sum(x[1] for x in kept) < keep_queries:
                 kept.append((off, m, idx, dist))
 
     th = threading.Thread(target=waiter)
@@ -230,17 +271,135 @@ def serve_headline(args):
     """p50/p99 request latency of the serving loop on the serving configuration
     at one arrival rate (10,000 queries/s), for the headline line."""
     from fuel_amd import Server, _lib
+    import oracle
     X, pool, rng = c5_data(args)
     ix = _lib.Index(X, "euclidean", mode=args.serve_mode)
     with Server(ix, workers=args.serve_workers, max_batch=4096) as srv:
         serve_run(srv, pool, 2000, 0.5, rng)  # warm-up
-        r, _ = serve_run(srv, pool, 10000, args.serve_p99_seconds, rng)
+        r, kept = serve_run(srv, pool, 10000, args.serve_p99_seconds, rng, keep_queries=args.serve_verify)
     ix.close()
     r = dict(r)
+    # served answers against the oracle, every row of the first requests
+    t0 = time.perf_counter()
+    same, nchk, nreq = True, 0, 0
+    for off, m, idx, dist in kept:
+        oi, od = oracle.knn(X, pool[off:off + m], 10, "euclidean", "none", threads=oracle.default_threads())
+        same = same and bool(np.array_equal(oi, idx) and od.tobytes() == dist.tobytes())
+        nchk += m
+        nreq += 1
+    print(f"# serve: {nchk} served queries of {nreq} requests checked against the oracle in "
+          f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     r.update({"workload": f"serving: resident index {X.shape[0]:,} x {X.shape[1]} f32 (50-component mixture), "
                           "k=10, Poisson arrivals, geometric batch sizes (mean 64, cap 4096)",
-              "mode": args.serve_mode, "seconds": args.serve_p99_seconds})
+              "mode": args.serve_mode, "seconds": args.serve_p99_seconds,
+              "verify": {"requests": nreq, "queries": nchk, "identical": same}})
     return r
+
+
+def c3_full(threads):
+    """LOF of the full 70,000 x 784 workload (k=20) on the GPU against the
+    oracle's restatement of fuel's for_each_knn + LOF, bit for bit."""
+    from fuel_amd import _lib, workloads
+    import oracle
+    t0 = time.time()
+    X = workloads.c3()
+    ix = _lib.Index(X, "euclidean")
+    g = ix.lof(20)
+    t_g = timed(lambda: ix.lof(20), 2)
+    st = ix.stats()
+    ix.close()
+    print(f"# c3: GPU LOF in {t_g * 1e3:.1f} ms; oracle on the full problem ...", file=sys.stderr, flush=True)
+    t1 = time.perf_counter()
+    o = oracle.lof(X, 20, threads=threads)
+    t_c = time.perf_counter() - t1
+    print(f"# c3: oracle LOF in {t_c:.1f}s ({time.time() - t0:.0f}s with data)", file=sys.stderr, flush=True)
+    return {"workload": "LOF n=70000 d=784 integers 0..255 (80% zero) k=20", "gpu_ms": round(t_g * 1e3, 3),
+            "knn_stats": {k: st[k] for k in ("mode_used", "fallback_queries", "exact_queries")},
+            "identical": bool(g.tobytes() == o.tobytes()),
+            "cpu_baseline": {"value": round(t_c * 1e3, 1), "unit": "ms", "cores": threads, "kind": "port",
+                             "sample": "the full problem (oracle for_each_knn + LOF)"}}
+
+
+def c4_run(args, threads, steps, warmup):
+    """The 512 batched uneven self-kNN problems: plan (autotuned), graph
+    replays timed, answers of the smallest third checked against the oracle and
+    a hash of every problem's answer."""
+    import hashlib
+
+    import torch
+
+    from fuel_amd import BatchKnn, workloads
+    import oracle
+    t0 = time.time()
+    probs = workloads.c4()
+    print(f"# c4: {len(probs)} problems generated in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    t0 = time.time()
+    b = BatchKnn(probs, 15, mode=args.batch_mode)
+    tune_s = time.time() - t0
+    print(f"# c4: planned in {tune_s:.1f}s", file=sys.stderr, flush=True)
+    for _ in range(warmup):
+        b.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.run()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    print(f"# c4: {ms:.1f} ms per pass", file=sys.stderr, flush=True)
+    info = b.describe()
+    flop = float(sum(2.0 * X.shape[0] ** 2 * X.shape[1] for X in probs))
+    achieved = flop / (ms * 1e-3)
+    res = b.results()
+    h = hashlib.sha256()
+    for gi, gd in res:
+        h.update(gi.tobytes())
+        h.update(gd.tobytes())
+    order = sorted(range(len(probs)), key=lambda i: probs[i].shape[0])
+    checked = order[:len(probs) // 3]  # the smallest third, in full
+    t0 = time.perf_counter()
+    same = True
+    for i in checked:
+        oi, od = oracle.knn(probs[i], probs[i], 15, "euclidean", "for_each_knn", threads=threads)
+        same = same and bool(np.array_equal(oi, res[i][0]) and od.tobytes() == res[i][1].tobytes())
+    print(f"# c4: {len(checked)} problems checked against the oracle in {time.perf_counter() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    out = {"workload": "512 self-kNN problems, n log-uniform in [1e3, 2e5], d in {8,32,64,256}, 3-20 Gaussian "
+                       "components (means U[-5,5]^d), k=15, for_each_knn exclusion, autotuned, hipGraph replay",
+           "rows": int(sum(X.shape[0] for X in probs)), "algorithmic_flop": flop, "ms_per_pass": round(ms, 2),
+           "steps": steps, "warmup": warmup, "plan_seconds": round(tune_s, 1),
+           "plan": [{k: g[k] for k in ("d", "mode", "widen", "fill") if k in g} for g in info["groups"]],
+           "fallback_queries": info["last_fallback_queries"], "exact_queries": info["last_exact_queries"],
+           "last_pass_split_ms": {"level0": info["last_level0_ms"],
+                                  "chain": round(info["last_ms"] - info["last_level0_ms"], 3)},
+           "roofline": {"bound": "mfma", "achieved": round(achieved / 1e12, 2), "peak": PEAKS["bf16"][0] / 1e12,
+                        "unit": "TFLOP/s", "frac": round(achieved / PEAKS["bf16"][0], 4),
+                        "note": "2 n^2 d per problem over the whole pass; bf16 dense peak"},
+           "answers_sha256": h.hexdigest(), "checked_problems": len(checked), "identical": same}
+    b.close()
+    return out, probs, info
+
+
+def calibration():
+    """Achievable bf16 matrix-core rate on this box: a large torch bf16 matmul
+    (hipBLASLt), beside the nominal dense peak."""
+    import torch
+    n = 8192
+    a = torch.randn((n, n), dtype=torch.bfloat16, device="cuda")
+    b = torch.randn((n, n), dtype=torch.bfloat16, device="cuda")
+    for _ in range(3):
+        torch.matmul(a, b)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    e0.record()
+    for _ in range(reps):
+        torch.matmul(a, b)
+    e1.record()
+    torch.cuda.synchronize()
+    s = e0.elapsed_time(e1) / reps / 1e3
+    rate = 2.0 * n ** 3 / s
+    return {"bf16_gemm_tflops": round(rate / 1e12, 1), "shape": f"{n}x{n}x{n} bf16 torch.matmul",
+            "frac_of_nominal": round(rate / PEAKS["bf16"][0], 4)}
 
 
 def secondary(args):
@@ -294,33 +453,7 @@ def secondary(args):
         g = _lib.Index(Xs, "euclidean").lof(20)
         line["lof_identical_6000_subset"] = bool(g.tobytes() == oracle.lof(Xs, 20, threads=threads).tobytes())
     elif args.workload == "c4":
-        from fuel_amd import BatchKnn
-        t0 = time.time()
-        probs = workloads.c4()
-        print(f"# c4: {len(probs)} problems generated in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-        t0 = time.time()
-        b = BatchKnn(probs, 15, mode=args.batch_mode)
-        tune_s = time.time() - t0
-        print(f"# c4: planned (autotune) in {tune_s:.1f}s", file=sys.stderr, flush=True)
-        for _ in range(args.warmup):
-            b.run()
-        import torch
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            b.run()
-        torch.cuda.synchronize()
-        ms = (time.perf_counter() - t0) / args.steps * 1e3
-        info = b.describe()
-        flop = float(sum(2.0 * X.shape[0] ** 2 * X.shape[1] for X in probs))
-        achieved = flop / (ms * 1e-3)
-        # exactness: the smallest problems in full against the oracle
-        small = sorted(range(len(probs)), key=lambda i: probs[i].shape[0])[:6]
-        res = b.results()
-        same = True
-        for i in small:
-            oi, od = oracle.knn(probs[i], probs[i], 15, "euclidean", "for_each_knn", threads=threads)
-            same = same and bool(np.array_equal(oi, res[i][0]) and od.tobytes() == res[i][1].tobytes())
+        c4, probs, info = c4_run(args, threads, args.steps, args.warmup)
         # CPU baseline: the oracle's for_each_knn on two mid-size problems, rate in flop/s
         mid = sorted(range(len(probs)), key=lambda i: probs[i].shape[0])[len(probs) // 2:len(probs) // 2 + 2]
         t0 = time.perf_counter()
@@ -329,27 +462,13 @@ def secondary(args):
             oracle.knn(probs[i], probs[i], 15, "euclidean", "for_each_knn", threads=threads)
             f_s += 2.0 * probs[i].shape[0] ** 2 * probs[i].shape[1]
         t_c = time.perf_counter() - t0
-        line.update({"config": "512 self-kNN problems, n log-uniform in [1e3, 2e5], d in {8,32,64,256}, "
-                               "3-20 Gaussian components (means U[-5,5]^d), k=15, for_each_knn exclusion",
-                     "rows": int(sum(X.shape[0] for X in probs)), "algorithmic_flop": flop,
-                     "ms_per_pass": round(ms, 2), "steps": args.steps, "warmup": args.warmup,
-                     "plan_seconds": round(tune_s, 1), "plan": info["groups"], "graph": info["graph"],
-                     "tuning_best_ms": {str(dd): {m: min(t["ms"] for t in info["tuning"] if t["d"] == dd and t["mode"] == m)
-                                                  for m in {t["mode"] for t in info["tuning"] if t["d"] == dd}}
-                                        for dd in sorted({t["d"] for t in info["tuning"]})},
-                     "fallback_queries": info["last_fallback_queries"], "exact_queries": info["last_exact_queries"],
-                     "last_pass_split_ms": {"level0": info["last_level0_ms"],
-                                            "chain": round(info["last_ms"] - info["last_level0_ms"], 3)},
-                     "roofline": {"bound": "mfma", "achieved": round(achieved / 1e12, 2),
-                                  "peak": PEAKS["bf16"][0] / 1e12, "unit": "TFLOP/s",
-                                  "frac": round(achieved / PEAKS["bf16"][0], 4),
-                                  "note": "2 n^2 d per problem over the whole pass; bf16 dense peak"},
-                     "identical_smallest_6": same,
-                     "cpu_baseline": {"value": round(flop / (f_s / t_c) * 1e3, 1), "unit": "ms per pass (extrapolated)",
-                                      "cores": threads, "kind": "port",
-                                      "sample": f"oracle for_each_knn on 2 mid-size problems ({t_c:.1f}s), "
-                                                "scaled by 2 n^2 d"}})
-        b.close()
+        c4["tuning_best_ms"] = {str(dd): {m: min(t["ms"] for t in info["tuning"] if t["d"] == dd and t["mode"] == m)
+                                          for m in {t["mode"] for t in info["tuning"] if t["d"] == dd}}
+                                for dd in sorted({t["d"] for t in info["tuning"]})}
+        c4["cpu_baseline"] = {"value": round(c4["algorithmic_flop"] / (f_s / t_c) * 1e3, 1),
+                              "unit": "ms per pass (extrapolated)", "cores": threads, "kind": "port",
+                              "sample": f"oracle for_each_knn on 2 mid-size problems ({t_c:.1f}s), scaled by 2 n^2 d"}
+        line.update(c4)
     elif args.workload == "c5":
         from fuel_amd import Server
         t0 = time.time()
@@ -504,7 +623,12 @@ def main():
                         out_d)
     print(f"# {args.mode}: {main_run['ms_per_step']:.2f} ms/step, {main_run['qps']:.0f} q/s", file=sys.stderr,
           flush=True)
-    checks = verify(oracle, X, Q, out_i.cpu().numpy(), out_d.cpu().numpy(), k, args.verify)
+    t_v = time.time()
+    checks = verify(oracle, X, Q, out_i.cpu().numpy(), out_d.cpu().numpy(), k, args.verify,
+                    extra=main_run["fail_rows"], per_block=args.verify_per_block)
+    print(f"# verified {checks['queries']} queries ({checks.get('second_pass_queries', 0)} of them from the "
+          f"certification chain) in {time.time() - t_v:.1f}s: {checks['exact_rows']} identical", file=sys.stderr,
+          flush=True)
 
     # the plain FP32 MFMA form beside it (the metric is mixed vs FP32 at exact recall)
     fp32 = None
@@ -540,10 +664,30 @@ def main():
                "sample": f"{S} of the {m} queries against the full {n}x{d} database, k={k}, "
                          f"{c1 - c0:.1f}s; restated AVX2 kernel + KNNHeap, one thread per query batch"}
 
+    calib = None
+    if args.calibrate:
+        calib = calibration()
+        calib["scan_frac_of_achievable"] = round(achieved / (calib["bf16_gemm_tflops"] * 1e12), 4) \
+            if args.mode in ("bf16", "bf16r") else None
+        print(f"# calibration: {calib}", file=sys.stderr, flush=True)
+
+    del Xd, Qd
+    torch.cuda.empty_cache()
     serve = None
     if args.serve_p99_seconds > 0:
         serve = serve_headline(args)
         print(f"# serve: {serve}", file=sys.stderr, flush=True)
+    torch.cuda.empty_cache()
+
+    c3 = None
+    if args.c3_full:
+        c3 = c3_full(oracle.default_threads())
+        torch.cuda.empty_cache()
+
+    c4 = None
+    if args.c4_line:
+        c4, _, _ = c4_run(args, oracle.default_threads(), args.c4_steps, 1)
+        torch.cuda.empty_cache()
 
     stats = main_run["stats"]
     line = {
@@ -568,6 +712,10 @@ def main():
         "cpu_baseline": cpu,
         "p99_serve_ms": serve["p99_ms"] if serve else None,
         "serve": serve,
+        "calibration": calib,
+        "c3_full_identical": c3["identical"] if c3 else None,
+        "c3": c3,
+        "c4": c4,
     }
     print(json.dumps(line), flush=True)
 

@@ -47,6 +47,7 @@ EXPORTED = [
     "fuel_amd_index_create", "fuel_amd_index_create_device", "fuel_amd_index_destroy",
     "fuel_amd_knn", "fuel_amd_knn_device", "fuel_amd_range", "fuel_amd_knn_ties", "fuel_amd_csr_free",
     "fuel_amd_pairwise", "fuel_amd_index_pairwise", "fuel_amd_lof", "fuel_amd_knn_outlier", "fuel_amd_last_stats",
+    "fuel_amd_last_fallback_rows",
     "fuel_amd_dbscan", "fuel_amd_optics", "fuel_amd_pairwise_sqdist", "fuel_amd_pairwise_sqdist_device",
     "fuel_amd_timing_enable", "fuel_amd_timing_read",
     "fuel_amd_server_create", "fuel_amd_server_submit", "fuel_amd_server_wait", "fuel_amd_server_stats",
@@ -164,6 +165,8 @@ def load():
     lib.fuel_amd_knn_outlier.restype = I32
     lib.fuel_amd_last_stats.argtypes = [P, ctypes.POINTER(Stats)]
     lib.fuel_amd_last_stats.restype = I32
+    lib.fuel_amd_last_fallback_rows.argtypes = [P, ctypes.POINTER(I64), I64, ctypes.POINTER(I64)]
+    lib.fuel_amd_last_fallback_rows.restype = I32
     lib.fuel_amd_timing_enable.argtypes = [P, I32]
     lib.fuel_amd_timing_enable.restype = I32
     lib.fuel_amd_timing_read.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(I64),
@@ -384,6 +387,17 @@ class Index:
         name = ctypes.create_string_buffer(128)
         check(load().fuel_amd_timing_read(self.handle, ctypes.byref(ms), ctypes.byref(cnt), name, 128))
         return ms.value, cnt.value, name.value.decode()
+
+    def fallback_rows(self):
+        """Query rows of the last kNN call whose first-level certificate failed."""
+        lib = load()
+        cnt = ctypes.c_int64()
+        check(lib.fuel_amd_last_fallback_rows(self.handle, None, 0, ctypes.byref(cnt)))
+        rows = np.empty(cnt.value, np.int64)
+        if cnt.value:
+            check(lib.fuel_amd_last_fallback_rows(self.handle, rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                                   cnt.value, ctypes.byref(cnt)))
+        return rows
 
     def stats(self):
         s = Stats()

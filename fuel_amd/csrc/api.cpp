@@ -303,6 +303,14 @@ int32_t fuel_amd_last_stats(const fuel_amd_index* ix, fuel_amd_stats* out) {
   return FUEL_AMD_OK;
 }
 
+int32_t fuel_amd_last_fallback_rows(const fuel_amd_index* ix, int64_t* rows, int64_t cap, int64_t* count) {
+  if (!ix || !count || (cap > 0 && !rows) || cap < 0) return fail(FUEL_AMD_ERR_INVALID, "bad argument");
+  const auto& v = ix->ctx.fail_rows;
+  *count = (int64_t)v.size();
+  for (int64_t i = 0; i < cap && i < (int64_t)v.size(); ++i) rows[i] = v[i];
+  return FUEL_AMD_OK;
+}
+
 // ---- kNN --------------------------------------------------------------------
 
 int32_t fuel_amd_knn_device(fuel_amd_index* ix, const void* d_query, int64_t m, int64_t ldq, int32_t k,

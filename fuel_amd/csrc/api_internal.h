@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <cstdint>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -58,6 +59,7 @@ struct QueryCtx {
   DevBuf slots, lcnts;  // exact scan
   DevBuf raw, lo, rcnt;  // exact kNN in rounds (k beyond one slot)
   fuel_amd_stats stats{};
+  std::vector<int64_t> fail_rows;  // query rows whose first-level certificate failed (last call)
   bool timed = false;  // record kernel timing events (the index's own context)
   void release() {
     for (auto& w : lv) {

@@ -400,6 +400,7 @@ int32_t knn_run(fuel_amd_index* ix, QueryCtx* ctx, const void* d_query, int64_t 
   }
   ctx->stats = fuel_amd_stats{};
   ctx->stats.queries = m;
+  ctx->fail_rows.clear();
   int kind = mode == FUEL_AMD_MODE_BF16 ? 0 : (mode == FUEL_AMD_MODE_FP32 ? 1 : (mode == FUEL_AMD_MODE_BF16R ? 2 : -1));
   if (kind == 2 && !mfma_supported(ix, 2, k, exclude)) kind = 0;  // no residual variant this wide
   if (kind >= 0 && !mfma_supported(ix, kind, k, exclude)) kind = -1;

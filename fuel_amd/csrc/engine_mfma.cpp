@@ -26,6 +26,7 @@ namespace {
 
 constexpr size_t MFMA_SLOT_BUDGET = size_t(2) << 30;  // bytes of candidate slots per batch
 constexpr int KP_MAX = 128;
+constexpr size_t kFailRowsKept = size_t(1) << 20;  // rows of failed queries recorded per call
 
 int kp_for(const fuel_amd_index* ix, int k, int exclude) {
   const int KK = k + (exclude != EXCL_NONE ? 1 : 0);
@@ -594,6 +595,15 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
                             st);
       }
       if (rc) return rc;
+    }
+    if (level == 0 && !probe && nfail > 0 && ctx->fail_rows.size() < kFailRowsKept) {
+      // record the rows of the failed queries (in the caller's row numbering)
+      const size_t take = std::min<size_t>(size_t(nfail), kFailRowsKept - ctx->fail_rows.size());
+      std::vector<int64_t> h(take);
+      e = hipMemcpyAsync(h.data(), ra.fail_list, sizeof(int64_t) * take, hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      if (e != hipSuccess) return hip_fail(e, "certification list");
+      for (int64_t r : h) ctx->fail_rows.push_back(d_rows ? r : done + r);
     }
     if (level == 0) {
       ctx->stats.fallback_queries += nfail;

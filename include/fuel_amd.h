@@ -263,6 +263,13 @@ int32_t fuel_amd_knn_outlier(fuel_amd_index* index, int32_t k, void* out_scores)
 
 /* Statistics of the last kNN call on this index. */
 int32_t fuel_amd_last_stats(const fuel_amd_index* index, fuel_amd_stats* out);
+/* Query rows of the last kNN call on this index whose first-level certificate
+ * failed (stats.fallback_queries of them; up to 2^20 are kept), in the call's
+ * row numbering: *count receives how many are kept, the first min(cap, count)
+ * are written to rows.  A diagnostic with no counterpart in fuel (its linear
+ * scan has no certificate): it names the queries the certification chain
+ * answered, e.g. for checking them against a CPU scan. */
+int32_t fuel_amd_last_fallback_rows(const fuel_amd_index* index, int64_t* rows, int64_t cap, int64_t* count);
 
 /* Profiling aid (no counterpart in fuel, which has no GPU): when enabled, the
  * dominant scan kernel of every kNN call on this index is bracketed by HIP
