@@ -29,6 +29,7 @@ SOURCES = [
     "serve.cpp",
     "batch.cpp",
     "priority.cpp",
+    "noalloc.hip",
     "resid.hip",
     "scan_exact.hip",
     "scan_mfma.hip",

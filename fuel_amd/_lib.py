@@ -45,7 +45,7 @@ EXPORTED = [
     "fuel_amd_version", "fuel_amd_device_count", "fuel_amd_last_error",
     "fuel_amd_request_cancel", "fuel_amd_clear_cancel",
     "fuel_amd_index_create", "fuel_amd_index_create_device", "fuel_amd_index_destroy",
-    "fuel_amd_knn", "fuel_amd_knn_device", "fuel_amd_range", "fuel_amd_knn_ties", "fuel_amd_csr_free",
+    "fuel_amd_knn", "fuel_amd_knn_device", "fuel_amd_knn_device_noalloc", "fuel_amd_range", "fuel_amd_knn_ties", "fuel_amd_csr_free",
     "fuel_amd_pairwise", "fuel_amd_index_pairwise", "fuel_amd_lof", "fuel_amd_knn_outlier", "fuel_amd_last_stats",
     "fuel_amd_last_fallback_rows",
     "fuel_amd_dbscan", "fuel_amd_optics", "fuel_amd_pairwise_sqdist", "fuel_amd_pairwise_sqdist_device",

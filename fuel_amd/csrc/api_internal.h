@@ -157,6 +157,11 @@ namespace fa {
 // index's own unless a serving worker passes its own).
 int32_t knn_run(fuel_amd_index* ix, QueryCtx* ctx, const void* d_query, int64_t m, int64_t ldq, int k,
                 int exclude, int64_t* d_out_idx, void* d_out_dist, hipStream_t st);
+// The exact kNN of m device query rows in one pass, its candidate slots and
+// list counts in the given device memory: allocates nothing and never waits for
+// the stream.  Refuses (FUEL_AMD_ERR_INVALID) a call that does not fit.
+int32_t knn_run_scratch(fuel_amd_index* ix, const void* d_query, int64_t m, int64_t ldq, int k, int exclude,
+                        int64_t* d_out_idx, void* d_out_dist, hipStream_t st, void* scratch, size_t bytes);
 // Prepare the operand copy of the mode the index's options select (a no-op for
 // the exact mode); the serving workers call the kNN path only after this.
 int32_t prepare_mode(fuel_amd_index* ix, hipStream_t st);

@@ -382,6 +382,80 @@ int32_t outlier_impl(fuel_amd_index* ix, int k, int knn_outlier, void* out_score
 
 }  // namespace
 
+// The exact kNN of m query rows in one pass whose candidate slots and list
+// counts live in caller-provided device memory (`scratch`, `bytes`): no
+// allocation, no host synchronization.  The split count is the one knn_exact
+// would choose, reduced until the pass fits; a pass that does not fit with one
+// split is refused.
+template <typename F>
+int32_t knn_exact_in_scratch(fuel_amd_index* ix, const F* Q, int64_t m, int64_t ldq, int k, int mode, int64_t* out_idx,
+                             F* out_dist, hipStream_t st, void* scratch, size_t bytes) {
+  using T = typename Comp<F>::T;
+  const int KK = k + (mode != EXCL_NONE ? 1 : 0);
+  if (KK > EXACT_NS / 2)
+    return fail(FUEL_AMD_ERR_UNSUPPORTED,
+                "the allocation-free kNN takes k + exclusion <= " + std::to_string(EXACT_NS / 2));
+  const int qw = exact_qw(m);
+  int nsplit = choose_splits((m + qw - 1) / qw, ix->n, device_cu_count(ix->device), 512);
+  auto need = [&](int s) {
+    return ((size_t(s) * EXACT_NS * sizeof(T) * m + 255) & ~size_t(255)) + sizeof(int) * size_t(s) * m;
+  };
+  while (nsplit > 1 && need(nsplit) > bytes) nsplit = (nsplit + 1) / 2;
+  if (need(nsplit) > bytes)
+    return fail(FUEL_AMD_ERR_INVALID, "the allocation-free kNN holds at most " +
+                                          std::to_string(bytes / (EXACT_NS * sizeof(T) + sizeof(int))) +
+                                          " queries per call (" + std::to_string(m) + " given)");
+  int64_t rps = (ix->n + nsplit - 1) / nsplit;
+  rps = std::max<int64_t>(256, (rps + 255) & ~int64_t(255));
+  nsplit = (int)std::max<int64_t>(1, (ix->n + rps - 1) / rps);
+  ScanExactArgs<F> sa{};
+  sa.X = ix->X.as<F>();
+  sa.ldx = ix->ld;
+  sa.n = ix->n;
+  sa.d = ix->d;
+  sa.Q = Q;
+  sa.ldq = ldq;
+  sa.nq = m;
+  sa.metric = ix->metric;
+  sa.arith = ix->arith;
+  sa.KP = KK;
+  sa.rows_per_split = rps;
+  // every (split, query) list count is written by its workgroup before the
+  // finalize reads it, and slots are read only below those counts: nothing in
+  // the scratch needs clearing between calls
+  sa.slots = static_cast<T*>(scratch);
+  sa.lcnts = reinterpret_cast<int*>(static_cast<char*>(scratch) +
+                                    ((size_t(nsplit) * EXACT_NS * sizeof(T) * m + 255) & ~size_t(255)));
+  hipError_t e = launch_scan_exact<F>(sa, qw, nsplit, st);
+  if (e != hipSuccess) return hip_fail(e, "scan_exact launch");
+  FinalizeArgs<F> fa_{};
+  fa_.slots = sa.slots;
+  fa_.lcnts = sa.lcnts;
+  fa_.slot_stride = EXACT_NS;
+  fa_.nsplit = nsplit;
+  fa_.nq = m;
+  fa_.KK = KK;
+  fa_.k = k;
+  fa_.mode = mode;
+  fa_.qbase = 0;
+  fa_.out_idx = out_idx;
+  fa_.out_dist = out_dist;
+  e = launch_finalize_exact<F>(fa_, st);
+  if (e != hipSuccess) return hip_fail(e, "finalize launch");
+  return FUEL_AMD_OK;
+}
+
+int32_t knn_run_scratch(fuel_amd_index* ix, const void* d_query, int64_t m, int64_t ldq, int k, int exclude,
+                        int64_t* d_out_idx, void* d_out_dist, hipStream_t st, void* scratch, size_t bytes) {
+  if (m == 0 || k == 0) return FUEL_AMD_OK;
+  if (!d_query || !d_out_idx || !d_out_dist) return fail(FUEL_AMD_ERR_INVALID, "NULL buffer");
+  return ix->dtype == FUEL_AMD_F32
+             ? knn_exact_in_scratch<float>(ix, (const float*)d_query, m, ldq, k, exclude, d_out_idx,
+                                           (float*)d_out_dist, st, scratch, bytes)
+             : knn_exact_in_scratch<double>(ix, (const double*)d_query, m, ldq, k, exclude, d_out_idx,
+                                            (double*)d_out_dist, st, scratch, bytes);
+}
+
 int32_t knn_exact_rows(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t ldq, const int64_t* d_rows,
                        int64_t nrows, int64_t qbase, int k, int exclude, int64_t* out_idx, float* out_dist,
                        hipStream_t st) {

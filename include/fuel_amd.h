@@ -162,6 +162,22 @@ int32_t fuel_amd_knn_device(fuel_amd_index* index, const void* d_query, int64_t 
                             int32_t k, int32_t exclude_self, int64_t* d_out_idx, void* d_out_dist,
                             void* stream);
 
+/* Allocation-free kNN.  No counterpart in fuel: it is for callers that cannot
+ * allocate device memory inside the call (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and answers
+ * as fuel_amd_knn_device, computed by the exact restated scan whatever the
+ * index mode, with k + (exclude_self ? 1 : 0) <= 256.  The candidate slots and
+ * list counts live in a fixed 64 MiB device array per GPU that the library
+ * holds until the process ends; the call calls no hipMalloc / hipFree and never
+ * waits for `stream`.  A batch that does not fit the array even with one DB
+ * split (about 16,000 f32 or 8,000 f64 queries) is refused with
+ * FUEL_AMD_ERR_INVALID.  Calls on one index are serialized, but every call on
+ * a GPU uses that GPU's one array: calls on different indexes of one GPU whose
+ * work overlaps on the device (different streams) race; order them on one
+ * stream. */
+int32_t fuel_amd_knn_device_noalloc(fuel_amd_index* index, const void* d_query, int64_t m, int64_t ldq, int32_t k,
+                                    int32_t exclude_self, int64_t* d_out_idx, void* d_out_dist, void* stream);
+
 /* All neighbours within `radius` (inclusive), sorted by (distance, index).
  * Replaces SearchIndex.radius_search for the LinearScan variants
  * (python/search.rs:776-787, linear_scan_range_batch :494-514,
