@@ -44,6 +44,7 @@ def parse():
     p.add_argument("--d", type=int, default=128)
     p.add_argument("--k", type=int, default=10)
     p.add_argument("--widen", type=int, default=4, help="candidates kept per query = widen * k (<= 128)")
+    p.add_argument("--seed", type=int, default=1, help="seeded first-pass thresholds of the plain MFMA scans (0: off)")
     p.add_argument("--verify", type=int, default=64, help="random queries checked against the oracle")
     p.add_argument("--verify-per-block", type=int, default=6,
                    help="queries checked from every block of 256 queries (beside every query whose first "
@@ -121,12 +122,13 @@ def copy_probe(nbytes):
 METRIC = "kNN queries/s, 1M\u00d7128 k=10: mixed vs FP32 at exact recall; p99 serve ms"
 
 
-def run_mode(_lib, Xd, Qd, n, m, d, k, mode, widen, steps, warmup, stream, out_i, out_d):
+def run_mode(_lib, Xd, Qd, n, m, d, k, mode, widen, steps, warmup, stream, out_i, out_d, seed=True):
     """Warm up, then time `steps` kNN passes of all queries on a resident index.
     Returns wall time per step, the dominant scan kernel's HIP-event time and the
     index statistics of the last pass."""
     import torch
-    ix = _lib.Index.from_device(Xd.data_ptr(), n, d, d, np.float32, "euclidean", mode=mode, widen=widen)
+    ix = _lib.Index.from_device(Xd.data_ptr(), n, d, d, np.float32, "euclidean", mode=mode, widen=widen,
+                                seed=seed)
 
     def step():
         ix.knn_device(Qd.data_ptr(), m, d, k, _lib.EXCLUDE_NONE, out_i.data_ptr(), out_d.data_ptr(), stream)
@@ -620,7 +622,7 @@ def main():
         ix.close()
         return
     main_run = run_mode(_lib, Xd, Qd, n, m, d, k, args.mode, args.widen, args.steps, args.warmup, stream, out_i,
-                        out_d)
+                        out_d, seed=bool(args.seed))
     print(f"# {args.mode}: {main_run['ms_per_step']:.2f} ms/step, {main_run['qps']:.0f} q/s", file=sys.stderr,
           flush=True)
     t_v = time.time()
@@ -633,7 +635,8 @@ def main():
     # the plain FP32 MFMA form beside it (the metric is mixed vs FP32 at exact recall)
     fp32 = None
     if args.mode != "fp32" and args.fp32_steps > 0:
-        r = run_mode(_lib, Xd, Qd, n, m, d, k, "fp32", args.widen, args.fp32_steps, 1, stream, out_i, out_d)
+        r = run_mode(_lib, Xd, Qd, n, m, d, k, "fp32", args.widen, args.fp32_steps, 1, stream, out_i, out_d,
+                     seed=bool(args.seed))
         print(f"# fp32: {r['ms_per_step']:.2f} ms/step, {r['qps']:.0f} q/s", file=sys.stderr, flush=True)
         c32 = verify(oracle, X, Q, out_i.cpu().numpy(), out_d.cpu().numpy(), k, args.verify)
         fp32 = {"value": round(r["qps"], 1), "ms_per_step": round(r["ms_per_step"], 3), "steps": args.fp32_steps,

@@ -35,6 +35,7 @@ FORM_CONDENSED = 1
 
 MODES = {"auto": MODE_AUTO, "exact": MODE_EXACT, "fp32": MODE_FP32, "bf16": MODE_BF16, "bf16r": MODE_BF16R}
 FLAG_FALLBACK_ARITH = 1
+FLAG_NO_SEED = 2
 # rounding of the restated distance: "avx" = fuel's AVX+FMA kernels
 # (src/distance/squared_euclidean.rs:13-67, x86 hosts with AVX2+FMA),
 # "fallback" = its portable kernel (:99-133, every other host)
@@ -252,26 +253,29 @@ def metric_code(name):
 class Index:
     """Owner of a fuel_amd_index handle."""
 
-    def __init__(self, data, metric="euclidean", mode="auto", widen=4, device=0, arith="avx"):
+    def __init__(self, data, metric="euclidean", mode="auto", widen=4, device=0, arith="avx", seed=True):
         lib = load()
         data = np.ascontiguousarray(data)
         if data.ndim != 2:
             raise ValueError("data must be a 2D array")
         self.dtype = data.dtype
         self.n, self.d = data.shape
-        opts = Opts(MODES[mode] if isinstance(mode, str) else int(mode), int(widen), int(device), ARITH[arith])
+        opts = Opts(MODES[mode] if isinstance(mode, str) else int(mode), int(widen), int(device),
+                    ARITH[arith] | (0 if seed else FLAG_NO_SEED))
         h = ctypes.c_void_p()
         check(lib.fuel_amd_index_create(data.ctypes.data, self.n, self.d, dtype_code(data.dtype),
                                         metric_code(metric), ctypes.byref(opts), ctypes.byref(h)))
         self.handle = h
 
     @classmethod
-    def from_device(cls, ptr, n, d, ld, dtype, metric="euclidean", mode="auto", widen=4, device=0, arith="avx"):
+    def from_device(cls, ptr, n, d, ld, dtype, metric="euclidean", mode="auto", widen=4, device=0, arith="avx",
+                    seed=True):
         lib = load()
         self = cls.__new__(cls)
         self.dtype = np.dtype(dtype)
         self.n, self.d = n, d
-        opts = Opts(MODES[mode] if isinstance(mode, str) else int(mode), int(widen), int(device), ARITH[arith])
+        opts = Opts(MODES[mode] if isinstance(mode, str) else int(mode), int(widen), int(device),
+                    ARITH[arith] | (0 if seed else FLAG_NO_SEED))
         h = ctypes.c_void_p()
         check(lib.fuel_amd_index_create_device(ctypes.c_void_p(ptr), n, d, ld, dtype_code(dtype),
                                                metric_code(metric), ctypes.byref(opts), ctypes.byref(h)))

@@ -187,7 +187,8 @@ static int32_t index_create_common(const void* data, bool on_device, int64_t n, 
   if (opts) o = *opts;
   if (o.widen <= 0) o.widen = 4;
   if (o.mode < FUEL_AMD_MODE_AUTO || o.mode > FUEL_AMD_MODE_BF16R) return fail(FUEL_AMD_ERR_INVALID, "bad mode");
-  if (o.flags & ~FUEL_AMD_FLAG_FALLBACK_ARITH) return fail(FUEL_AMD_ERR_INVALID, "unknown option flags");
+  if (o.flags & ~(FUEL_AMD_FLAG_FALLBACK_ARITH | FUEL_AMD_FLAG_NO_SEED))
+    return fail(FUEL_AMD_ERR_INVALID, "unknown option flags");
   int32_t rc = check_device(o.device);
   if (rc) return rc;
   auto* ix = new (std::nothrow) fuel_amd_index();

@@ -54,7 +54,7 @@ hipError_t upload_rows(void* dst, int64_t ld, const void* host, int64_t rows, in
 // against one resident index (whose rows and operand copies are read-only).
 struct QueryCtx {
   struct LevelWs {  // one per level of the certification chain
-    DevBuf Qop, Q32, qnorm, qint, slots, lcnts, fail_count, fail_list, fail_theta, gthr, qtab, qs2;
+    DevBuf Qop, Q32, qnorm, qint, slots, lcnts, fail_count, fail_list, fail_theta, gthr, qtab, qs2, seed;
   } lv[3];
   DevBuf slots, lcnts;  // exact scan
   DevBuf raw, lo, rcnt;  // exact kNN in rounds (k beyond one slot)
@@ -75,6 +75,7 @@ struct QueryCtx {
       w.gthr.release();
       w.qtab.release();
       w.qs2.release();
+      w.seed.release();
     }
     slots.release();
     lcnts.release();
@@ -209,6 +210,13 @@ struct MfmaChunkPlan {
 };
 MfmaChunkPlan plan_mfma_chunk(int mk, bool resid, int ks, int64_t m_total, int64_t remaining, int KP, int64_t n_pad,
                               int cus);
+// Seed pass of a plain MFMA scan (ntiles = 0: none): sample tiles, the rows
+// between them and the workgroups per query block.
+struct SeedPlan {
+  int64_t ntiles = 0, stride = 0;
+  int parts = 0;
+};
+SeedPlan plan_seed(int mk, int ks, bool resid, bool has_theta, int flags, int64_t n_pad, int rows_tile, int KP);
 // Padded residual layout of lists with cnt[j] rows (returns the operand rows).
 int64_t resid_layout(const std::vector<uint32_t>& cnt, std::vector<int64_t>& pad, std::vector<int>& tl);
 // Residual operand layout of n rows (f32, pitch ld): centroids (L x d), padded

@@ -63,6 +63,8 @@ struct ProbPlan {
   size_t o_xop = 0, o_hx = 0, o_consts = 0, o_qop = 0, o_q32 = 0, o_qn = 0, o_qint = 0, o_gthr = 0, o_slots = 0,
          o_lcnts = 0, o_lconsts = 0, o_qtab = 0, o_qs2 = 0;  // level-0 pool (shared by the groups in turn)
   size_t o_fl = 0, o_ft = 0;  // kept pool: certification failures of every problem
+  SeedPlan seed;              // seeded thresholds of the plain operands (none when ntiles = 0)
+  size_t o_seed = 0;
 };
 
 // Residual layout of one problem (plan time; the graph rebuilds the operand
@@ -253,6 +255,11 @@ int32_t plan_batch(fuel_amd_batch* b, const std::map<int, Cfg>& cfg_by_d) {
       off += align256(size_t(pr.n) * 4);
       pp.o_gthr = off;
       off += align256(size_t(pp.nq_pad) * 4);
+      pp.seed = plan_seed(mk, bk.ks, resid, false, 0, pp.n_pad, rows_tile, pp.KP);
+      if (pp.seed.ntiles > 0) {
+        pp.o_seed = off;
+        off += align256(size_t(pp.nq_pad) * pp.seed.parts * 2 * SEED_GROUPS * 4);
+      }
       pp.o_slots = off;
       off += align256(size_t(pp.nsplit) * pp.nq_pad * NS * 8);
       pp.o_lcnts = off;
@@ -415,6 +422,23 @@ hipError_t enqueue_bucket(fuel_amd_batch* b, const Bucket& B, hipStream_t st) {
     qa.qint = at<int>(b->pool, pp.o_qint);
     qa.gthr = at<uint32_t>(b->pool, pp.o_gthr);
     if (e == hipSuccess) e = launch_prep_queries(qa, st);
+    if (pp.seed.ntiles > 0 && e == hipSuccess) {
+      SeedArgs sd{};
+      sd.Xop = at<void>(b->pool, pp.o_xop);
+      sd.hx = at<float>(b->pool, pp.o_hx);
+      sd.Kp = pp.Kp;
+      sd.Qop = qa.Qop;
+      sd.nq_pad = pp.nq_pad;
+      sd.nqb = (int)pp.nqb;
+      sd.stride = pp.seed.stride;
+      sd.ntiles = pp.seed.ntiles;
+      sd.parts = pp.seed.parts;
+      sd.mins = at<float>(b->pool, pp.o_seed);
+      sd.m = pr.n;
+      sd.KP = pp.KP;
+      sd.gthr = qa.gthr;
+      e = launch_seed_mfma(sd, pp.kind, pp.ks, st);
+    }
     if (pp.kind == 2 && e == hipSuccess) {
       KmeansPairwiseArgs<float> pw{};
       pw.P1 = qa.Q32;

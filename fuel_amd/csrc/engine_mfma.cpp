@@ -268,6 +268,30 @@ int mfma_splits(int64_t nqb, int64_t n_pad, int cus, int64_t rows_tile, int wg_p
 
 }  // namespace
 
+// Sample of the seed pass (ntiles = 0: no seed).  Every group of every part
+// must receive at least one row block (parts * SEED_GROUPS / R tiles), the
+// groups must number at least KP, and the sample stays at about 1/16 of the
+// operand's tiles, at most 1024 tiles; operands of fewer than 8 samples' worth
+// of tiles start cold.
+SeedPlan plan_seed(int mk, int ks, bool resid, bool has_theta, int flags, int64_t n_pad, int rows_tile, int KP) {
+  SeedPlan p;
+  if (resid || has_theta || (flags & FUEL_AMD_FLAG_NO_SEED)) return p;
+  // the f32 form at more than 4 K steps is bound by its matrix-core rate, not by
+  // the candidate appends the seed saves (1M x 128: scan -4.6 ms, seed +13 ms)
+  if (mk == 1 && ks > 4) return p;
+  const int R = rows_tile / 32;
+  p.parts = std::max(4, (KP + 2 * SEED_GROUPS - 1) / (2 * SEED_GROUPS));
+  const int64_t tiles = n_pad / rows_tile;
+  const int64_t need = int64_t(p.parts) * SEED_GROUPS / R;
+  if (tiles < 8 * need) {
+    p.parts = 0;
+    return p;
+  }
+  p.ntiles = std::min<int64_t>(1024, std::max<int64_t>(need, tiles / 16));
+  p.stride = (tiles / p.ntiles) * rows_tile;
+  return p;
+}
+
 MfmaChunkPlan plan_mfma_chunk(int mk, bool resid, int ks, int64_t m_total, int64_t remaining, int KP, int64_t n_pad,
                               int cus) {
   MfmaChunkPlan p{};
@@ -483,6 +507,38 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
     }
     if (e == hipSuccess) e = launch_fill_u32(ws.fail_count.as<uint32_t>(), 1, 0u, st);
     if (e != hipSuccess) return hip_fail(e, "query preparation");
+
+    // Seeded thresholds for a cold start of the plain operands: the KP-th
+    // smallest group minimum over an evenly strided sample of about 1/16 of the
+    // tiles (at most 1024) lowers each query's shared threshold before the scan,
+    // so its splits append few rows that a tighter threshold would drop.
+    {
+      const SeedPlan sp = plan_seed(mk, ks, resid, d_theta != nullptr, ix->opts.flags, n_pad, pl.rows_tile, KP);
+      if (sp.ntiles > 0) {
+        e = ws.seed.ensure(sizeof(float) * size_t(nq_pad) * sp.parts * 2 * SEED_GROUPS);
+        if (e != hipSuccess) return hip_fail(e, "seed workspace");
+        SeedArgs sd{};
+        sd.Xop = op.Xop.p;
+        sd.hx = op.hx.as<float>();
+        sd.Kp = Kp;
+        sd.Qop = ws.Qop.p;
+        sd.nq_pad = nq_pad;
+        sd.nqb = (int)nqb;
+        sd.stride = sp.stride;
+        sd.ntiles = sp.ntiles;
+        sd.parts = sp.parts;
+        sd.mins = ws.seed.as<float>();
+        sd.m = chunk;
+        sd.KP = KP;
+        sd.gthr = ws.gthr.as<uint32_t>();
+        // the seed pass reads the same operand rows and query fragments as the scan
+        if (size_t(sp.stride) * (sp.ntiles - 1) + size_t(pl.rows_tile) > size_t(n_pad) ||
+            size_t(nq_pad) * Kp * es > ws.Qop.bytes)
+          return fail(FUEL_AMD_ERR_DEVICE, "internal error: seed plan outside its buffers");
+        e = launch_seed_mfma(sd, mk, ks, st);
+        if (e != hipSuccess) return hip_fail(e, "seed launch");
+      }
+    }
 
     ScanMfmaArgs sa{};
     sa.Xop = op.Xop.p;

@@ -441,6 +441,33 @@ int mfma_rows_per_tile(int kind, int ksteps, bool resid);
 int mfma_workgroups_per_cu(int kind, int ksteps, int waves, bool resid);
 bool mfma_one_wave_supported(int kind, int ksteps, int V, bool resid);
 
+// Seeded thresholds of the plain MFMA scans.  Workgroup (query block qb, part
+// p) runs the scan's own tile pass (same fragments, C inputs and MFMA order,
+// so the keys are the scan's) over the sample tiles p, p + parts, ... < ntiles,
+// tile i starting at row i * stride, and keeps no candidates: per lane the
+// smallest key of each of SEED_GROUPS groups of rows.  The group minima of a
+// query are keys of distinct rows, so the KP-th smallest of them bounds the
+// query's KP-th smallest key from above; the select pass lowers the query's
+// shared threshold to it.  Exactness never depends on the seed: the
+// certificate uses the thresholds the scan actually applied.
+constexpr int SEED_GROUPS = 16;  // per lane half
+struct SeedArgs {
+  const void* Xop;
+  const float* hx;
+  int Kp;
+  const void* Qop;
+  int64_t nq_pad;
+  int nqb;
+  int64_t stride;  // rows between sample tiles (a multiple of the tile)
+  int64_t ntiles;  // sample tiles
+  int parts;       // workgroups per query block
+  float* mins;     // [nq_pad][parts][2][SEED_GROUPS] group minima (+inf when empty)
+  int64_t m;       // real queries (slots beyond keep their threshold)
+  int KP;
+  uint32_t* gthr;  // [nq_pad] shared thresholds, lowered by the select pass
+};
+hipError_t launch_seed_mfma(const SeedArgs& s, int kind, int ksteps, hipStream_t st);
+
 // Bytes addressable from each pointer of a ScanMfmaArgs.
 struct ScanExtents {
   size_t xop, hx, qop, slots, lcnts, gthr, tile_list, cents, q32, qtab;

@@ -571,6 +571,201 @@ bool mfma_one_wave_supported(int kind, int ksteps, int V, bool resid) {
   return kind == 0 && resid && V == 4 && ksteps <= 12;
 }
 
+// ---- seeded thresholds (SeedArgs in kernels.h) ----------------------------------
+// The tile pass of scan_mfma_body without its selection: per (query group, row
+// block) a max tree over the lane's 16 accumulators, folded into the lane's
+// group register; the group registers rotate by R per tile, so tile i's row
+// block rb lands in group (i R + rb) mod SEED_GROUPS.
+template <int KIND, int KS, int A, int R>
+__device__ __forceinline__ void seed_mfma_body(const SeedArgs& a, const int qb, const int part) {
+  constexpr int W = 4;
+  constexpr int G = SEED_GROUPS;
+  static_assert(G % R == 0, "groups rotate by whole tiles");
+  constexpr int BN = 32 * R;
+  constexpr int ROWB = (2 * KS + 1) * 16;
+  constexpr int WAVE_LOADS = scan_wave_loads(KS, R, W);
+  constexpr int TILE_BYTES = scan_tile_bytes(KS, R, W);
+  __shared__ __attribute__((aligned(16))) char tile[2][TILE_BYTES];
+  __shared__ __attribute__((aligned(16))) float tile_hx[2][64];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int half = lane >> 5;
+  const int col = lane & 31;
+  const int wave = tid >> 6;
+  const int64_t q0 = scan_wave_q0(qb, wave, A, W);
+  const size_t kp_bytes = size_t(KS) * 32;
+  const char* xop = reinterpret_cast<const char*>(a.Xop);
+  uint4 qf[A][KS];
+#pragma unroll
+  for (int g = 0; g < A; ++g) {
+    const char* qrow = reinterpret_cast<const char*>(a.Qop) + size_t(q0 + g * 32 + col) * kp_bytes + half * 16;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) qf[g][s] = *reinterpret_cast<const uint4*>(qrow + s * 32);
+  }
+  float best[A][G];  // largest accumulator = smallest key, per group
+#pragma unroll
+  for (int g = 0; g < A; ++g)
+#pragma unroll
+    for (int j = 0; j < G; ++j) best[g][j] = -INFINITY;
+  uint32_t soff[WAVE_LOADS];
+#pragma unroll
+  for (int j = 0; j < WAVE_LOADS; ++j) soff[j] = scan_stage_src(KS, R, W, j, wave, lane);
+  const uint32_t lds_tile = (uint32_t)(uintptr_t)&tile[0][0];
+  const uint32_t lds_hx = (uint32_t)(uintptr_t)&tile_hx[0][0];
+  auto stage = [&](int b, int64_t t) {
+    const char* base = xop + size_t(t) * kp_bytes;
+#pragma unroll
+    for (int j = 0; j < WAVE_LOADS; ++j)
+      lds_dma16(__builtin_amdgcn_readfirstlane(lds_tile + b * TILE_BYTES + (j * W + wave) * 1024), base + soff[j]);
+    if (wave == 0) {
+      const int r = lane < BN ? lane : BN - 1;
+      lds_dma4(__builtin_amdgcn_readfirstlane(lds_hx + b * 64 * 4), a.hx + t + r);
+    }
+  };
+  const int64_t step = a.parts;
+  int64_t i = part;
+  if (i < a.ntiles) stage(0, i * a.stride);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (i + step < a.ntiles) stage(1, (i + step) * a.stride);
+  int buf = 0;
+  for (; i < a.ntiles; i += step) {
+    const char* tb = tile[buf];
+    const float* hb = tile_hx[buf];
+    floatx16 acc[A][R];
+#pragma unroll
+    for (int rb = 0; rb < R; ++rb) {
+      floatx16 h;
+#pragma unroll
+      for (int q4 = 0; q4 < 4; ++q4) {
+        const float4 h4 = *reinterpret_cast<const float4*>(hb + rb * 32 + q4 * 8 + half * 4);
+        h[q4 * 4 + 0] = h4.x;
+        h[q4 * 4 + 1] = h4.y;
+        h[q4 * 4 + 2] = h4.z;
+        h[q4 * 4 + 3] = h4.w;
+      }
+#pragma unroll
+      for (int g = 0; g < A; ++g) acc[g][rb] = h;
+    }
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      uint4 af[R];
+#pragma unroll
+      for (int rb = 0; rb < R; ++rb)
+        af[rb] = *reinterpret_cast<const uint4*>(tb + (rb * 32 + col) * ROWB + s * 32 + half * 16);
+#pragma unroll
+      for (int g = 0; g < A; ++g)
+#pragma unroll
+        for (int rb = 0; rb < R; ++rb) Mfma<KIND>::step(acc[g][rb], af[rb], qf[g][s]);
+    }
+#pragma unroll
+    for (int g = 0; g < A; ++g) {
+#pragma unroll
+      for (int rb = 0; rb < R; ++rb) {
+        float mx = acc[g][rb][0];
+#pragma unroll
+        for (int r = 1; r < 16; ++r) mx = fmaxf(mx, acc[g][rb][r]);
+        best[g][rb] = fmaxf(best[g][rb], mx);
+      }
+      float rot[G];
+#pragma unroll
+      for (int j = 0; j < G; ++j) rot[j] = best[g][(j + R) % G];
+#pragma unroll
+      for (int j = 0; j < G; ++j) best[g][j] = rot[j];
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (i + 2 * step < a.ntiles) stage(buf, (i + 2 * step) * a.stride);
+    buf ^= 1;
+  }
+#pragma unroll
+  for (int g = 0; g < A; ++g) {
+    float* out = a.mins + ((size_t(q0 + g * 32 + col) * a.parts + part) * 2 + half) * G;
+#pragma unroll
+    for (int j = 0; j < G; ++j) out[j] = -best[g][j];
+  }
+}
+
+template <int KIND, int KS, int A, int R>
+__global__ __launch_bounds__(256, 2) void seed_mfma_kernel(SeedArgs a) {
+  seed_mfma_body<KIND, KS, A, R>(a, (int)(blockIdx.x % (unsigned)a.nqb), (int)(blockIdx.x / (unsigned)a.nqb));
+}
+
+// One wave per real query: the KP-th smallest of its parts * 2 * SEED_GROUPS
+// group minima (a 32-step ballot search on the order-preserving bits), one ulp
+// up so that the row attaining it still passes the scan's strict test, then
+// the shared threshold drops to it.
+__global__ __launch_bounds__(64) void seed_select_kernel(SeedArgs a) {
+  const int64_t q = blockIdx.x;
+  if (q >= a.m) return;
+  const int lane = __lane_id();
+  const int total = a.parts * 2 * SEED_GROUPS;
+  const float* v = a.mins + size_t(q) * total;
+  constexpr int VMAX = 16;  // up to 1024 minima
+  uint32_t x[VMAX];
+#pragma unroll
+  for (int r = 0; r < VMAX; ++r) {
+    const int e = r * 64 + lane;
+    x[r] = e < total ? ord_f32(v[e]) : 0xFFFFFFFFu;
+  }
+  if (total < a.KP) return;
+  uint32_t p = 0;
+  for (int b = 31; b >= 0; --b) {
+    const uint32_t probe = p | ((1u << b) - 1u);
+    int c = 0;
+#pragma unroll
+    for (int r = 0; r < VMAX; ++r) c += __popcll(__ballot(x[r] <= probe));
+    if (c < a.KP) p |= (1u << b);
+  }
+  const float t = unord_f32(p);
+  if (!(t < INFINITY)) return;  // fewer than KP non-empty groups
+  if (lane == 0) atomicMin(&a.gthr[q], ord_f32(nextafterf(t, INFINITY)));
+}
+
+template <int KIND, int KS>
+static hipError_t launch_seed_ks(const SeedArgs& s, hipStream_t st) {
+  constexpr int A = scan_groups(KS);
+  constexpr int R = scan_row_blocks(KIND, KS, false);
+  hipLaunchKernelGGL((seed_mfma_kernel<KIND, KS, A, R>), dim3((unsigned)(s.nqb * s.parts)), dim3(256), 0, st, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_seed_mfma(const SeedArgs& s, int kind, int ks, hipStream_t st) {
+  if (s.ntiles <= 0 || s.m <= 0) return hipSuccess;
+  if (s.parts * 2 * SEED_GROUPS > 1024 || s.parts * 2 * SEED_GROUPS < s.KP) return hipErrorInvalidValue;
+  hipError_t e = hipErrorInvalidValue;
+  if (kind == 0) {
+    switch (ks) {
+#define KS_CASE(K) \
+  case K:          \
+    e = launch_seed_ks<0, K>(s, st); \
+    break;
+      KS_CASE(1) KS_CASE(2) KS_CASE(3) KS_CASE(4) KS_CASE(5) KS_CASE(6) KS_CASE(7) KS_CASE(8) KS_CASE(9)
+      KS_CASE(10) KS_CASE(12) KS_CASE(14) KS_CASE(16) KS_CASE(20) KS_CASE(24) KS_CASE(28) KS_CASE(32)
+      KS_CASE(40) KS_CASE(48) KS_CASE(56) KS_CASE(64)
+#undef KS_CASE
+      default:
+        return hipErrorInvalidValue;
+    }
+  } else {
+    switch (ks) {
+#define KS_CASE(K) \
+  case K:          \
+    e = launch_seed_ks<1, K>(s, st); \
+    break;
+      KS_CASE(1) KS_CASE(2) KS_CASE(3) KS_CASE(4) KS_CASE(6) KS_CASE(8) KS_CASE(12) KS_CASE(16) KS_CASE(20)
+      KS_CASE(24) KS_CASE(32)
+#undef KS_CASE
+      default:
+        return hipErrorInvalidValue;
+    }
+  }
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(seed_select_kernel, dim3((unsigned)s.m), dim3(64), 0, st, s);
+  return hipGetLastError();
+}
+
 template <int KIND, int KS, bool RESID>
 static hipError_t launch_ks(const ScanMfmaArgs& a, int V, int nsplit, hipStream_t st) {
   constexpr int A = scan_groups(KS);

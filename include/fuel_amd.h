@@ -81,6 +81,11 @@ typedef struct fuel_amd_index fuel_amd_index;
  * fuel on such a host; every mode and entry point of the index follows the flag
  * (the batched problems of fuel_amd_batch_* always use the AVX form). */
 #define FUEL_AMD_FLAG_FALLBACK_ARITH 1
+/* FUEL_AMD_FLAG_NO_SEED: start the plain MFMA scans (FP32 / BF16) at no
+ * threshold instead of the seeded one (the KP-th smallest of per-row-group key
+ * minima over a ~1/16 row sample, computed on the matrix cores first).  Answers
+ * are the same either way; the seed only saves the scan candidate appends. */
+#define FUEL_AMD_FLAG_NO_SEED 2
 
 typedef struct fuel_amd_opts {
   int32_t mode;   /* FUEL_AMD_MODE_* */
