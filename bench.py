@@ -54,6 +54,8 @@ def parse():
     p.add_argument("--c4-line", type=int, default=1,
                    help="headline line: the 512 batched problems (plan, passes, parity on the smallest third)")
     p.add_argument("--c4-steps", type=int, default=3, help="timed passes of the batched problems")
+    p.add_argument("--tune-cache", default=None,
+                   help="file keeping the batch autotuner's choices across runs (read, then appended to)")
     p.add_argument("--calibrate", type=int, default=1,
                    help="headline line: achievable bf16 rate of a large torch matmul beside the nominal peak")
     p.add_argument("--cpu-sample", type=int, default=1000, help="queries timed on the CPU baseline (0: skip)")
@@ -336,7 +338,7 @@ def c4_run(args, threads, steps, warmup):
     probs = workloads.c4()
     print(f"# c4: {len(probs)} problems generated in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     t0 = time.time()
-    b = BatchKnn(probs, 15, mode=args.batch_mode)
+    b = BatchKnn(probs, 15, mode=args.batch_mode, tune_cache=args.tune_cache)
     tune_s = time.time() - t0
     print(f"# c4: planned in {tune_s:.1f}s", file=sys.stderr, flush=True)
     for _ in range(warmup):
@@ -464,9 +466,10 @@ def secondary(args):
             oracle.knn(probs[i], probs[i], 15, "euclidean", "for_each_knn", threads=threads)
             f_s += 2.0 * probs[i].shape[0] ** 2 * probs[i].shape[1]
         t_c = time.perf_counter() - t0
-        c4["tuning_best_ms"] = {str(dd): {m: min(t["ms"] for t in info["tuning"] if t["d"] == dd and t["mode"] == m)
-                                          for m in {t["mode"] for t in info["tuning"] if t["d"] == dd}}
-                                for dd in sorted({t["d"] for t in info["tuning"]})}
+        timed = [t for t in info["tuning"] if "ms" in t]  # cached groups carry no timing
+        c4["tuning_best_ms"] = {str(dd): {m: min(t["ms"] for t in timed if t["d"] == dd and t["mode"] == m)
+                                          for m in {t["mode"] for t in timed if t["d"] == dd}}
+                                for dd in sorted({t["d"] for t in timed})}
         c4["cpu_baseline"] = {"value": round(c4["algorithmic_flop"] / (f_s / t_c) * 1e3, 1),
                               "unit": "ms per pass (extrapolated)", "cores": threads, "kind": "port",
                               "sample": f"oracle for_each_knn on 2 mid-size problems ({t_c:.1f}s), scaled by 2 n^2 d"}

@@ -96,7 +96,7 @@ class Problem(ctypes.Structure):
 
 class BatchOpts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("widen", ctypes.c_int32), ("device", ctypes.c_int32),
-                ("no_graph", ctypes.c_int32), ("reserved", ctypes.c_int64 * 4)]
+                ("no_graph", ctypes.c_int32), ("tune_cache", ctypes.c_char_p), ("reserved", ctypes.c_int64 * 3)]
 
 
 class FuelAmdError(RuntimeError):

@@ -19,7 +19,9 @@ from . import _lib
 
 class BatchKnn:
     def __init__(self, datasets, k, metric="euclidean", exclude=_lib.EXCLUDE_FOR_EACH_KNN, mode="auto", widen=4,
-                 graph=True, device=0):
+                 graph=True, device=0, tune_cache=None):
+        """``tune_cache``: optional path of a text file that keeps the autotuner's
+        choices across processes (read here, appended to for every group tuned)."""
         import torch
         self._dev = torch.device(f"cuda:{device}")
         self._data, self._idx, self._dist, probs = [], [], [], []
@@ -41,7 +43,8 @@ class BatchKnn:
             self._dist.append(od)
             probs.append(_lib.Problem(t.data_ptr(), n, ld, d, kk, oi.data_ptr(), od.data_ptr()))
         arr = (_lib.Problem * max(1, len(probs)))(*probs)
-        opts = _lib.BatchOpts(_lib.MODES[mode], int(widen), int(device), 0 if graph else 1)
+        self._tune_cache = None if tune_cache is None else str(tune_cache).encode()
+        opts = _lib.BatchOpts(_lib.MODES[mode], int(widen), int(device), 0 if graph else 1, self._tune_cache)
         h = ctypes.c_void_p()
         _lib.check(_lib.load().fuel_amd_batch_create(arr, len(probs), _lib.metric_code(metric), int(exclude),
                                                       ctypes.byref(opts), ctypes.byref(h)))

@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -709,12 +710,63 @@ const char* kind_name(int kind) {
   return kind == 0 ? "bf16" : (kind == 1 ? "fp32" : (kind == 2 ? "bf16r" : "exact"));
 }
 
-// Times every configuration on a sample of each dimension's problems.
+// Plan cache of the autotuner.  A group's key is its dimension, metric,
+// exclusion, the device's CU count and the (size bucket, k) of every sampled
+// problem, the size bucket being floor(4 log2 n): groups of like shape share a
+// choice.  Entries live for the process and, when the caller names a file, in
+// that file (one "key<TAB>kind widen fill" line each, appended as groups are
+// tuned), so that later processes skip the timing.
+std::mutex g_tune_mu;
+std::map<std::string, Cfg> g_tune_cache;
+std::string g_tune_loaded;  // the file already read into the cache
+
+std::string tune_key(int d, int metric, int exclude, int device, const fuel_amd_problem* probs,
+                     const std::vector<int>& sample) {
+  std::string key = "d" + std::to_string(d) + "/m" + std::to_string(metric) + "/x" + std::to_string(exclude) + "/cu" +
+                    std::to_string(device_cu_count(device)) + "/";
+  for (int i : sample) {
+    const int bucket = (int)std::floor(4.0 * std::log2((double)std::max<int64_t>(1, probs[i].n)));
+    key += std::to_string(bucket) + ":" + std::to_string(probs[i].k) + ",";
+  }
+  return key;
+}
+
+void tune_cache_load(const char* path) {
+  if (!path || !*path || g_tune_loaded == path) return;
+  g_tune_loaded = path;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return;  // no file yet: it is written as groups are tuned
+  char line[4096];
+  while (std::fgets(line, sizeof(line), f)) {
+    char* tab = std::strchr(line, '\t');
+    if (!tab) continue;
+    *tab = 0;
+    Cfg c;
+    if (std::sscanf(tab + 1, "%d %d %d", &c.kind, &c.widen, &c.fill) == 3 && c.kind >= -1 && c.kind <= 2 &&
+        (c.widen == 2 || c.widen == 4 || c.widen == 8) && (c.fill == 1 || c.fill == 4))
+      g_tune_cache[line] = c;
+  }
+  std::fclose(f);
+}
+
+void tune_cache_store(const char* path, const std::string& key, const Cfg& c) {
+  g_tune_cache[key] = c;
+  if (!path || !*path) return;
+  FILE* f = std::fopen(path, "a");
+  if (!f) return;  // an unwritable cache file only costs the next process its timing
+  std::fprintf(f, "%s\t%d %d %d\n", key.c_str(), c.kind, c.widen, c.fill);
+  std::fclose(f);
+}
+
+// Times every configuration on a sample of each dimension's problems (or
+// takes the cached choice for a group of the same shape).
 int32_t autotune(const fuel_amd_problem* probs, int count, int metric, int exclude, int device,
-                 std::map<int, Cfg>& best, std::string& log) {
+                 const char* cache_path, std::map<int, Cfg>& best, std::string& log) {
   std::map<int, std::vector<int>> by_d;
   for (int i = 0; i < count; ++i)
     if (probs[i].n > 0 && probs[i].k > 0) by_d[probs[i].d].push_back(i);
+  std::lock_guard<std::mutex> cache_lock(g_tune_mu);
+  tune_cache_load(cache_path);
   log = "[";
   bool first = true;
   for (auto& kv : by_d) {
@@ -724,6 +776,17 @@ int32_t autotune(const fuel_amd_problem* probs, int count, int metric, int exclu
     std::vector<int> sample;
     const size_t step = std::max<size_t>(1, members.size() / 12);
     for (size_t j = 0; j < members.size(); j += step) sample.push_back(members[j]);
+    const std::string key = tune_key(d, metric, exclude, device, probs, sample);
+    auto hit = g_tune_cache.find(key);
+    if (hit != g_tune_cache.end()) {
+      best[d] = hit->second;
+      char buf[200];
+      std::snprintf(buf, sizeof(buf), "%s{\"d\": %d, \"mode\": \"%s\", \"widen\": %d, \"fill\": %d, \"cached\": true}",
+                    first ? "" : ", ", d, kind_name(hit->second.kind), hit->second.widen, hit->second.fill);
+      log += buf;
+      first = false;
+      continue;
+    }
     struct Trial {
       Cfg c;
       double ms;
@@ -782,6 +845,7 @@ int32_t autotune(const fuel_amd_problem* probs, int count, int metric, int exclu
       }
     }
     best[d] = best_cfg;
+    tune_cache_store(cache_path, key, best_cfg);
   }
   log += "]";
   return FUEL_AMD_OK;
@@ -820,7 +884,7 @@ int32_t fuel_amd_batch_create(const fuel_amd_problem* problems, int32_t count, i
   std::map<int, Cfg> cfg;
   std::string log = "[]";
   if (o.mode == FUEL_AMD_MODE_AUTO) {
-    rc = autotune(problems, count, metric, exclude_self, o.device, cfg, log);
+    rc = autotune(problems, count, metric, exclude_self, o.device, o.tune_cache, cfg, log);
     if (rc) return rc;
   } else {
     const int kind = o.mode == FUEL_AMD_MODE_BF16 ? 0

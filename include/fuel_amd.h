@@ -386,7 +386,12 @@ typedef struct fuel_amd_batch_opts {
   int32_t widen;    /* candidate widening with a fixed mode (0: 4) */
   int32_t device;
   int32_t no_graph; /* 1: launch directly instead of replaying the captured graph */
-  int64_t reserved[4];
+  const char* tune_cache; /* NULL, or a text file of the autotuner's choices: read at create,
+                             appended to for every group tuned, so that groups of the same
+                             shape (dimension, metric, exclusion, CU count, the sampled
+                             problems' k and size bucket floor(4 log2 n)) skip the timing in
+                             later processes; within one process choices are kept anyway */
+  int64_t reserved[3];
 } fuel_amd_batch_opts;
 
 int32_t fuel_amd_batch_create(const fuel_amd_problem* problems, int32_t count, int32_t metric, int32_t exclude_self,

@@ -86,6 +86,32 @@ def test_autotuned_plan(gpu, oracle):
     b.close()
 
 
+def test_tune_cache_file(gpu, oracle, tmp_path):
+    """A second batch of the same shape takes every group's choice from the
+    cache file the first one wrote, with the same plan and answers."""
+    from fuel_amd import BatchKnn
+    probs = _problems(53, 8, 300, 2000, [8, 32])
+    path = tmp_path / "tune.txt"
+    a = BatchKnn(probs, 5, mode="auto", tune_cache=path)
+    a.run()
+    ia = a.describe()
+    assert not any(t.get("cached") for t in ia["tuning"])
+    lines = path.read_text().splitlines()
+    assert len(lines) == 2 and all("\t" in ln for ln in lines)
+    b = BatchKnn(probs, 5, mode="auto", tune_cache=path)
+    b.run()
+    ib = b.describe()
+    assert sorted(t["d"] for t in ib["tuning"]) == [8, 32] and all(t["cached"] for t in ib["tuning"])
+    key = lambda g: (g["d"], g["mode"], g["widen"], g["fill"])
+    assert sorted(map(key, ia["groups"])) == sorted(map(key, ib["groups"]))
+    _check(oracle, probs, b.results(), 5)
+    for (x, y), (u, v) in zip(a.results(), b.results()):
+        assert np.array_equal(x, u) and y.tobytes() == v.tobytes()
+    assert len(path.read_text().splitlines()) == 2  # nothing tuned, nothing appended
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("mode", ["bf16", "bf16r"])
 @pytest.mark.parametrize("exclude,omode", [(0, "none"), (2, "python_linear")])
 def test_other_exclusion_and_metric(gpu, oracle, problems, exclude, omode, mode):
