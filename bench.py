@@ -68,7 +68,7 @@ def parse():
                         "under Poisson arrivals, dbscan on c1, kmeans distance matrix 1M x 64 vs 100 centres")
     p.add_argument("--batch-mode", default="auto", choices=["auto", "bf16", "bf16r", "fp32", "exact"],
                    help="configuration of the batched problems (c4): autotuned or fixed")
-    p.add_argument("--serve-mode", default="bf16r", choices=["bf16", "bf16r", "fp32", "exact"],
+    p.add_argument("--serve-mode", default="bf16r", choices=["bf16", "bf16r", "fp8r", "fp32", "exact"],
                    help="candidate mode of the served index (c5)")
     p.add_argument("--serve-seconds", type=float, default=4.0, help="seconds of arrivals per rate (c5)")
     p.add_argument("--serve-p99-seconds", type=float, default=3.0,
@@ -271,6 +271,29 @@ def c5_data(args):
     return X, pool, rng
 
 
+def serve_scan_roofline(ix, pool, X, mode, reps=20, m=64):
+    """The serving scan alone: `reps` launches of m queries (one serving batch)
+    through the synchronous entry point, the level-0 scan timed by HIP events
+    on its stream; HBM roofline against the algorithmic bytes of one pass over
+    the resident copy, N d s + N 8 + Q d 4 + Q k 12 (s = 1 byte for the E4M3
+    copy, 2 for bf16; 8 bytes per row of norm and scale terms)."""
+    n, d = X.shape
+    s_db = {"fp8r": 1, "bf16r": 2, "bf16": 2, "fp32": 4}.get(mode, 4)
+    q = np.ascontiguousarray(pool[:m])
+    ix.knn(q, 10)  # warm (workspaces)
+    ix.timing_enable(True)
+    for r in range(reps):
+        ix.knn(np.ascontiguousarray(pool[(r + 1) * m:(r + 2) * m]), 10)
+    kernel_ms, launches, kname = ix.timing_read()
+    ix.timing_enable(False)
+    t = kernel_ms / max(launches, 1) / 1e3
+    alg = n * d * s_db + n * 8 + m * d * 4 + m * 10 * 12
+    return {"kernel": kname, "launch_queries": m, "kernel_ms": round(t * 1e3, 4), "launches": launches,
+            "algorithmic_bytes": alg, "achieved_GBps": round(alg / t / 1e9, 1),
+            "roofline": {"bound": "hbm", "achieved": round(alg / t / 1e9, 1), "peak": 8000.0, "unit": "GB/s",
+                         "frac": round(alg / t / 8.0e12, 4)}}
+
+
 def serve_headline(args):
     """p50/p99 request latency of the serving loop on the serving configuration
     at one arrival rate (10,000 queries/s), for the headline line."""
@@ -281,8 +304,10 @@ def serve_headline(args):
     with Server(ix, workers=args.serve_workers, max_batch=4096) as srv:
         serve_run(srv, pool, 2000, 0.5, rng)  # warm-up
         r, kept = serve_run(srv, pool, 10000, args.serve_p99_seconds, rng, keep_queries=args.serve_verify)
+    scan = serve_scan_roofline(ix, pool, X, args.serve_mode)
     ix.close()
     r = dict(r)
+    r["scan"] = scan
     # served answers against the oracle, every row of the first requests
     t0 = time.perf_counter()
     same, nchk, nreq = True, 0, 0
@@ -496,6 +521,8 @@ def secondary(args):
                 srv.wait(t)
             sat = 64 * 64 / (time.perf_counter() - t1)
             sst = srv.stats(reset=True)
+        scan = serve_scan_roofline(ix, pool, X, args.serve_mode)
+        print(f"# serving scan: {scan}", file=sys.stderr, flush=True)
         # exactness of served answers: a sample of rows against the oracle
         rows = [(off, min(m, 12), idx[:12], dist[:12]) for off, m, idx, dist in kept]
         t0 = time.perf_counter()
@@ -508,7 +535,7 @@ def secondary(args):
         line.update({"config": f"serving: resident index {X.shape[0]:,} x {X.shape[1]} f32 (50-component mixture), "
                                "k=10, Poisson arrivals, geometric batch sizes (mean 64, cap 4096)",
                      "mode": args.serve_mode, "workers": args.serve_workers, "rates": rates,
-                     "saturated_qps": round(sat, 1), "saturated_p99_ms": round(sst["p99_ms"], 3),
+                     "saturated_qps": round(sat, 1), "saturated_p99_ms": round(sst["p99_ms"], 3), "scan": scan,
                      "identical_sample": same, "checked_queries": nchk,
                      "cpu_baseline": {"value": round(nchk / t_c, 2), "unit": "queries/s", "cores": threads,
                                       "kind": "port", "sample": f"{nchk} served queries re-answered by the oracle"}})

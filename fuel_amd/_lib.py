@@ -30,10 +30,12 @@ MODE_EXACT = 1
 MODE_FP32 = 2
 MODE_BF16 = 3
 MODE_BF16R = 4
+MODE_FP8R = 5
 FORM_SQUARE = 0
 FORM_CONDENSED = 1
 
-MODES = {"auto": MODE_AUTO, "exact": MODE_EXACT, "fp32": MODE_FP32, "bf16": MODE_BF16, "bf16r": MODE_BF16R}
+MODES = {"auto": MODE_AUTO, "exact": MODE_EXACT, "fp32": MODE_FP32, "bf16": MODE_BF16, "bf16r": MODE_BF16R,
+         "fp8r": MODE_FP8R}
 FLAG_FALLBACK_ARITH = 1
 FLAG_NO_SEED = 2
 # rounding of the restated distance: "avx" = fuel's AVX+FMA kernels
@@ -253,7 +255,7 @@ def metric_code(name):
 class Index:
     """Owner of a fuel_amd_index handle."""
 
-    def __init__(self, data, metric="euclidean", mode="auto", widen=4, device=0, arith="avx", seed=True):
+    def __init__(self, data, metric="euclidean", mode="auto", widen=0, device=0, arith="avx", seed=True):
         lib = load()
         data = np.ascontiguousarray(data)
         if data.ndim != 2:
@@ -268,7 +270,7 @@ class Index:
         self.handle = h
 
     @classmethod
-    def from_device(cls, ptr, n, d, ld, dtype, metric="euclidean", mode="auto", widen=4, device=0, arith="avx",
+    def from_device(cls, ptr, n, d, ld, dtype, metric="euclidean", mode="auto", widen=0, device=0, arith="avx",
                     seed=True):
         lib = load()
         self = cls.__new__(cls)

@@ -183,10 +183,10 @@ static int32_t index_create_common(const void* data, bool on_device, int64_t n, 
   if (n > 0 && !data) return fail(FUEL_AMD_ERR_INVALID, "data must not be NULL");
   fuel_amd_opts o{};
   o.mode = FUEL_AMD_MODE_AUTO;
-  o.widen = 4;
+  o.widen = 0;
   if (opts) o = *opts;
-  if (o.widen <= 0) o.widen = 4;
-  if (o.mode < FUEL_AMD_MODE_AUTO || o.mode > FUEL_AMD_MODE_BF16R) return fail(FUEL_AMD_ERR_INVALID, "bad mode");
+  if (o.widen < 0) o.widen = 0;  // 0: the mode's default (4; 16 for FP8R)
+  if (o.mode < FUEL_AMD_MODE_AUTO || o.mode > FUEL_AMD_MODE_FP8R) return fail(FUEL_AMD_ERR_INVALID, "bad mode");
   if (o.flags & ~(FUEL_AMD_FLAG_FALLBACK_ARITH | FUEL_AMD_FLAG_NO_SEED))
     return fail(FUEL_AMD_ERR_INVALID, "unknown option flags");
   int32_t rc = check_device(o.device);

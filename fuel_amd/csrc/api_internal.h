@@ -55,6 +55,7 @@ hipError_t upload_rows(void* dst, int64_t ld, const void* host, int64_t rows, in
 struct QueryCtx {
   struct LevelWs {  // one per level of the certification chain
     DevBuf Qop, Q32, qnorm, qint, slots, lcnts, fail_count, fail_list, fail_theta, gthr, qtab, qs2, seed;
+    DevBuf qfrag, qsc;  // E4M3 residual scan: per-(list, query) B fragments and scales
   } lv[3];
   DevBuf slots, lcnts;  // exact scan
   DevBuf raw, lo, rcnt;  // exact kNN in rounds (k beyond one slot)
@@ -76,6 +77,8 @@ struct QueryCtx {
       w.qtab.release();
       w.qs2.release();
       w.seed.release();
+      w.qfrag.release();
+      w.qsc.release();
     }
     slots.release();
     lcnts.release();
@@ -103,18 +106,20 @@ struct fuel_amd_index {
   fa::DevBuf hq, hout_idx, hout_dist;    // staging for host-pointer calls
   fa::DevBuf ws_a, ws_b, ws_c, ws_d, ws_e, ws_tmp;  // range / pairwise / outlier workspaces
   // mixed-precision candidate path: one operand copy per kind (0 bf16, 1 f32,
-  // 2 bf16 residual), prepared once (under prep_mu) on first use, read-only
-  // afterwards
+  // 2 bf16 residual, 3 E4M3 residual), prepared once (under prep_mu) on first
+  // use, read-only afterwards
   struct MfmaOperand {
     std::atomic<bool> ready{false};
     bool usable = true;  // false when some squared norm reaches MFMA_NORM2_LIMIT
     int ks = 0;  // K steps
     int kp = 0;  // padded K (elements)
-    int64_t n_pad = 0;  // operand rows (kind 2: lists padded to 64-row blocks)
-    fa::DevBuf Xop, hx, consts;  // operand copy, ||x||^2/2 (kind 2: of the residual), bound constants + flags
-    int L = 0;                   // kind 2: lists, centroids (L x d), position -> row, list of each
+    int64_t n_pad = 0;  // operand rows (kinds 2, 3: lists padded to 64-row blocks)
+    fa::DevBuf Xop, hx, consts;  // operand copy, ||x||^2/2 (kinds 2, 3: of the residual), bound constants + flags
+    int L = 0;                   // kinds 2, 3: lists, centroids (L x d), position -> row, list of each
     fa::DevBuf cents, rperm, tile_list, lconsts, lcount;  // 64-row block, per-list bound constants, rows
-  } mop[3];
+    fa::DevBuf xsc;              // kind 3: E8M0 row-scale codes
+    fa::DevBuf loff;             // kinds 2, 3: first position of every list
+  } mop[4];
   std::mutex prep_mu;
   // FUEL_AMD_MODE_AUTO: the mode chosen by the first kNN call's probe (-1 until
   // then), with the probe's outcome
@@ -148,6 +153,8 @@ struct fuel_amd_index {
       m.tile_list.release();
       m.lconsts.release();
       m.lcount.release();
+      m.xsc.release();
+      m.loff.release();
       m.ready = false;
     }
   }
@@ -200,8 +207,11 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
 // probe with this k on the first call): FUEL_AMD_MODE_BF16, _BF16R or _EXACT.
 int32_t resolve_auto_mode(fuel_amd_index* ix, QueryCtx* ctx, int k, hipStream_t st, int* mode);
 bool mfma_supported(const fuel_amd_index* ix, int kind, int k, int exclude);
+// Operand type of the scan for a candidate kind: 0 bf16 (kinds 0 and 2), 1 f32,
+// 2 E4M3 (kind 3).
+inline int mfma_operand_type(int kind) { return kind == 2 ? 0 : (kind == 3 ? 2 : kind); }
 // Launch plan of the next chunk of an MFMA kNN call: `remaining` of m_total
-// queries, operand type mk (0 bf16, 1 f32), residual operand or not, K steps,
+// queries, operand type mk (0 bf16, 1 f32, 2 E4M3), residual operand or not, K steps,
 // KP kept candidates, n_pad operand rows, cus compute units.
 struct MfmaChunkPlan {
   int V, NS, csub, waves, qpb, rows_tile, wg_per_cu, nsplit;
@@ -221,8 +231,9 @@ SeedPlan plan_seed(int mk, int ks, bool resid, bool has_theta, int flags, int64_
 int64_t resid_layout(const std::vector<uint32_t>& cnt, std::vector<int64_t>& pad, std::vector<int>& tl);
 // Residual operand layout of n rows (f32, pitch ld): centroids (L x d), padded
 // position -> row (RPERM_NONE for padding) and the list of every 64-row block.
+// loff receives the first position of every list (L + 1 entries).
 int32_t build_partition(const float* X, int64_t ld, int64_t n, int d, hipStream_t st, DevBuf& cents,
-                        DevBuf& rperm, DevBuf& tile_list, DevBuf& lcount, int& L, int64_t& n_rpad);
+                        DevBuf& rperm, DevBuf& tile_list, DevBuf& lcount, DevBuf& loff, int& L, int64_t& n_rpad);
 // Exact path on an explicit list of query rows (the certification fallback).
 int32_t knn_exact_rows(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t ldq, const int64_t* d_rows,
                        int64_t nrows, int64_t qbase, int k, int exclude, int64_t* out_idx, float* out_dist,

@@ -71,7 +71,7 @@ struct ProbPlan {
 // Residual layout of one problem (plan time; the graph rebuilds the operand
 // from the current rows over it).
 struct ResidPart {
-  DevBuf cents, rperm, tile_list, lcount;
+  DevBuf cents, rperm, tile_list, lcount, loff;
   int L = 0;
   int64_t n_rpad = 0;
 };
@@ -207,7 +207,8 @@ int32_t plan_batch(fuel_amd_batch* b, const std::map<int, Cfg>& cfg_by_d) {
         if (!b->parts[i]) {
           auto part = std::make_unique<ResidPart>();
           int32_t rc = build_partition(static_cast<const float*>(pr.data), pr.ld, pr.n, pr.d, b->stream, part->cents,
-                                       part->rperm, part->tile_list, part->lcount, part->L, part->n_rpad);
+                                       part->rperm, part->tile_list, part->lcount, part->loff, part->L,
+                                       part->n_rpad);
           if (rc) return rc;
           b->parts[i] = std::move(part);
         }
@@ -464,6 +465,22 @@ hipError_t enqueue_bucket(fuel_amd_batch* b, const Bucket& B, hipStream_t st) {
       la.KP = pp.KP;
       la.gthr = at<uint32_t>(b->pool, pp.o_gthr);
       if (e == hipSuccess) e = launch_prep_qlist(la, st);
+      // the sampled threshold: exact distances to the nearest list's first rows
+      SampleSeedArgs ss{};
+      ss.s2 = pw.out;
+      ss.m = pr.n;
+      ss.L = b->parts[i]->L;
+      ss.KP = pp.KP;
+      ss.lcount = la.lcount;
+      ss.loff = b->parts[i]->loff.as<int64_t>();
+      ss.rperm = b->parts[i]->rperm.as<uint32_t>();
+      ss.X = static_cast<const float*>(pr.data);
+      ss.ldx = pr.ld;
+      ss.Q32 = qa.Q32;
+      ss.ld32 = pr.ld;
+      ss.d = pr.d;
+      ss.gthr = la.gthr;
+      if (e == hipSuccess) e = launch_seed_sample(ss, st);
     }
     if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(b->fail_counts.as<int>() + i), 1, 0u, st);
     if (e != hipSuccess) return e;

@@ -475,12 +475,19 @@ int32_t knn_run(fuel_amd_index* ix, QueryCtx* ctx, const void* d_query, int64_t 
   ctx->stats = fuel_amd_stats{};
   ctx->stats.queries = m;
   ctx->fail_rows.clear();
-  int kind = mode == FUEL_AMD_MODE_BF16 ? 0 : (mode == FUEL_AMD_MODE_FP32 ? 1 : (mode == FUEL_AMD_MODE_BF16R ? 2 : -1));
+  int kind = mode == FUEL_AMD_MODE_BF16    ? 0
+             : mode == FUEL_AMD_MODE_FP32  ? 1
+             : mode == FUEL_AMD_MODE_BF16R ? 2
+             : mode == FUEL_AMD_MODE_FP8R  ? 3
+                                           : -1;
+  if (kind == 3 && !mfma_supported(ix, 3, k, exclude)) kind = 2;  // no E4M3 variant this wide
   if (kind == 2 && !mfma_supported(ix, 2, k, exclude)) kind = 0;  // no residual variant this wide
   if (kind >= 0 && !mfma_supported(ix, kind, k, exclude)) kind = -1;
   int32_t rc;
   if (kind >= 0) {
-    ctx->stats.mode_used = kind == 0 ? FUEL_AMD_MODE_BF16 : (kind == 1 ? FUEL_AMD_MODE_FP32 : FUEL_AMD_MODE_BF16R);
+    static const int kModeOfKind[4] = {FUEL_AMD_MODE_BF16, FUEL_AMD_MODE_FP32, FUEL_AMD_MODE_BF16R,
+                                       FUEL_AMD_MODE_FP8R};
+    ctx->stats.mode_used = kModeOfKind[kind];
     rc = knn_mfma(ix, ctx, (const float*)d_query, m, ldq, k, exclude, kind, d_out_idx, (float*)d_out_dist, st);
   } else if (ix->dtype == FUEL_AMD_F32) {
     ctx->stats.mode_used = FUEL_AMD_MODE_EXACT;

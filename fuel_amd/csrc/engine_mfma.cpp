@@ -2,7 +2,8 @@
 //   prep (once per index and kind) -> per query batch: prep queries -> MFMA
 //   candidate scan with fused selection -> exact rerank + certification ->
 //   the queries whose bound did not certify go one level down the chain:
-//   level 0  the requested kind (bf16 or f32), KP from the widening factor;
+//   level 0  the requested kind (bf16, f32, or bf16 / E4M3 on the residual
+//            operand), KP from the widening factor;
 //   level 1  the same kind, started at the certified threshold theta the
 //            rerank computed from the query's current KK-th exact key (every
 //            row that can still enter the answer lies below it, so the scan
@@ -25,13 +26,20 @@ namespace fa {
 namespace {
 
 constexpr size_t MFMA_SLOT_BUDGET = size_t(2) << 30;  // bytes of candidate slots per batch
-constexpr int KP_MAX = 128;
+constexpr size_t MFMA_QFRAG_BUDGET = size_t(1) << 30;  // bytes of E4M3 query fragments per batch
+constexpr int kMaxLists = 256;                          // resid_lists never exceeds it
+// Widest kept list: 128 (slots of 512 hold it with two 192-entry append
+// segments), 256 for the E4M3 copy, whose bound band is wider (two 128-entry
+// segments, still whole tiles of appends).
+constexpr int kp_max(int kind) { return kind == 3 ? 256 : 128; }
 constexpr size_t kFailRowsKept = size_t(1) << 20;  // rows of failed queries recorded per call
 
-int kp_for(const fuel_amd_index* ix, int k, int exclude) {
+// Candidates kept per query: widen * k (default 4; 16 for the E4M3 copy, whose
+// bound band is about ten times the bf16 one), at least KK, at most kp_max.
+int kp_for(const fuel_amd_index* ix, int kind, int k, int exclude) {
   const int KK = k + (exclude != EXCL_NONE ? 1 : 0);
-  const int w = ix->opts.widen > 0 ? ix->opts.widen : 4;
-  return std::max(KK, std::min(KP_MAX, w * k));
+  const int w = ix->opts.widen > 0 ? ix->opts.widen : (kind == 3 ? 16 : 4);
+  return std::max(KK, std::min(kp_max(kind), w * k));
 }
 
 // Lists of the residual operand: about 256 rows per list, at most 256.  Many
@@ -63,7 +71,8 @@ int64_t resid_layout(const std::vector<uint32_t>& cnt, std::vector<int64_t>& pad
 // Partition (Lloyd steps from evenly spaced rows) and the sorted padded layout
 // of the residual operand.
 int32_t build_partition(const float* X, int64_t ld, int64_t n, int d, hipStream_t st, DevBuf& cents,
-                        DevBuf& rperm, DevBuf& tile_list, DevBuf& lcount, int& L_out, int64_t& n_rpad_out) {
+                        DevBuf& rperm, DevBuf& tile_list, DevBuf& lcount, DevBuf& loff, int& L_out,
+                        int64_t& n_rpad_out) {
   const int L = resid_lists(n);
   DevBuf list, counts, keys_out, vals_in, vals_out, tmp, offs, poffs;
   hipError_t e = list.ensure(sizeof(uint32_t) * std::max<int64_t>(n, 1));
@@ -122,9 +131,11 @@ int32_t build_partition(const float* X, int64_t ld, int64_t n, int d, hipStream_
   e = rperm.ensure(sizeof(uint32_t) * n_rpad);
   if (e == hipSuccess) e = tile_list.ensure(sizeof(int) * htl.size());
   if (e == hipSuccess) e = lcount.ensure(sizeof(uint32_t) * L);
+  if (e == hipSuccess) e = loff.ensure(sizeof(int64_t) * (L + 1));
   if (e == hipSuccess) e = hipMemcpyAsync(lcount.p, hcnt.data(), sizeof(uint32_t) * L, hipMemcpyHostToDevice, st);
   if (e != hipSuccess) return hip_fail(e, "residual layout");
   e = hipMemcpyAsync(poffs.p, hpad.data(), sizeof(int64_t) * (L + 1), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(loff.p, hpad.data(), sizeof(int64_t) * (L + 1), hipMemcpyHostToDevice, st);
   if (e == hipSuccess)
     e = hipMemcpyAsync(tile_list.p, htl.data(), sizeof(int) * htl.size(), hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = launch_fill_u32(rperm.as<uint32_t>(), n_rpad, RPERM_NONE, st);
@@ -140,21 +151,24 @@ int32_t build_partition(const float* X, int64_t ld, int64_t n, int d, hipStream_
 
 namespace {
 
-// The residual operand over the partition: operand rows, per-list constants.
-int32_t prepare_resid(fuel_amd_index* ix, fuel_amd_index::MfmaOperand& op, hipStream_t st) {
+// The residual operand over the partition: operand rows (bf16, or E4M3 codes
+// with per-row scales), per-list constants.
+int32_t prepare_resid(fuel_amd_index* ix, fuel_amd_index::MfmaOperand& op, bool fp8, hipStream_t st) {
   const int64_t n = ix->n;
   const int d = ix->d;
-  const int ks = mfma_ksteps_bucket(0, d);
+  const int ks = mfma_ksteps_bucket(fp8 ? 2 : 0, d);
   if (ks < 0 || ks > mfma_resid_max_ksteps())
     return fail(FUEL_AMD_ERR_UNSUPPORTED, "dimension too large for the residual operand");
-  const int Kp = ks * 16;
+  const int Kp = ks * (fp8 ? 32 : 16);
   const float* X = ix->X.as<float>();
   int L = 0;
   int64_t n_rpad = 0;
-  int32_t rc = build_partition(X, ix->ld, n, d, st, op.cents, op.rperm, op.tile_list, op.lcount, L, n_rpad);
+  int32_t rc =
+      build_partition(X, ix->ld, n, d, st, op.cents, op.rperm, op.tile_list, op.lcount, op.loff, L, n_rpad);
   if (rc) return rc;
   hipError_t e = op.lconsts.ensure(sizeof(unsigned long long) * 8 * L);
-  if (e == hipSuccess) e = op.Xop.ensure(sizeof(uint16_t) * size_t(n_rpad) * Kp);
+  if (e == hipSuccess) e = op.Xop.ensure((fp8 ? 1 : sizeof(uint16_t)) * size_t(n_rpad) * Kp);
+  if (e == hipSuccess && fp8) e = op.xsc.ensure(size_t(n_rpad));
   if (e == hipSuccess) e = op.hx.ensure(sizeof(float) * n_rpad);
   if (e == hipSuccess) e = op.consts.ensure(128);
   if (e != hipSuccess) return hip_fail(e, "residual operand");
@@ -170,7 +184,9 @@ int32_t prepare_resid(fuel_amd_index* ix, fuel_amd_index::MfmaOperand& op, hipSt
   pr.tile_list = op.tile_list.as<int>();
   pr.cents = op.cents.as<float>();
   pr.ldc = d;
-  pr.Xop = op.Xop.as<uint16_t>();
+  pr.Xop = op.Xop.p;
+  pr.xsc = fp8 ? op.xsc.as<uint8_t>() : nullptr;
+  pr.fp8 = fp8 ? 1 : 0;
   pr.hx = op.hx.as<float>();
   pr.lconsts = op.lconsts.as<unsigned long long>();
   pr.gconsts = op.consts.as<unsigned long long>();
@@ -194,8 +210,8 @@ int32_t ensure_prepared(fuel_amd_index* ix, int kind, hipStream_t st) {
   if (op.ready.load(std::memory_order_acquire)) return FUEL_AMD_OK;
   std::lock_guard<std::mutex> lock(ix->prep_mu);
   if (op.ready.load(std::memory_order_relaxed)) return FUEL_AMD_OK;
-  if (kind == 2) {
-    int32_t rc = prepare_resid(ix, op, st);
+  if (kind == 2 || kind == 3) {
+    int32_t rc = prepare_resid(ix, op, kind == 3, st);
     if (rc) return rc;
     op.ready.store(true, std::memory_order_release);
     return FUEL_AMD_OK;
@@ -312,6 +328,15 @@ MfmaChunkPlan plan_mfma_chunk(int mk, bool resid, int ks, int64_t m_total, int64
     p.chunk = max_chunk;
     p.nqb = p.chunk / p.qpb;
   }
+  if (mk == 2) {
+    // the E4M3 scan reads prepared B fragments: 2 x Kp bytes per (list, query slot)
+    const size_t per_qf = size_t(2) * kMaxLists * size_t(ks) * 32;
+    const int64_t max_qf = std::max<int64_t>(p.qpb, int64_t(MFMA_QFRAG_BUDGET / per_qf) / p.qpb * p.qpb);
+    if (p.chunk > max_qf) {
+      p.chunk = max_qf;
+      p.nqb = p.chunk / p.qpb;
+    }
+  }
   p.nq_pad = p.nqb * p.qpb;
   p.rps = (n_pad + nsplit - 1) / nsplit;
   p.rps = (p.rps + p.rows_tile - 1) / p.rows_tile * p.rows_tile;
@@ -322,10 +347,10 @@ MfmaChunkPlan plan_mfma_chunk(int mk, bool resid, int ks, int64_t m_total, int64
 
 bool mfma_supported(const fuel_amd_index* ix, int kind, int k, int exclude) {
   if (ix->dtype != FUEL_AMD_F32) return false;
-  const int ks = mfma_ksteps_bucket(kind == 2 ? 0 : kind, ix->d);
-  if (ks < 0 || (kind == 2 && ks > mfma_resid_max_ksteps())) return false;
+  const int ks = mfma_ksteps_bucket(mfma_operand_type(kind), ix->d);
+  if (ks < 0 || (kind >= 2 && ks > mfma_resid_max_ksteps())) return false;
   const int KK = k + (exclude != EXCL_NONE ? 1 : 0);
-  return KK <= KP_MAX;
+  return KK <= kp_max(kind);
 }
 
 // Plain bf16 certifies when the neighbour gaps are wide next to ~2^-8 |q| |x|
@@ -403,9 +428,10 @@ int32_t prepare_mode(fuel_amd_index* ix, hipStream_t st) {
     if (rc) return rc;
   }
   if (mode == FUEL_AMD_MODE_EXACT || ix->dtype != FUEL_AMD_F32) return FUEL_AMD_OK;
-  int kind = mode == FUEL_AMD_MODE_BF16 ? 0 : (mode == FUEL_AMD_MODE_BF16R ? 2 : 1);
-  if (kind == 2 && !mfma_supported(ix, 2, 1, 0)) kind = 0;  // as knn_run
-  if (mfma_ksteps_bucket(kind == 2 ? 0 : kind, ix->d) < 0) return FUEL_AMD_OK;
+  int kind = mode == FUEL_AMD_MODE_BF16 ? 0 : (mode == FUEL_AMD_MODE_BF16R ? 2 : (mode == FUEL_AMD_MODE_FP8R ? 3 : 1));
+  if (kind == 3 && !mfma_supported(ix, 3, 1, 0)) kind = 2;  // as knn_run
+  if (kind == 2 && !mfma_supported(ix, 2, 1, 0)) kind = 0;
+  if (mfma_ksteps_bucket(mfma_operand_type(kind), ix->d) < 0) return FUEL_AMD_OK;
   int32_t rc = ensure_prepared(ix, kind, st);
   // the certification chain continues bf16 with f32
   if (rc == FUEL_AMD_OK && kind != 1 && mfma_ksteps_bucket(1, ix->d) >= 0) rc = ensure_prepared(ix, 1, st);
@@ -425,13 +451,13 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
   }
   auto& ws = ctx->lv[level];
   const int KK = k + (exclude != EXCL_NONE ? 1 : 0);
-  const int KP = kp > 0 ? std::max(kp, k + (exclude != EXCL_NONE ? 1 : 0)) : kp_for(ix, k, exclude);
+  const int KP = kp > 0 ? std::max(kp, k + (exclude != EXCL_NONE ? 1 : 0)) : kp_for(ix, kind, k, exclude);
   const int ks = op.ks;
   const int Kp = op.kp;
-  const bool resid = kind == 2;
-  const int mk = resid ? 0 : kind;  // MFMA operand type
+  const bool resid = kind >= 2;
+  const int mk = mfma_operand_type(kind);
   const int cus = device_cu_count(ix->device);
-  const size_t es = mk == 0 ? 2 : 4;
+  const size_t es = mk == 0 ? 2 : (mk == 1 ? 4 : 1);
   const int64_t n_pad = op.n_pad;
   const int64_t ld32 = ix->ld;
   const bool timed = level == 0 && ctx->timed;
@@ -445,6 +471,8 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
 
     hipError_t e = resid ? ws.qtab.ensure(sizeof(float) * size_t(op.L) * nq_pad) : ws.Qop.ensure(size_t(nq_pad) * Kp * es);
     if (resid && e == hipSuccess) e = ws.qs2.ensure(sizeof(float) * size_t(op.L) * chunk);
+    if (kind == 3 && e == hipSuccess) e = ws.qfrag.ensure(size_t(2) * op.L * size_t(nq_pad) * Kp);
+    if (kind == 3 && e == hipSuccess) e = ws.qsc.ensure(sizeof(int) * size_t(op.L) * nq_pad);
     if (e == hipSuccess) e = ws.Q32.ensure(size_t(chunk) * ld32 * 4);
     if (e == hipSuccess) e = ws.qnorm.ensure(sizeof(double) * 3 * size_t(chunk));
     if (e == hipSuccess) e = ws.qint.ensure(sizeof(int) * size_t(chunk));
@@ -498,12 +526,46 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
       la.nq_pad = nq_pad;
       la.d = ix->d;
       la.L = op.L;
+      la.fp8 = kind == 3 ? 1 : 0;
       la.lconsts = op.lconsts.as<unsigned long long>();
       la.qtab = ws.qtab.as<float>();
       la.lcount = op.lcount.as<uint32_t>();
       la.KP = KP;
       la.gthr = ws.gthr.as<uint32_t>();
       if (e == hipSuccess) e = launch_prep_qlist(la, st);
+      if (kind == 3 && e == hipSuccess) {
+        QFragArgs fa{};
+        fa.Q32 = qa.Q32;
+        fa.ld32 = ld32;
+        fa.m = chunk;
+        fa.nq_pad = nq_pad;
+        fa.d = ix->d;
+        fa.cents = op.cents.as<float>();
+        fa.ldc = ix->d;
+        fa.L = op.L;
+        fa.Kp = Kp;
+        fa.qfrag = ws.qfrag.as<uint8_t>();
+        fa.qsc = ws.qsc.as<int>();
+        e = launch_prep_qfrag_fp8(fa, st);
+      }
+      if (!d_theta && !(ix->opts.flags & FUEL_AMD_FLAG_NO_SEED) && e == hipSuccess) {
+        // a first level: exact distances to a sample of each query's nearest list
+        SampleSeedArgs sd{};
+        sd.s2 = ws.qs2.as<float>();
+        sd.m = chunk;
+        sd.L = op.L;
+        sd.KP = KP;
+        sd.lcount = op.lcount.as<uint32_t>();
+        sd.loff = op.loff.as<int64_t>();
+        sd.rperm = op.rperm.as<uint32_t>();
+        sd.X = ix->X.as<float>();
+        sd.ldx = ix->ld;
+        sd.Q32 = qa.Q32;
+        sd.ld32 = ld32;
+        sd.d = ix->d;
+        sd.gthr = ws.gthr.as<uint32_t>();
+        e = launch_seed_sample(sd, st);
+      }
     }
     if (e == hipSuccess) e = launch_fill_u32(ws.fail_count.as<uint32_t>(), 1, 0u, st);
     if (e != hipSuccess) return hip_fail(e, "query preparation");
@@ -565,6 +627,10 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
       sa.m = chunk;
       sa.d = ix->d;
       sa.qtab = ws.qtab.as<float>();
+      sa.xsc = kind == 3 ? op.xsc.as<uint8_t>() : nullptr;
+      sa.qfrag = kind == 3 ? ws.qfrag.as<uint8_t>() : nullptr;
+      sa.qsc = kind == 3 ? ws.qsc.as<int>() : nullptr;
+      sa.L = op.L;
     }
     {
       // refuse a launch whose addresses would leave the buffers it was given
@@ -579,13 +645,19 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
       ex.cents = resid ? op.cents.bytes : 0;
       ex.q32 = ws.Q32.bytes;
       ex.qtab = resid ? ws.qtab.bytes : 0;
+      ex.xsc = kind == 3 ? op.xsc.bytes : 0;
+      ex.qfrag = kind == 3 ? ws.qfrag.bytes : 0;
+      ex.qsc = kind == 3 ? ws.qsc.bytes : 0;
       std::string why;
       if (scan_check_launch(sa, mk, ks, V, nsplit, ex, resid ? op.L : 0, &why))
         return fail(FUEL_AMD_ERR_DEVICE, "internal error: MFMA scan plan outside its buffers: " + why);
     }
     if (timed)
       timing_mark(ix, st, true,
-                  kind == 0 ? "scan_mfma_kernel<bf16>" : (kind == 1 ? "scan_mfma_kernel<f32>" : "scan_mfma_kernel<bf16r>"));
+                  kind == 0   ? "scan_mfma_kernel<bf16>"
+                  : kind == 1 ? "scan_mfma_kernel<f32>"
+                  : kind == 2 ? "scan_mfma_kernel<bf16r>"
+                              : "scan_mfma_kernel<fp8r>");
     e = launch_scan_mfma(sa, mk, ks, V, nsplit, st);
     if (timed) timing_mark(ix, st, false, nullptr);
     if (e != hipSuccess) return hip_fail(e, "MFMA scan launch");
@@ -644,7 +716,7 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
         // (||q - x||^2 / 2) are not those of the f32 form
         const float* theta = (next_kind == kind || !resid) ? ra.fail_theta : nullptr;
         rc = knn_mfma(ix, ctx, Qb, nfail, ldq, k, exclude, next_kind, ra.out_idx, ra.out_dist, st, ra.fail_list,
-                      ra.qbase, level + 1, KP_MAX, theta);
+                      ra.qbase, level + 1, kp_max(next_kind), theta);
       } else {
         ctx->stats.exact_queries += nfail;  // queries answered by the exact scan
         rc = knn_exact_rows(ix, ctx, Qb, ldq, ra.fail_list, nfail, ra.qbase, k, exclude, ra.out_idx, ra.out_dist,
@@ -679,21 +751,21 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
 // check can be seen to catch it.  Returns the number of plans flagged.
 int64_t check_scan_plans(int perturb, int64_t* launches, std::string* first) {
   int64_t bad = 0, count = 0;
-  const int KPs[] = {10, 11, 16, 20, 40, 64, 65, 128};
+  const int KPs[] = {10, 11, 16, 20, 40, 64, 65, 128, 160, 256};
   const int64_t ms[] = {1, 63, 64, 65, 192, 193, 257, 4097, 100000};
   const int64_t ns[] = {300, 65536 + 13, 1000000, 10000000};
   const int cus_list[] = {256, 1};
-  for (int variant = 0; variant < 3; ++variant) {
-    const int mk = variant == 1 ? 1 : 0;
-    const bool resid = variant == 2;
+  for (int variant = 0; variant < 4; ++variant) {
+    const int mk = variant == 1 ? 1 : (variant == 3 ? 2 : 0);  // variant 3: the E4M3 residual operand
+    const bool resid = variant >= 2;
     for (int dd = 1; dd <= (mk == 0 ? 1024 : 256) + 1; ++dd) {
       // the widest and the narrowest dimension of every K step bucket
       const int ks = mfma_ksteps_bucket(mk, dd);
       if (ks < 0 || (resid && ks > mfma_resid_max_ksteps())) continue;
       if (mfma_ksteps_bucket(mk, dd - 1) == ks && mfma_ksteps_bucket(mk, dd + 1) == ks) continue;
       const int d = dd;
-      const int Kp = ks * (mk == 0 ? 16 : 8);
-      const size_t es = mk == 0 ? 2 : 4;
+      const int Kp = ks * (mk == 0 ? 16 : (mk == 1 ? 8 : 32));
+      const size_t es = mk == 0 ? 2 : (mk == 1 ? 4 : 1);
       const int64_t ld32 = (d + 7) & ~int64_t(7);
       for (int64_t n : ns) {
         int L = 0;
@@ -742,6 +814,12 @@ int64_t check_scan_plans(int perturb, int64_t* launches, std::string* first) {
                   ex.cents = size_t(L) * d * 4;
                   ex.qtab = size_t(L) * pl.nq_pad * 4;
                 }
+                if (mk == 2) {
+                  sa.L = L;
+                  ex.xsc = size_t(n_pad);
+                  ex.qfrag = size_t(2) * L * pl.nq_pad * Kp;
+                  ex.qsc = size_t(L) * pl.nq_pad * 4;
+                }
                 int nsplit = pl.nsplit;
                 const size_t need_slots = size_t(pl.nsplit) * pl.nq_pad * pl.NS * 8;
                 switch (perturb) {
@@ -762,6 +840,14 @@ int64_t check_scan_plans(int perturb, int64_t* launches, std::string* first) {
                     break;
                   case 12:
                     if (resid) ex.cents -= 4; else ex.lcnts -= 4;
+                    break;
+                  case 13:  // E4M3: row scales, fragments or their scales short; others as case 1
+                    if (mk == 2) {
+                      const int which = int(count % 3);
+                      if (which == 0) ex.xsc -= 1; else if (which == 1) ex.qfrag -= 16; else ex.qsc -= 4;
+                    } else {
+                      ex.slots = need_slots - 8;
+                    }
                     break;
                   default: break;
                 }

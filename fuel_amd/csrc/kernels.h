@@ -284,6 +284,10 @@ struct ScanMfmaArgs {
   int d;
   const float* qtab;       // [L][nq_pad]
   int waves;               // waves per workgroup: 4, or 1 (see mfma_one_wave_supported); 0 means 4
+  const uint8_t* xsc;      // E4M3 operand (kind 2): n_pad E8M0 row-scale codes
+  const uint8_t* qfrag;    // E4M3: [2][L][nq_pad][Kp] hi, then lo codes of q - c_j (prep_qfrag_fp8)
+  const int* qsc;          // E4M3: [L][nq_pad] E8M0 code of the hi scale
+  int L;                   // E4M3: lists
 };
 
 // ---- residual (centroid-relative) operand ------------------------------------
@@ -331,7 +335,9 @@ struct PrepResidArgs {
   const int* tile_list;
   const float* cents;
   int ldc;
-  uint16_t* Xop;  // n_rpad x Kp bf16
+  void* Xop;      // n_rpad x Kp: bf16, or E4M3 codes of r 2^-e_p (fp8)
+  uint8_t* xsc;   // fp8: n_rpad E8M0 codes of the per-row scales 2^e_p
+  int fp8;
   float* hx;
   unsigned long long* lconsts;  // L x 8, zero before the launch
   unsigned long long* gconsts;  // [0] max ||x||^2 (the sentinel-range check); zero before the launch
@@ -355,6 +361,7 @@ struct QListArgs {
   int64_t m;
   int64_t nq_pad;
   int d;
+  int fp8;          // the operand is the E4M3 copy, its B operand the hi + lo E4M3 split of q - c_j
   int L;
   const unsigned long long* lconsts;
   float* qtab;  // [L][nq_pad]
@@ -363,6 +370,48 @@ struct QListArgs {
   uint32_t* gthr;          // [nq_pad], initialised by the query preparation
 };
 hipError_t launch_prep_qlist(const QListArgs& a, hipStream_t st);
+// Sampled threshold of a residual scan, one wave per real query: the nearest
+// list with at least KP rows (smallest s2, ties to the smaller index), its
+// first min(rows, SEED_SAMPLE_ROWS) rows, each row's ||q - x||^2 / 2 in f64
+// raised to an upper bound of its scan key (x (1 + 2^-16) + 2^-140: the key is
+// a lower bound of that value up to the row tag), and the KP-th smallest of
+// them one float step up: at least KP rows keep a key below it, so the scan
+// drops nothing its lists need, and gthr falls to it when lower.
+constexpr int SEED_SAMPLE_ROWS = 512;
+struct SampleSeedArgs {
+  const float* s2;          // m x L, ||q - c_j||^2
+  int64_t m;
+  int L;
+  int KP;
+  const uint32_t* lcount;   // rows per list
+  const int64_t* loff;      // first position of every list (L + 1)
+  const uint32_t* rperm;    // position -> row
+  const float* X;           // rows, pitch ldx
+  int64_t ldx;
+  const float* Q32;         // queries, pitch ld32
+  int64_t ld32;
+  int d;
+  uint32_t* gthr;
+};
+hipError_t launch_seed_sample(const SampleSeedArgs& a, hipStream_t st);
+// B fragments of the E4M3 residual scan, one wave per (list j, query slot q;
+// slots >= m repeat query m - 1): dq = fl32(q - c_j), e = fp8_scale_exp(max
+// |dq|), hi = e4m3(dq 2^-e), lo = e4m3((dq - hi 2^e) 2^-(e-4)); byte k of a
+// fragment row is coordinate k (zero beyond d), as in the operand rows.
+struct QFragArgs {
+  const float* Q32;
+  int64_t ld32;
+  int64_t m;
+  int64_t nq_pad;
+  int d;
+  const float* cents;
+  int ldc;
+  int L;
+  int Kp;          // bytes per fragment row (a multiple of 64)
+  uint8_t* qfrag;  // [2][L][nq_pad][Kp]
+  int* qsc;        // [L][nq_pad]
+};
+hipError_t launch_prep_qfrag_fp8(const QFragArgs& a, hipStream_t st);
 
 struct RerankArgs {
   const uint64_t* slots;
@@ -422,20 +471,22 @@ struct ScanBatchArgs {
 
 hipError_t launch_prep_db(const PrepDbArgs& a, hipStream_t st);
 hipError_t launch_prep_queries(const PrepQArgs& a, hipStream_t st);
-// kind 0: bf16 MFMA, 1: f32 MFMA.  Returns hipErrorInvalidValue for an
-// unsupported K step count.
+// kind 0: bf16 MFMA, 1: f32 MFMA, 2: E4M3 MFMA (residual operand only, with
+// its row scales in a.xsc).  Returns hipErrorInvalidValue for an unsupported K
+// step count.
 hipError_t launch_scan_mfma(const ScanMfmaArgs& a, int kind, int ksteps, int V, int nsplit, hipStream_t st);
 hipError_t launch_rerank(const RerankArgs& a, int V, hipStream_t st);
 // grid = the number of persistent workgroups
 hipError_t launch_scan_mfma_batch(const ScanBatchArgs& b, int kind, int ksteps, int V, int grid, bool resid,
                                   hipStream_t st);
 // Supported K step counts of the MFMA scan (bf16: 16 dims per step; f32: 8 dims
-// per step group); the padded K is steps * 16 (bf16) or steps * 8 (f32).
+// per step group; E4M3: 32 dims per step, an even count); the padded K is
+// steps * 16 (bf16), steps * 8 (f32) or steps * 32 (E4M3).
 int mfma_ksteps_bucket(int kind, int d);
 // Largest K step count with a residual-operand variant of the bf16 scan.
 int mfma_resid_max_ksteps();
 // Launch shape: waves per workgroup is 4, or 1 where mfma_one_wave_supported
-// (residual bf16 scans with V = 4 and at most 12 K steps; small batches).
+// (residual bf16 or E4M3 scans with V = 4 and at most 12 K steps; small batches).
 int mfma_queries_per_block(int kind, int ksteps, int waves = 4);
 int mfma_rows_per_tile(int kind, int ksteps, bool resid);
 int mfma_workgroups_per_cu(int kind, int ksteps, int waves, bool resid);
@@ -470,7 +521,7 @@ hipError_t launch_seed_mfma(const SeedArgs& s, int kind, int ksteps, hipStream_t
 
 // Bytes addressable from each pointer of a ScanMfmaArgs.
 struct ScanExtents {
-  size_t xop, hx, qop, slots, lcnts, gthr, tile_list, cents, q32, qtab;
+  size_t xop, hx, qop, slots, lcnts, gthr, tile_list, cents, q32, qtab, xsc, qfrag, qsc;
 };
 // Host-side check of one scan launch with the arguments launch_scan_mfma (or
 // one problem of the persistent form) receives: every global address the

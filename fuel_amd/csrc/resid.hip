@@ -13,6 +13,7 @@
 // the scan's keys are lower bounds of ||q - x||^2 / 2 (up to the row tag).
 // The partition only affects speed: every answer is still certified against
 // the exact restated distances.
+#include "fp8.h"
 #include "kernels.h"
 
 namespace fa {
@@ -148,10 +149,19 @@ hipError_t launch_iota_u32(uint32_t* p, int64_t n, hipStream_t st) {
   return hipGetLastError();
 }
 
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fmaxf(v, __shfl_xor(v, m));
+  return v;
+}
+
 // One workgroup per 64-row block (every block lies in one list): each wave
 // prepares 16 positions, the per-list maxima are reduced in the workgroup and
 // published with one atomic per constant (a per-row atomic on the few
-// addresses of a list serialises).
+// addresses of a list serialises).  FP8: position p holds the E4M3 codes of
+// fl32(x - c_j) 2^-e_p, e_p = fp8_scale_exp of the row's largest |coordinate|,
+// and xsc[p] the E8M0 code of 2^e_p; r~ is the dequantized row.
+template <bool FP8>
 __global__ __launch_bounds__(256) void prep_resid_kernel(PrepResidArgs a) {
   __shared__ double red[4][6];
   const int wave = threadIdx.x >> 6;
@@ -163,22 +173,45 @@ __global__ __launch_bounds__(256) void prep_resid_kernel(PrepResidArgs a) {
   double m[6] = {0, 0, 0, 0, 0, 0};  // ||r||^2, ||r~-r||^2, ||r~||^2, hxn error, fl32(||r||^2/2), ||x||^2
   for (int i = 0; i < 16; ++i) {
     const int64_t pos = blk * 64 + wave * 16 + i;
-    uint16_t* dst = a.Xop + size_t(pos) * a.Kp;
+    uint16_t* dst = static_cast<uint16_t*>(a.Xop) + size_t(pos) * a.Kp;
+    uint8_t* dst8 = static_cast<uint8_t*>(a.Xop) + size_t(pos) * a.Kp;
     const uint32_t row = a.rperm[pos];
     if (row == RPERM_NONE) {
-      for (int c = lane; c < a.Kp; c += 64) dst[c] = 0;
-      if (lane == 0) a.hx[pos] = MFMA_PAD_HXN;
+      for (int c = lane; c < a.Kp; c += 64) {
+        if constexpr (FP8)
+          dst8[c] = 0;
+        else
+          dst[c] = 0;
+      }
+      if (lane == 0) {
+        a.hx[pos] = MFMA_PAD_HXN;
+        if constexpr (FP8) a.xsc[pos] = (uint8_t)fp8_scale_code(0);
+      }
       continue;
     }
     const float* x = a.X + int64_t(row) * a.ldx;
+    int e = 0;
+    if constexpr (FP8) {
+      float mx = 0.0f;
+      for (int c = lane; c < a.d; c += 64) mx = fmaxf(mx, fabsf((float)((double)x[c] - (double)cj[c])));
+      e = fp8_scale_exp(wave_max(mx));
+      if (lane == 0) a.xsc[pos] = (uint8_t)fp8_scale_code(e);
+    }
     double s2 = 0.0, e2 = 0.0, t2 = 0.0, x2 = 0.0;
     for (int c = lane; c < a.Kp; c += 64) {
       const double xv = c < a.d ? (double)x[c] : 0.0;
       x2 += xv * xv;
       const double r = c < a.d ? xv - (double)cj[c] : 0.0;
-      const uint16_t h = bf16_rne((float)r);
-      dst[c] = h;
-      const double rt = (double)bf16_val(h);
+      double rt;
+      if constexpr (FP8) {
+        const uint32_t code = fp8_e4m3_encode((float)r, e);
+        dst8[c] = (uint8_t)code;
+        rt = ldexp((double)fp8_e4m3_value(code), e);
+      } else {
+        const uint16_t h = bf16_rne((float)r);
+        dst[c] = h;
+        rt = (double)bf16_val(h);
+      }
       s2 += r * r;
       e2 += (rt - r) * (rt - r);
       t2 += rt * rt;
@@ -215,7 +248,11 @@ __global__ __launch_bounds__(256) void prep_resid_kernel(PrepResidArgs a) {
 hipError_t launch_prep_resid(const PrepResidArgs& a, hipStream_t st) {
   if (a.n_rpad == 0) return hipSuccess;
   if (a.n_rpad % 64 != 0) return hipErrorInvalidValue;  // lists are whole 64-row blocks
-  hipLaunchKernelGGL(prep_resid_kernel, dim3((unsigned)(a.n_rpad / 64)), dim3(256), 0, st, a);
+  if (a.fp8 && (!a.xsc || a.Kp % 64 != 0)) return hipErrorInvalidValue;  // whole 64-byte MFMA steps
+  if (a.fp8)
+    hipLaunchKernelGGL(prep_resid_kernel<true>, dim3((unsigned)(a.n_rpad / 64)), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(prep_resid_kernel<false>, dim3((unsigned)(a.n_rpad / 64)), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
@@ -233,6 +270,14 @@ hipError_t launch_prep_resid(const PrepResidArgs& a, hipStream_t st) {
 //   underflow          2 (d + 2) 2^-126
 // e solves e >= base + (4 (d + 2) u + u) e; the term is -Q2lo + e rounded up,
 // Q2lo <= ||q - c||^2 / 2 <= Q2.
+// FP8 operand: b~ = hi 2^e + lo 2^(e-4) with hi = e4m3(dq 2^-e), lo =
+// e4m3((dq - hi 2^e) 2^-(e-4)), dq = fl32(q - c) and 2^e <= max |dq| / 224
+// (or 2^-100).  Per coordinate |dq - hi 2^e| <= 2^-4 |dq| + 2^(e-10) and the lo
+// rounding adds 2^-4 of that plus 2^(e-14), so |dq - b~| <= 2^-8 |dq| + 2^(e-13):
+//   EA = N (2^-8 + 2^-23 + sqrt(d) 2^-13 / 224) + sqrt(d) 2^-112
+// (2^-23 covers the f32 subtraction; the last term a clamped scale and flushed
+// subnormal differences).  The MFMA sums 2d products against hi and lo, with
+// sum |a||b| <= RT (||hi|| + ||lo||) <= RT (N + EA)(1 + 2^-3).
 __global__ __launch_bounds__(256) void prep_qlist_kernel(QListArgs a) {
   const int64_t w = int64_t(blockIdx.x) * 256 + threadIdx.x;
   if (w >= int64_t(a.L) * a.nq_pad) return;
@@ -252,13 +297,16 @@ __global__ __launch_bounds__(256) void prep_qlist_kernel(QListArgs a) {
   const double EH = __longlong_as_double((long long)lc[3]);
   const double H = __longlong_as_double((long long)lc[4]);
   const double N = sqrt(s_hi) * (1 + 0x1p-50);
-  const double EA = N * (0x1p-9 + 0x1p-23) * (1 + 0x1p-40);
-  const double B = (N + EA) * (1 + 0x1p-50);
+  const double sd = sqrt((double)d) * (1 + 0x1p-50);
+  const double EA = a.fp8 ? (N * (0x1p-8 + 0x1p-23 + sd * 0x1p-13 / 224.0) + sd * 0x1p-112) * (1 + 0x1p-40)
+                          : N * (0x1p-9 + 0x1p-23) * (1 + 0x1p-40);
+  const double B = a.fp8 ? (N + EA) * (1 + 0x1p-3) * (1 + 0x1p-40) : (N + EA) * (1 + 0x1p-50);
   const double Q2 = 0.5 * s_hi;
   const double Q2lo = 0.5 * s_lo;
-  const double gacc = 4.0 * (d + 2) * u;
+  const int terms = a.fp8 ? 2 * d : d;
+  const double gacc = 4.0 * (terms + 2) * u;
   const double base = N * EX + EA * RN + EA * EX + EH + gacc * (B * RT + H + Q2) + u * (H + Q2) +
-                      2.0 * (d + 2) * 0x1p-126;
+                      2.0 * (terms + 2) * 0x1p-126;
   const double e = base / (1.0 - gacc - u) * (1 + 0x1p-40) + 0x1p-140;
   const double t = -Q2lo + e;
   float tf = (float)t;
@@ -291,6 +339,105 @@ __global__ __launch_bounds__(256) void seed_theta_kernel(QListArgs a) {
     const uint32_t o = ord_f32(t);
     if (o < a.gthr[q]) a.gthr[q] = o;
   }
+}
+
+__global__ __launch_bounds__(256) void prep_qfrag_fp8_kernel(QFragArgs a) {
+  const int64_t w = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (w >= int64_t(a.L) * a.nq_pad) return;
+  const int lane = __lane_id();
+  const int j = (int)(w / a.nq_pad);
+  const int64_t q = w - int64_t(j) * a.nq_pad;
+  const float* qv = a.Q32 + (q < a.m ? q : a.m - 1) * a.ld32;
+  const float* cj = a.cents + size_t(j) * a.ldc;
+  float mx = 0.0f;
+  for (int c = lane; c < a.d; c += 64) mx = fmaxf(mx, fabsf(qv[c] - cj[c]));
+  const int e = fp8_scale_exp(wave_max(mx));
+  uint8_t* hi = a.qfrag + size_t(w) * a.Kp;
+  uint8_t* lo = hi + size_t(a.L) * a.nq_pad * a.Kp;
+  for (int c = lane; c < a.Kp; c += 64) {
+    const float v = c < a.d ? qv[c] - cj[c] : 0.0f;
+    const uint32_t ch = fp8_e4m3_encode(v, e);
+    const float res = v - ldexpf(fp8_e4m3_value(ch), e);  // exact: a multiple of v's unit, |res| <= |v|
+    hi[c] = (uint8_t)ch;
+    lo[c] = (uint8_t)fp8_e4m3_encode(res, e - 4);
+  }
+  if (lane == 0) a.qsc[w] = (int)fp8_scale_code(e);
+}
+
+hipError_t launch_prep_qfrag_fp8(const QFragArgs& a, hipStream_t st) {
+  const int64_t cnt = int64_t(a.L) * a.nq_pad;
+  if (cnt == 0 || a.m == 0) return hipSuccess;
+  if (a.Kp % 64 != 0 || a.d > a.Kp) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(prep_qfrag_fp8_kernel, dim3((unsigned)((cnt + 3) / 4)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void seed_sample_kernel(SampleSeedArgs a) {
+  constexpr int RPL = SEED_SAMPLE_ROWS / 64;  // sample rows per lane
+  const int64_t q = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (q >= a.m) return;
+  const int lane = __lane_id();
+  const float* s2 = a.s2 + q * a.L;
+  float best = INFINITY;
+  int bj = a.L;  // none
+  for (int j = lane; j < a.L; j += 64)
+    if (a.lcount[j] >= (uint32_t)a.KP && s2[j] < best) {
+      best = s2[j];
+      bj = j;
+    }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    const float ov = __shfl_xor(best, m);
+    const int oj = __shfl_xor(bj, m);
+    if (ov < best || (ov == best && oj < bj)) {
+      best = ov;
+      bj = oj;
+    }
+  }
+  if (bj >= a.L) return;  // wave-uniform: no list holds KP rows
+  const uint32_t cnt = a.lcount[bj];
+  const int S = cnt < (uint32_t)SEED_SAMPLE_ROWS ? (int)cnt : SEED_SAMPLE_ROWS;
+  const int64_t p0 = a.loff[bj];  // positions p0 .. p0 + cnt - 1 are the list's rows
+  const float* qv = a.Q32 + q * a.ld32;
+  uint32_t key[RPL];
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) {
+    const int p = r * 64 + lane;
+    uint32_t o = 0xFFFFFFFFu;
+    if (p < S) {
+      const float* x = a.X + int64_t(a.rperm[p0 + p]) * a.ldx;
+      double s = 0.0;
+      for (int c = 0; c < a.d; ++c) {
+        const double t = (double)qv[c] - (double)x[c];
+        s = fma(t, t, s);
+      }
+      const double v = 0.5 * s * (1 + 0x1p-16) + 0x1p-140;
+      float f = (float)v;
+      if ((double)f < v) f = nextafterf(f, INFINITY);
+      o = ord_f32(f);
+    }
+    key[r] = o;
+  }
+  uint32_t p = 0;  // the KP-th smallest bound: smallest t with #(key <= t) >= KP
+  for (int b = 31; b >= 0; --b) {
+    const uint32_t probe = p | ((1u << b) - 1u);
+    int c = 0;
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) c += __popcll(__ballot(key[r] <= probe));
+    if (c < a.KP) p |= (1u << b);
+  }
+  const float t = unord_f32(p);
+  if (lane == 0 && t < MFMA_KEY_LIMIT) {
+    const uint32_t o = ord_f32(nextafterf(t, INFINITY));
+    if (o < a.gthr[q]) a.gthr[q] = o;
+  }
+}
+
+hipError_t launch_seed_sample(const SampleSeedArgs& a, hipStream_t st) {
+  if (a.m <= 0 || a.L <= 0) return hipSuccess;
+  if (a.KP < 1 || a.KP > SEED_SAMPLE_ROWS) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(seed_sample_kernel, dim3((unsigned)((a.m + 3) / 4)), dim3(256), 0, st, a);
+  return hipGetLastError();
 }
 
 hipError_t launch_prep_qlist(const QListArgs& a, hipStream_t st) {

@@ -34,6 +34,14 @@
 // KP-th key is the new threshold.  Thresholds are shared by the splits of a
 // query through a per-query atomicMin; every tile's copy brings the values the
 // other splits published.
+//
+// E4M3 operand (KIND 2, residual only): rows hold the E4M3 codes of r 2^-e_x
+// with a per-row E8M0 scale 2^e_x staged beside the tile; the query side is
+// q - c_j split into hi 2^e_q + lo 2^(e_q - 4), both E4M3.  Two 32-byte steps
+// form one v_mfma_scale_f32_32x32x64_f8f6f4 operand (the lane half supplies
+// its 16 bytes of each), issued once against hi and once against lo with the
+// row's and the query's scales: lanes l and l + 32 carry the same scale, so
+// the product is sa sb sum a b whatever K block a byte belongs to.
 #include <algorithm>
 #include <cstdio>
 #include <map>
@@ -42,11 +50,13 @@
 #include <vector>
 
 #include "common.h"
+#include "fp8.h"
 #include "kernels.h"
 
 namespace fa {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef int intx8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 template <int KIND>
@@ -202,6 +212,8 @@ constexpr int scan_tile_bytes(int ks, int R, int W) { return scan_wave_loads(ks,
 constexpr int scan_lds_bytes(int ks, int R, int W, bool batch) {
   return 2 * scan_tile_bytes(ks, R, W) + 2 * 64 * 4 + 2 * W * 64 * 4 + (batch ? 16 : 0);
 }
+// the E4M3 operand's row scales: 64 words per buffer
+constexpr int scan_lds_extra(int kind) { return kind == 2 ? 2 * 64 * 4 : 0; }
 constexpr int kLdsPerCu = 160 * 1024;  // gfx950
 constexpr int kLdsGranule = 512;       // allocation granularity per workgroup
 
@@ -237,6 +249,7 @@ __host__ __device__ __forceinline__ size_t scan_slot_index(int split, int64_t nq
 
 template <int KIND, int KS, int A, int R, int V, bool RESID = false, int W = 4>
 __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int qb, const int split) {
+  static_assert(KIND != 2 || (RESID && KS % 2 == 0), "the E4M3 operand is residual, in whole 64-byte steps");
   using C = Comp<float>;
   constexpr int NS = 64 * V;
   constexpr int QPW = 32 * A;
@@ -251,6 +264,7 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
   __shared__ __attribute__((aligned(16))) char tile[2][TILE_BYTES];
   __shared__ __attribute__((aligned(16))) float tile_hx[2][64];      // one dword per lane of wave 0
   __shared__ __attribute__((aligned(16))) uint32_t thr_lds[2][W][64];  // published thresholds, per wave
+  __shared__ __attribute__((aligned(16))) uint32_t tile_sc[2][KIND == 2 ? 64 : 1];  // E8M0 row scales (E4M3)
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -268,9 +282,14 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
   // query fragments, resident for the whole scan (residual operand: rebuilt at
   // every change of list, below)
   uint4 qf[A][KS];
+  uint4 qlo[KIND == 2 ? A : 1][KIND == 2 ? KS : 1];  // E4M3: the lo part of q - c_j
+  int qsh[A], qsl[A];                                 // E4M3: E8M0 codes of the hi and lo scales
   float qt[A];
 #pragma unroll
-  for (int g = 0; g < A; ++g) qt[g] = 0.0f;
+  for (int g = 0; g < A; ++g) {
+    qt[g] = 0.0f;
+    qsh[g] = qsl[g] = 127;
+  }
   int cur_list = -1;
   if constexpr (!RESID) {
 #pragma unroll
@@ -298,6 +317,7 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
   const uint32_t lds_tile = (uint32_t)(uintptr_t)&tile[0][0];
   const uint32_t lds_hx = (uint32_t)(uintptr_t)&tile_hx[0][0];
   const uint32_t lds_thr = (uint32_t)(uintptr_t)&thr_lds[0][0][0];
+  const uint32_t lds_sc = (uint32_t)(uintptr_t)&tile_sc[0][0];
   const uint32_t* gthr_src = a.gthr + q0 + (lane < QPW ? lane : QPW - 1);
   auto stage = [&](int b, int64_t t) {
     const char* base = xop + size_t(t) * kp_bytes;
@@ -307,6 +327,11 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
     if (wave == 0) {
       const int r = lane < BN ? lane : BN - 1;
       lds_dma4(__builtin_amdgcn_readfirstlane(lds_hx + b * 64 * 4), a.hx + t + r);
+      if constexpr (KIND == 2) {
+        // the BN scale bytes of the tile, a word per lane (lanes past BN/4 repeat the last)
+        const int w = lane < BN / 4 ? lane : BN / 4 - 1;
+        lds_dma4(__builtin_amdgcn_readfirstlane(lds_sc + b * 64 * 4), reinterpret_cast<const uint32_t*>(a.xsc + t) + w);
+      }
     }
     // the thresholds other splits published for this wave's queries ride along
     lds_dma4(__builtin_amdgcn_readfirstlane(lds_thr + (b * W + wave) * 256), gthr_src);
@@ -328,6 +353,26 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
         // repeat the last query (their results are never read)
         cur_list = jl;
         const float* cj = a.cents + size_t(jl) * a.ldc;
+        if constexpr (KIND == 2) {
+          // E4M3 fragments of this list, prepared per (list, query slot) by
+          // prep_qfrag_fp8_kernel: hi codes, then lo codes, in the operand's byte
+          // layout, and the E8M0 code of the hi scale (lo: 2^-4 of it)
+#pragma unroll
+          for (int g = 0; g < A; ++g) {
+            const int64_t qs = q0 + g * 32 + col;
+            const size_t fi = size_t(jl) * a.nq_pad + qs;
+            const char* fh = reinterpret_cast<const char*>(a.qfrag) + fi * kp_bytes + half * 16;
+            const char* fl = fh + size_t(a.L) * a.nq_pad * kp_bytes;
+#pragma unroll
+            for (int s = 0; s < KS; ++s) {
+              qf[g][s] = *reinterpret_cast<const uint4*>(fh + s * 32);
+              qlo[g][s] = *reinterpret_cast<const uint4*>(fl + s * 32);
+            }
+            qsh[g] = a.qsc[fi];
+            qsl[g] = qsh[g] - 4;
+            qt[g] = a.qtab[fi];
+          }
+        } else {
 #pragma unroll
         for (int g = 0; g < A; ++g) {
           const int64_t qs = q0 + g * 32 + col;
@@ -357,6 +402,7 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
           }
           qt[g] = a.qtab[size_t(jl) * a.nq_pad + qs];
         }
+        }
       }
     }
     // C input: hxn of the 16 rows this lane holds, shared by the query groups
@@ -382,16 +428,52 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
         for (int g = 0; g < A; ++g) acc[g][rb] = h;
       }
     }
+    if constexpr (KIND == 2) {
+      const uint8_t* scb = reinterpret_cast<const uint8_t*>(&tile_sc[buf][0]);
+      int sa[R];
 #pragma unroll
-    for (int s = 0; s < KS; ++s) {
-      uint4 af[R];
+      for (int rb = 0; rb < R; ++rb) sa[rb] = scb[rb * 32 + col];
 #pragma unroll
-      for (int rb = 0; rb < R; ++rb)
-        af[rb] = *reinterpret_cast<const uint4*>(tb + (rb * 32 + col) * ROWB + s * 32 + half * 16);
+      for (int p = 0; p < KS / 2; ++p) {
+        intx8 a8[R];
 #pragma unroll
-      for (int g = 0; g < A; ++g)
+        for (int rb = 0; rb < R; ++rb) {
+          const char* rp = tb + (rb * 32 + col) * ROWB + p * 64 + half * 16;
+          const uint4 x0 = *reinterpret_cast<const uint4*>(rp);
+          const uint4 x1 = *reinterpret_cast<const uint4*>(rp + 32);
+          a8[rb] = intx8{(int)x0.x, (int)x0.y, (int)x0.z, (int)x0.w, (int)x1.x, (int)x1.y, (int)x1.z, (int)x1.w};
+        }
 #pragma unroll
-        for (int rb = 0; rb < R; ++rb) Mfma<KIND>::step(acc[g][rb], af[rb], qf[g][s]);
+        for (int g = 0; g < A; ++g) {
+          const uint4 h0 = qf[g][2 * p], h1 = qf[g][2 * p + 1];
+          const intx8 bh{(int)h0.x, (int)h0.y, (int)h0.z, (int)h0.w, (int)h1.x, (int)h1.y, (int)h1.z, (int)h1.w};
+#pragma unroll
+          for (int rb = 0; rb < R; ++rb)
+            acc[g][rb] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a8[rb], bh, acc[g][rb], 0, 0, 0, sa[rb], 0,
+                                                                         qsh[g]);
+        }
+#pragma unroll
+        for (int g = 0; g < A; ++g) {
+          const uint4 l0 = qlo[g][2 * p], l1 = qlo[g][2 * p + 1];
+          const intx8 bl{(int)l0.x, (int)l0.y, (int)l0.z, (int)l0.w, (int)l1.x, (int)l1.y, (int)l1.z, (int)l1.w};
+#pragma unroll
+          for (int rb = 0; rb < R; ++rb)
+            acc[g][rb] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a8[rb], bl, acc[g][rb], 0, 0, 0, sa[rb], 0,
+                                                                         qsl[g]);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        uint4 af[R];
+#pragma unroll
+        for (int rb = 0; rb < R; ++rb)
+          af[rb] = *reinterpret_cast<const uint4*>(tb + (rb * 32 + col) * ROWB + s * 32 + half * 16);
+#pragma unroll
+        for (int g = 0; g < A; ++g)
+#pragma unroll
+          for (int rb = 0; rb < R; ++rb) Mfma<KIND>::step(acc[g][rb], af[rb], qf[g][s]);
+      }
     }
 
 #pragma unroll
@@ -527,12 +609,16 @@ __global__ __launch_bounds__(256, 4) void scan_mfma_batch_kernel_4wg(ScanBatchAr
 // The residual variants keep 64-row tiles and one wave per SIMD from 14 steps
 // (their list-change rebuild of the fragments leaves no room for two).
 constexpr int scan_groups(int ks) { return ks <= 16 ? 2 : 1; }
-constexpr bool scan_two_waves(int kind, int ks, bool resid) { return kind == 0 ? ks <= (resid ? 12 : 16) : ks <= 8; }
+// (E4M3, kind 2: 64-row tiles; two waves per SIMD up to 4 32-byte steps, whose
+// hi and lo fragments then fit without spills.)
+constexpr bool scan_two_waves(int kind, int ks, bool resid) {
+  return kind == 0 ? ks <= (resid ? 12 : 16) : (kind == 1 ? ks <= 8 : ks <= 4);
+}
 // Shapes whose fragments fit 128 registers run four workgroups (four waves per
 // SIMD): at few K steps the selection, not the MFMA, bounds the scan, and more
 // waves hide its latency.  They take 32-row tiles (half the accumulators) so
 // that the 128 registers hold them without spills.
-constexpr bool scan_four_wg(int kind, int ks) { return kind == 0 ? ks <= 4 : ks <= 2; }
+constexpr bool scan_four_wg(int kind, int ks) { return kind == 0 ? ks <= 4 : (kind == 1 && ks <= 2); }
 constexpr int scan_row_blocks(int kind, int ks, bool resid) {
   return scan_four_wg(kind, ks) ? 1 : ((kind == 0 && !resid ? ks <= 12 : ks <= 16) ? 2 : 1);
 }
@@ -543,13 +629,17 @@ constexpr int scan_row_blocks(int kind, int ks, bool resid) {
 // two tile buffers within one CU's LDS.
 static const int kBf16Steps[] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 56, 64};
 static const int kF32Steps[] = {1, 2, 3, 4, 6, 8, 12, 16, 20, 24, 32};
+// E4M3 steps are 32 coordinates (32 bytes), taken two at a time by the MFMA.
+static const int kFp8Steps[] = {2, 4, 6, 8};
 
 int mfma_ksteps_bucket(int kind, int d) {
-  const int per = kind == 0 ? 16 : 8;
+  const int per = kind == 0 ? 16 : (kind == 1 ? 8 : 32);
   int need = (d + per - 1) / per;
   if (need < 1) need = 1;
-  const int* tab = kind == 0 ? kBf16Steps : kF32Steps;
-  const int nt = kind == 0 ? (int)(sizeof(kBf16Steps) / sizeof(int)) : (int)(sizeof(kF32Steps) / sizeof(int));
+  const int* tab = kind == 0 ? kBf16Steps : (kind == 1 ? kF32Steps : kFp8Steps);
+  const int nt = kind == 0   ? (int)(sizeof(kBf16Steps) / sizeof(int))
+                 : kind == 1 ? (int)(sizeof(kF32Steps) / sizeof(int))
+                             : (int)(sizeof(kFp8Steps) / sizeof(int));
   for (int i = 0; i < nt; ++i)
     if (tab[i] >= need) return tab[i];
   return -1;
@@ -563,12 +653,13 @@ int mfma_rows_per_tile(int kind, int ksteps, bool resid) { return 32 * scan_row_
 int mfma_workgroups_per_cu(int kind, int ksteps, int waves, bool resid) {
   const int per = scan_four_wg(kind, ksteps) ? 4 : (scan_two_waves(kind, ksteps, resid) ? 2 : 1);
   const int by_waves = waves == 1 ? 4 * std::min(per, 2) : per;
-  const int lds = scan_lds_bytes(ksteps, scan_row_blocks(kind, ksteps, resid), waves == 1 ? 1 : 4, false);
+  const int lds = scan_lds_bytes(ksteps, scan_row_blocks(kind, ksteps, resid), waves == 1 ? 1 : 4, false) +
+                  scan_lds_extra(kind);
   const int by_lds = kLdsPerCu / ((lds + kLdsGranule - 1) / kLdsGranule * kLdsGranule);
   return std::max(1, std::min(by_waves, by_lds));
 }
 bool mfma_one_wave_supported(int kind, int ksteps, int V, bool resid) {
-  return kind == 0 && resid && V == 4 && ksteps <= 12;
+  return resid && ksteps <= 12 && ((kind == 0 && V == 4) || (kind == 2 && (V == 4 || V == 8)));
 }
 
 // ---- seeded thresholds (SeedArgs in kernels.h) ----------------------------------
@@ -771,12 +862,19 @@ static hipError_t launch_ks(const ScanMfmaArgs& a, int V, int nsplit, hipStream_
   constexpr int A = scan_groups(KS);
   constexpr int R = scan_row_blocks(KIND, KS, RESID);
   constexpr bool two_waves = scan_two_waves(KIND, KS, RESID);
-  if constexpr (KIND == 0 && RESID && KS <= 12) {
+  if constexpr ((KIND == 0 || KIND == 2) && RESID && KS <= 12) {
     if (a.waves == 1) {
-      if (V != 4) return hipErrorInvalidValue;
-      hipLaunchKernelGGL((scan_mfma_kernel_1w<KIND, KS, A, R, 4, RESID>), dim3((unsigned)(a.nqb * nsplit)), dim3(64),
-                         0, st, a);
-      return hipGetLastError();
+      if (V == 4) {
+        hipLaunchKernelGGL((scan_mfma_kernel_1w<KIND, KS, A, R, 4, RESID>), dim3((unsigned)(a.nqb * nsplit)),
+                           dim3(64), 0, st, a);
+        return hipGetLastError();
+      }
+      if constexpr (KIND == 2) {  // the E4M3 copy keeps KP = 120 candidates (slots of 512)
+        hipLaunchKernelGGL((scan_mfma_kernel_1w<KIND, KS, A, R, 8, RESID>), dim3((unsigned)(a.nqb * nsplit)),
+                           dim3(64), 0, st, a);
+        return hipGetLastError();
+      }
+      return hipErrorInvalidValue;
     }
   }
   if (a.waves == 1) return hipErrorInvalidValue;
@@ -871,8 +969,11 @@ int mfma_resid_max_ksteps() { return 40; }
 namespace {
 
 bool in_bucket(int kind, int ks) {
-  const int* tab = kind == 0 ? kBf16Steps : kF32Steps;
-  const int nt = kind == 0 ? (int)(sizeof(kBf16Steps) / sizeof(int)) : (int)(sizeof(kF32Steps) / sizeof(int));
+  if (kind < 0 || kind > 2) return false;
+  const int* tab = kind == 0 ? kBf16Steps : (kind == 1 ? kF32Steps : kFp8Steps);
+  const int nt = kind == 0   ? (int)(sizeof(kBf16Steps) / sizeof(int))
+                 : kind == 1 ? (int)(sizeof(kF32Steps) / sizeof(int))
+                             : (int)(sizeof(kFp8Steps) / sizeof(int));
   for (int i = 0; i < nt; ++i)
     if (tab[i] == ks) return true;
   return false;
@@ -931,8 +1032,9 @@ int64_t scan_check_launch(const ScanMfmaArgs& a, int kind, int ks, int V, int ns
   Violations v;
   const bool resid = a.resid != 0;
   const int W = a.waves == 1 ? 1 : 4;
-  if (kind != 0 && kind != 1) v.add("operand kind", kind, 1);
-  if (!in_bucket(kind, ks) || (resid && (kind != 0 || ks > mfma_resid_max_ksteps()))) v.add("K steps", ks, 0);
+  if (kind < 0 || kind > 2) v.add("operand kind", kind, 2);
+  if (kind == 2 && !resid) v.add("E4M3 operand without the residual layout", kind, 0);
+  if (!in_bucket(kind, ks) || (resid && (kind == 1 || ks > mfma_resid_max_ksteps()))) v.add("K steps", ks, 0);
   if (V != 4 && V != 8) v.add("slot width V", V, 8);
   if (W == 1 && !mfma_one_wave_supported(kind, ks, V, resid)) v.add("one-wave shape", ks, 12);
   if (v.count == 0) {
@@ -956,7 +1058,7 @@ int64_t scan_check_launch(const ScanMfmaArgs& a, int kind, int ks, int V, int ns
     const int A = scan_groups(ks), R = scan_row_blocks(kind, ks, resid);
     const int NS = 64 * V, BN = 32 * R, QPB = W * 32 * A;
     const size_t kp_bytes = size_t(ks) * 32;
-    const size_t es = kind == 0 ? 2 : 4;
+    const size_t es = kind == 0 ? 2 : (kind == 1 ? 4 : 1);
     if (size_t(a.Kp) * es != kp_bytes) v.add("operand row bytes", (long long)a.Kp * es, (long long)kp_bytes);
     if (a.KP < 1 || 2 * a.KP > NS) v.add("kept candidates KP", a.KP, NS / 2);
     // a merge leaves cnt <= Csub - 16 R before a tile, whose lanes then append at most 16 R
@@ -986,6 +1088,9 @@ int64_t scan_check_launch(const ScanMfmaArgs& a, int kind, int ks, int V, int ns
       if (size_t(tl + BN) * 4 > ex.hx) v.add("C inputs", (long long)(size_t(tl + BN) * 4), (long long)ex.hx);
       if (resid && size_t((tl >> 6) + 1) * 4 > ex.tile_list)
         v.add("block lists", (long long)((tl >> 6) + 1) * 4, (long long)ex.tile_list);
+      // E4M3 row scales: a tile copies BN bytes as BN / 4 words from row tl
+      if (kind == 2 && (size_t(tl + BN) > ex.xsc || (tl & 3) != 0 || (BN & 3) != 0))
+        v.add("row scales", (long long)(tl + BN), (long long)ex.xsc);
     }
     if (resid) {
       if (L < 1) v.add("lists", L, 1);
@@ -996,7 +1101,15 @@ int64_t scan_check_launch(const ScanMfmaArgs& a, int kind, int ks, int V, int ns
       // padding slots repeat query m - 1
       if (a.m < 1 || size_t(a.m - 1) * a.ld32 * 4 + size_t(a.d) * 4 > ex.q32)
         v.add("f32 queries", (long long)(size_t(a.m - 1) * a.ld32 * 4 + size_t(a.d) * 4), (long long)ex.q32);
-      if (a.d > 16 * ks || (a.ld32 & 7)) v.add("f32 query pitch", a.ld32, 8);
+      if (a.d > (kind == 2 ? 32 : 16) * ks || (a.ld32 & 7)) v.add("f32 query pitch", a.ld32, 8);
+      if (kind == 2) {
+        // prepared fragments: hi and lo rows of kp_bytes per (list, slot), one scale per (list, slot)
+        if (a.L != L) v.add("fragment lists", a.L, L);
+        const size_t frag_end = size_t(2) * L * a.nq_pad * kp_bytes;
+        if (frag_end > ex.qfrag) v.add("query fragments", (long long)frag_end, (long long)ex.qfrag);
+        if (size_t(L) * a.nq_pad * 4 > ex.qsc)
+          v.add("query scales", (long long)(size_t(L) * a.nq_pad * 4), (long long)ex.qsc);
+      }
     }
   }
   if (v.count && first) {
@@ -1010,6 +1123,18 @@ int64_t scan_check_launch(const ScanMfmaArgs& a, int kind, int ks, int V, int ns
 
 hipError_t launch_scan_mfma(const ScanMfmaArgs& a, int kind, int ks, int V, int nsplit, hipStream_t st) {
   if (V != 4 && V != 8) return hipErrorInvalidValue;
+  if (kind == 2) {
+    if (!a.resid || !a.xsc) return hipErrorInvalidValue;
+    switch (ks) {
+#define KS_CASE(K) \
+  case K:          \
+    return launch_ks<2, K, true>(a, V, nsplit, st);
+      KS_CASE(2) KS_CASE(4) KS_CASE(6) KS_CASE(8)
+#undef KS_CASE
+      default:
+        return hipErrorInvalidValue;
+    }
+  }
   if (kind == 0 && a.resid) {
     switch (ks) {
 #define KS_CASE(K) \

@@ -122,9 +122,12 @@ class SearchIndex:
 
     The rows are copied to HBM once; ``knn`` and ``radius_search`` then stream
     only the queries.  ``mode`` selects how candidates are generated ('auto',
-    'exact', 'fp32', 'bf16', or 'bf16r' -- bf16 on residuals against coarse
+    'exact', 'fp32', 'bf16', 'bf16r' -- bf16 on residuals against coarse
     centroids, for clustered data whose norms are large next to the neighbour
-    distances); every mode returns the exact fuel answer.
+    distances -- or 'fp8r', the same with an FP8 E4M3 resident copy of one byte
+    per coordinate and a power-of-two scale per row); every mode returns the
+    exact fuel answer.  ``widen`` (0: the mode's default, 4, or 16 for 'fp8r')
+    sets the candidates kept per query, widen * k.
 
     ``precompute=max_k`` fills fuel's precomputed kNN table (every point's
     neighbours with ties up to max_k) with one GPU call; self-queries with
@@ -134,7 +137,7 @@ class SearchIndex:
     """
 
     def __init__(self, data, *, distance='euclidean', tree=None, seed=None, precompute=None, mode='auto',
-                 widen=4, device=0):
+                 widen=0, device=0):
         data = _ensure_float(data)
         if data.ndim != 2:
             raise ValueError("data must be a 2D array")

@@ -64,6 +64,11 @@ extern "C" {
 #define FUEL_AMD_MODE_BF16R 4 /* as BF16, on residuals against coarse centroids: rows partitioned
                                  into lists at index build; for clustered data, where the plain
                                  form's bound (~ |q| |x|) is wider than the neighbour gaps */
+#define FUEL_AMD_MODE_FP8R 5  /* as BF16R with the resident copy in FP8 E4M3 (one byte per
+                                 coordinate, a power-of-two scale per row) and the query side
+                                 split into two E4M3 parts (hi + lo); d <= 256; default
+                                 widen 16, up to 256 candidates (the FP8 bound band is
+                                 wider) */
 
 /* pairwise output forms (fuel: compute_pairwise_dense / _condensed) */
 #define FUEL_AMD_FORM_SQUARE 0
@@ -83,13 +88,16 @@ typedef struct fuel_amd_index fuel_amd_index;
 #define FUEL_AMD_FLAG_FALLBACK_ARITH 1
 /* FUEL_AMD_FLAG_NO_SEED: start the plain MFMA scans (FP32 / BF16) at no
  * threshold instead of the seeded one (the KP-th smallest of per-row-group key
- * minima over a ~1/16 row sample, computed on the matrix cores first).  Answers
- * are the same either way; the seed only saves the scan candidate appends. */
+ * minima over a ~1/16 row sample, computed on the matrix cores first), and the
+ * residual scans (BF16R / FP8R) at their list bound instead of the sampled one
+ * (the KP-th smallest exact distance to the first rows of the query's nearest
+ * list).  Answers are the same either way; the seeds only save candidate
+ * appends. */
 #define FUEL_AMD_FLAG_NO_SEED 2
 
 typedef struct fuel_amd_opts {
   int32_t mode;   /* FUEL_AMD_MODE_* */
-  int32_t widen;  /* candidate widening factor for the MFMA modes (default 4) */
+  int32_t widen;  /* candidate widening factor for the MFMA modes (0: 4, or 16 for FP8R) */
   int32_t device; /* HIP device ordinal (default 0) */
   int32_t flags;  /* FUEL_AMD_FLAG_* (0: none) */
   int64_t reserved[6];

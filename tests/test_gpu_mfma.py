@@ -11,7 +11,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-MODES = ["bf16", "fp32", "bf16r"]
+MODES = ["bf16", "fp32", "bf16r", "fp8r"]
 
 
 def _same(a, b):
@@ -39,9 +39,9 @@ def test_c1_like(gpu, oracle, mode, exclude):
     # clusters far from the origin: plain bf16 rounding (relative to |x|) is too
     # coarse to certify most queries; the f32 form and the residual operand
     # (rounding relative to the distance to a centroid) are held to a rate
-    frac = {"fp32": 0.02, "bf16r": 0.10}.get(mode)
+    frac = {"fp32": 0.02, "bf16r": 0.10, "fp8r": 0.10}.get(mode)
     st = _run(gpu, oracle, X, X, 10, "euclidean", exclude, mode, max_fallback_frac=frac)
-    assert st["mode_used"] == {"bf16": 3, "fp32": 2, "bf16r": 4}[mode]
+    assert st["mode_used"] == {"bf16": 3, "fp32": 2, "bf16r": 4, "fp8r": 5}[mode]
 
 
 @pytest.mark.parametrize("mode", MODES)

@@ -1,8 +1,10 @@
-"""Seeded first-pass thresholds of the plain MFMA scans (the KP-th smallest of
+"""Seeded first-pass thresholds: of the plain MFMA scans (the KP-th smallest of
 per-row-group key minima over a strided sample, computed on the matrix cores
-before the scan).  The seed changes only how many candidates the scan appends:
-answers with and without it are identical and equal the oracle's, for both
-operand types, every exclusion mode, widening up to the 128-candidate cap and
+before the scan) and of the residual scans (the KP-th smallest upper bound of
+the exact distances to a sample of each query's nearest list).  The seed
+changes only how many candidates the scan appends:
+answers with and without it are identical and equal the oracle's, for every
+operand type, every exclusion mode, widening up to the 128-candidate cap and
 data with ties, duplicates and clusters."""
 import numpy as np
 import pytest
@@ -26,7 +28,7 @@ def cases(gpu):
 
 
 @pytest.mark.parametrize("name", ["uniform128", "grid16", "mixture64", "dup40"])
-@pytest.mark.parametrize("mode", ["bf16", "fp32"])
+@pytest.mark.parametrize("mode", ["bf16", "fp32", "bf16r", "fp8r"])
 def test_seed_does_not_change_answers(gpu, oracle, cases, name, mode):
     X = cases[name]
     rng = np.random.default_rng(len(name))
@@ -51,3 +53,18 @@ def test_seed_with_wide_candidate_lists(gpu, oracle, cases, widen):
         gi, gd = ix.knn(Q, k)
         oi, od = oracle.knn(X, Q, k, "sqeuclidean")
         assert np.array_equal(gi, oi) and gd.tobytes() == od.tobytes(), (widen, k, ix.stats())
+
+
+@pytest.mark.parametrize("mode", ["bf16r", "fp8r"])
+def test_sampled_seed_on_the_serving_shape(gpu, oracle, mode):
+    # the serving mixture: the sampled threshold lies near each query's KP-th
+    # key, and the answers equal the cold scan's and the oracle's
+    from fuel_amd import workloads
+    rng = np.random.default_rng(72)
+    X = workloads.gaussian_mixture(rng, 150000, 96, 50, -10.0, 10.0)
+    Q = workloads.gaussian_mixture(rng, 400, 96, 50, -10.0, 10.0)
+    a = gpu.Index(X, "euclidean", mode=mode).knn(Q, 10)
+    b = gpu.Index(X, "euclidean", mode=mode, seed=False).knn(Q, 10)
+    assert np.array_equal(a[0], b[0]) and a[1].tobytes() == b[1].tobytes()
+    oi, od = oracle.knn(X, Q, 10, "euclidean")
+    assert np.array_equal(a[0], oi) and a[1].tobytes() == od.tobytes()

@@ -33,7 +33,7 @@ def _requests(Q, rng, ks=(10,)):
     return reqs
 
 
-@pytest.mark.parametrize("mode", ["bf16", "bf16r", "fp32", "exact"])
+@pytest.mark.parametrize("mode", ["bf16", "bf16r", "fp8r", "fp32", "exact"])
 def test_mix_equals_alone_and_oracle(gpu, oracle, served, mode):
     from fuel_amd import Server
     X, Q = served

@@ -8,7 +8,7 @@ residual operand, the block lists, centroids, f32 queries and per-list terms.
 The library checks each launch against the sizes of the buffers it passes
 (scan_check_launch) and refuses one that would leave them.  This test drives
 the same check over the plans knn_mfma makes for every shipped shape -- bf16,
-f32 and residual operands, every K step bucket (narrowest and widest
+f32, residual bf16 and residual E4M3 operands, every K step bucket (narrowest and widest
 dimension of each), four-workgroup, two-wave, wide and one-wave launches --
 against sizes from 300 to 10^7 rows, 1 to 10^5 queries and KP 10..128, with
 the buffers knn_mfma allocates; then plants one mistake at a time in every plan
@@ -33,6 +33,7 @@ PERTURBATIONS = {
     10: "query operand / per-list terms short",
     11: "f32 queries / thresholds short",
     12: "centroids / list counts short",
+    13: "E4M3 row scales / query fragments / fragment scales short",
 }
 
 
