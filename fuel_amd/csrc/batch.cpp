@@ -82,6 +82,7 @@ struct Bucket {
   int ks = 0, V = 4, grid = 0;
   std::vector<int> members;  // largest first
   int64_t nitems = 0;
+  int64_t range_start[SCAN_XCDS + 1] = {};  // item ranges of about equal work, one per XCD
   size_t pool_bytes = 0;
   DevBuf probs_dev, starts_dev;
 };
@@ -170,7 +171,7 @@ int32_t plan_batch(fuel_amd_batch* b, const std::map<int, Cfg>& cfg_by_d) {
   }
   hipError_t e = b->keep.ensure(std::max<size_t>(keep_off, 256));
   if (e == hipSuccess) e = b->fail_counts.ensure(sizeof(int) * std::max(1, np));
-  if (e == hipSuccess) e = b->counter.ensure(256);
+  if (e == hipSuccess) e = b->counter.ensure(sizeof(uint32_t) * SCAN_XCDS * SCAN_COUNTER_STRIDE);
   if (e != hipSuccess) return hip_fail(e, "batch workspaces");
   size_t pool_max = 0;
   for (auto& kv : by_d) {
@@ -269,6 +270,26 @@ int32_t plan_batch(fuel_amd_batch* b, const std::map<int, Cfg>& cfg_by_d) {
       starts.push_back(starts.back() + pp.nqb * pp.nsplit);
     }
     bk.nitems = starts.back();
+    {
+      // XCD ranges: cut the item sequence where the running work (rows per item)
+      // reaches x / SCAN_XCDS of the total; an item's work is its split's rows
+      std::vector<double> acc(1, 0.0);
+      for (int i : bk.members) acc.push_back(acc.back() + double(b->plan[i].nqb * b->plan[i].nsplit) *
+                                                               double(b->plan[i].rps));
+      const double total = acc.back();
+      bk.range_start[0] = 0;
+      size_t pi = 0;  // problem whose items hold the cut
+      for (int x = 1; x < SCAN_XCDS; ++x) {
+        const double target = total * x / SCAN_XCDS;
+        while (pi + 1 < bk.members.size() && acc[pi + 1] < target) ++pi;
+        const auto& pp = b->plan[bk.members[pi]];
+        const double per = double(pp.rps);
+        int64_t within = per > 0 ? (int64_t)((target - acc[pi]) / per) : 0;
+        within = std::max<int64_t>(0, std::min<int64_t>(within, starts[pi + 1] - starts[pi]));
+        bk.range_start[x] = std::max(bk.range_start[x - 1], std::min(bk.nitems, starts[pi] + within));
+      }
+      bk.range_start[SCAN_XCDS] = bk.nitems;
+    }
     bk.pool_bytes = off;
     bk.grid = (int)slots;
     pool_max = std::max(pool_max, off);
@@ -485,13 +506,14 @@ hipError_t enqueue_bucket(fuel_amd_batch* b, const Bucket& B, hipStream_t st) {
     if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(b->fail_counts.as<int>() + i), 1, 0u, st);
     if (e != hipSuccess) return e;
   }
-  e = launch_fill_u32(b->counter.as<uint32_t>(), 1, 0u, st);
+  e = launch_fill_u32(b->counter.as<uint32_t>(), SCAN_XCDS * SCAN_COUNTER_STRIDE, 0u, st);
   ScanBatchArgs sb{};
   sb.probs = B.probs_dev.as<ScanMfmaArgs>();
   sb.item_start = B.starts_dev.as<int64_t>();
   sb.nprob = (int)B.members.size();
   sb.nitems = B.nitems;
   sb.counter = b->counter.as<unsigned int>();
+  for (int x = 0; x <= SCAN_XCDS; ++x) sb.range_start[x] = B.range_start[x];
   if (e == hipSuccess)
     e = launch_scan_mfma_batch(sb, B.cfg.kind == 2 ? 0 : B.cfg.kind, B.ks, B.V, B.grid, B.cfg.kind == 2, st);
   for (int i : B.members) {

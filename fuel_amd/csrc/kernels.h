@@ -461,12 +461,20 @@ hipError_t launch_fill_u32(uint32_t* p, int64_t count, uint32_t v, hipStream_t s
 // K steps and slot width): work item i of problem p (i in [0, nqb * nsplit))
 // is query block i % nqb of split i / nqb; workgroups take items from a shared
 // counter until none are left.
+// The persistent scan's work items are cut into SCAN_XCDS ranges of about equal
+// work; a workgroup starts on the range of its XCD (blockIdx.x mod SCAN_XCDS:
+// workgroups are dispatched to the XCDs in turn), so the items of one split,
+// which read the same rows, share that XCD's L2; a workgroup whose range is
+// drained helps the others, so every item is taken exactly once.
+constexpr int SCAN_XCDS = 8;
+constexpr int SCAN_COUNTER_STRIDE = 64;  // words between the ranges' counters
 struct ScanBatchArgs {
   const ScanMfmaArgs* probs;   // device array, one per problem
   const int64_t* item_start;   // nprob + 1 running totals of the problems' items
   int nprob;
   int64_t nitems;
-  unsigned int* counter;       // zero before the launch
+  unsigned int* counter;       // SCAN_XCDS counters, SCAN_COUNTER_STRIDE words apart, zero before the launch
+  int64_t range_start[SCAN_XCDS + 1];  // item ranges: [range_start[x], range_start[x + 1])
 };
 
 hipError_t launch_prep_db(const PrepDbArgs& a, hipStream_t st);

@@ -564,19 +564,24 @@ __global__ __launch_bounds__(64, 8) void scan_mfma_kernel_1w(ScanMfmaArgs a) {
 }
 
 // Persistent workgroups over the work items of many problems.  One lane takes
-// the next item from the counter and publishes it through LDS; the barriers
-// around the exchange also separate one item's LDS tiles from the next one's
-// (the body ends with every tile read and every copy into LDS completed).  Every
-// workgroup leaves once the counter passes the last item.
+// the next item of a range from that range's counter and publishes it through
+// LDS; the barriers around the exchange also separate one item's LDS tiles from
+// the next one's (the body ends with every tile read and every copy into LDS
+// completed).  A workgroup drains its XCD's range first, then the others in
+// turn, and leaves once every range's counter has passed its end.
 template <int KIND, int KS, int A, int R, int V, bool RESID>
 __device__ __forceinline__ void scan_mfma_batch_body(const ScanBatchArgs& b) {
   __shared__ int64_t item_s;
-  for (;;) {
-    if (threadIdx.x == 0) item_s = (int64_t)atomicAdd(b.counter, 1u);
+  const int home = (int)(blockIdx.x % (unsigned)SCAN_XCDS);
+  for (int c = 0; c < SCAN_XCDS; ++c) {
+   const int x = (home + c) % SCAN_XCDS;
+   const int64_t lo = b.range_start[x], hi = b.range_start[x + 1];
+   for (;;) {
+    if (threadIdx.x == 0) item_s = lo + (int64_t)atomicAdd(b.counter + x * SCAN_COUNTER_STRIDE, 1u);
     __syncthreads();
     const int64_t item = item_s;
     __syncthreads();
-    if (item >= b.nitems) return;
+    if (item >= hi) break;
     int lo = 0, hi = b.nprob - 1;  // last problem whose first item is <= item
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
@@ -587,6 +592,7 @@ __device__ __forceinline__ void scan_mfma_batch_body(const ScanBatchArgs& b) {
     const ScanMfmaArgs a = b.probs[p];
     const int64_t local = item - b.item_start[p];
     scan_mfma_body<KIND, KS, A, R, V, RESID>(a, (int)(local % a.nqb), (int)(local / a.nqb));
+   }
   }
 }
 template <int KIND, int KS, int A, int R, int V, bool RESID>
