@@ -61,16 +61,19 @@ def parse():
     p.add_argument("--cpu-sample", type=int, default=1000, help="queries timed on the CPU baseline (0: skip)")
     p.add_argument("--fp32-steps", type=int, default=2,
                    help="timed steps of the plain FP32 MFMA form run beside the mixed one (0: skip)")
-    p.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "c5", "dbscan", "kmeans"],
+    p.add_argument("--workload", default="c2", choices=["c2", "c1", "c3", "c4", "c5", "c5scan", "dbscan", "kmeans"],
                    help="c2: the headline kNN line; the others print their own line (not the headline): "
                         "c1 self-kNN 10k x 16 + pairwise 2000^2, c3 LOF 70k x 784 k=20, c4 512 uneven self-kNN "
                         "problems (batched), c5 serving 10M x 96 "
-                        "under Poisson arrivals, dbscan on c1, kmeans distance matrix 1M x 64 vs 100 centres")
+                        "under Poisson arrivals (c5scan: only 20 serving-size launches of 64 queries, for "
+                        "profiling the serving scan), dbscan on c1, kmeans distance matrix 1M x 64 vs 100 centres")
     p.add_argument("--batch-mode", default="auto", choices=["auto", "bf16", "bf16r", "fp32", "exact"],
                    help="configuration of the batched problems (c4): autotuned or fixed")
     p.add_argument("--serve-mode", default="bf16r", choices=["bf16", "bf16r", "fp8r", "fp32", "exact"],
                    help="candidate mode of the served index (c5)")
     p.add_argument("--serve-seconds", type=float, default=4.0, help="seconds of arrivals per rate (c5)")
+    p.add_argument("--serve-fp8-seconds", type=float, default=2.0,
+                   help="seconds of serving the FP8 E4M3 copy beside the serving mode in the headline line (0: skip)")
     p.add_argument("--serve-p99-seconds", type=float, default=3.0,
                    help="seconds of serving at 10,000 queries/s measured for the headline line's p99 (0: skip)")
     p.add_argument("--serve-rows", type=int, default=10_000_000, help="rows of the served index (c5)")
@@ -296,33 +299,43 @@ def serve_scan_roofline(ix, pool, X, mode, reps=20, m=64):
 
 def serve_headline(args):
     """p50/p99 request latency of the serving loop on the serving configuration
-    at one arrival rate (10,000 queries/s), for the headline line."""
+    at one arrival rate (10,000 queries/s), for the headline line, in the serving
+    mode and (--serve-fp8-seconds) with the FP8 E4M3 copy; every row of the
+    first requests of each run is checked against the oracle."""
     from fuel_amd import Server, _lib
     import oracle
     X, pool, rng = c5_data(args)
-    ix = _lib.Index(X, "euclidean", mode=args.serve_mode)
-    with Server(ix, workers=args.serve_workers, max_batch=4096) as srv:
-        serve_run(srv, pool, 2000, 0.5, rng)  # warm-up
-        r, kept = serve_run(srv, pool, 10000, args.serve_p99_seconds, rng, keep_queries=args.serve_verify)
-    scan = serve_scan_roofline(ix, pool, X, args.serve_mode)
-    ix.close()
-    r = dict(r)
-    r["scan"] = scan
-    # served answers against the oracle, every row of the first requests
-    t0 = time.perf_counter()
-    same, nchk, nreq = True, 0, 0
-    for off, m, idx, dist in kept:
-        oi, od = oracle.knn(X, pool[off:off + m], 10, "euclidean", "none", threads=oracle.default_threads())
-        same = same and bool(np.array_equal(oi, idx) and od.tobytes() == dist.tobytes())
-        nchk += m
-        nreq += 1
-    print(f"# serve: {nchk} served queries of {nreq} requests checked against the oracle in "
-          f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
-    r.update({"workload": f"serving: resident index {X.shape[0]:,} x {X.shape[1]} f32 (50-component mixture), "
-                          "k=10, Poisson arrivals, geometric batch sizes (mean 64, cap 4096)",
-              "mode": args.serve_mode, "seconds": args.serve_p99_seconds,
-              "verify": {"requests": nreq, "queries": nchk, "identical": same}})
-    return r
+    out = {}
+    runs = [(args.serve_mode, args.serve_p99_seconds)]
+    if args.serve_fp8_seconds > 0 and args.serve_mode != "fp8r":
+        runs.append(("fp8r", args.serve_fp8_seconds))
+    for mode, seconds in runs:
+        ix = _lib.Index(X, "euclidean", mode=mode)
+        with Server(ix, workers=args.serve_workers, max_batch=4096) as srv:
+            serve_run(srv, pool, 2000, 0.5, rng)  # warm-up
+            r, kept = serve_run(srv, pool, 10000, seconds, rng, keep_queries=args.serve_verify)
+        scan = serve_scan_roofline(ix, pool, X, mode)
+        ix.close()
+        r = dict(r)
+        r["scan"] = scan
+        t0 = time.perf_counter()
+        same, nchk, nreq = True, 0, 0
+        for off, m, idx, dist in kept:
+            oi, od = oracle.knn(X, pool[off:off + m], 10, "euclidean", "none", threads=oracle.default_threads())
+            same = same and bool(np.array_equal(oi, idx) and od.tobytes() == dist.tobytes())
+            nchk += m
+            nreq += 1
+        print(f"# serve {mode}: {nchk} served queries of {nreq} requests checked against the oracle in "
+              f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+        r.update({"workload": f"serving: resident index {X.shape[0]:,} x {X.shape[1]} f32 (50-component mixture), "
+                              "k=10, Poisson arrivals, geometric batch sizes (mean 64, cap 4096)",
+                  "mode": mode, "seconds": seconds,
+                  "verify": {"requests": nreq, "queries": nchk, "identical": same}})
+        out[mode] = r
+    main = out[args.serve_mode]
+    if "fp8r" in out and args.serve_mode != "fp8r":
+        main["fp8r"] = out["fp8r"]
+    return main
 
 
 def c3_full(threads):
@@ -539,6 +552,15 @@ def secondary(args):
                      "identical_sample": same, "checked_queries": nchk,
                      "cpu_baseline": {"value": round(nchk / t_c, 2), "unit": "queries/s", "cores": threads,
                                       "kind": "port", "sample": f"{nchk} served queries re-answered by the oracle"}})
+    elif args.workload == "c5scan":
+        X, pool, rng = c5_data(args)
+        ix = _lib.Index(X, "euclidean", mode=args.serve_mode)
+        scan = serve_scan_roofline(ix, pool, X, args.serve_mode)
+        st = ix.stats()
+        ix.close()
+        line.update({"config": f"serving scan: resident index {X.shape[0]:,} x {X.shape[1]} f32 (50-component "
+                               "mixture), 20 launches of 64 queries, k=10", "mode": args.serve_mode, "scan": scan,
+                     "kp": st["kp"], "splits": st["splits"], "fallback_queries_last": st["fallback_queries"]})
     elif args.workload == "dbscan":
         from fuel_amd.cluster import dbscan, optics
         X = workloads.c1()

@@ -372,11 +372,17 @@ hipError_t launch_prep_qfrag_fp8(const QFragArgs& a, hipStream_t st) {
   return hipGetLastError();
 }
 
+// One workgroup of four waves per query: every wave finds the nearest list
+// (the same answer), wave w bounds rows w * 128 + lane and + 64 of its sample
+// (float4 loads, f64 sums in four chains), the bounds meet in LDS and wave 0
+// selects the KP-th smallest.
 __global__ __launch_bounds__(256) void seed_sample_kernel(SampleSeedArgs a) {
-  constexpr int RPL = SEED_SAMPLE_ROWS / 64;  // sample rows per lane
-  const int64_t q = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
-  if (q >= a.m) return;
+  constexpr int RPL = SEED_SAMPLE_ROWS / 256;  // sample rows per thread
+  constexpr int RW = SEED_SAMPLE_ROWS / 64;    // sample rows per lane of wave 0's select
+  __shared__ uint32_t keys[SEED_SAMPLE_ROWS];
+  const int64_t q = blockIdx.x;
   const int lane = __lane_id();
+  const int wave = threadIdx.x >> 6;
   const float* s2 = a.s2 + q * a.L;
   float best = INFINITY;
   int bj = a.L;  // none
@@ -394,37 +400,54 @@ __global__ __launch_bounds__(256) void seed_sample_kernel(SampleSeedArgs a) {
       bj = oj;
     }
   }
-  if (bj >= a.L) return;  // wave-uniform: no list holds KP rows
+  if (bj >= a.L) return;  // uniform over the workgroup: no list holds KP rows
   const uint32_t cnt = a.lcount[bj];
   const int S = cnt < (uint32_t)SEED_SAMPLE_ROWS ? (int)cnt : SEED_SAMPLE_ROWS;
   const int64_t p0 = a.loff[bj];  // positions p0 .. p0 + cnt - 1 are the list's rows
   const float* qv = a.Q32 + q * a.ld32;
-  uint32_t key[RPL];
+  const int d4 = a.d & ~3;
 #pragma unroll
   for (int r = 0; r < RPL; ++r) {
-    const int p = r * 64 + lane;
+    const int p = wave * (64 * RPL) + r * 64 + lane;
     uint32_t o = 0xFFFFFFFFu;
     if (p < S) {
       const float* x = a.X + int64_t(a.rperm[p0 + p]) * a.ldx;
-      double s = 0.0;
-      for (int c = 0; c < a.d; ++c) {
-        const double t = (double)qv[c] - (double)x[c];
-        s = fma(t, t, s);
+      double s[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int c = 0; c < d4; c += 4) {  // rows and queries are 16-byte aligned (pitch a multiple of 8)
+        const float4 xv = *reinterpret_cast<const float4*>(x + c);
+        const float4 qq = *reinterpret_cast<const float4*>(qv + c);
+        const double t0 = (double)qq.x - (double)xv.x, t1 = (double)qq.y - (double)xv.y;
+        const double t2 = (double)qq.z - (double)xv.z, t3 = (double)qq.w - (double)xv.w;
+        s[0] = fma(t0, t0, s[0]);
+        s[1] = fma(t1, t1, s[1]);
+        s[2] = fma(t2, t2, s[2]);
+        s[3] = fma(t3, t3, s[3]);
       }
-      const double v = 0.5 * s * (1 + 0x1p-16) + 0x1p-140;
+      for (int c = d4; c < a.d; ++c) {
+        const double t = (double)qv[c] - (double)x[c];
+        s[0] = fma(t, t, s[0]);
+      }
+      // an upper bound of the row's scan key (f64 sums in any order err by far
+      // less than the 2^-16 slack)
+      const double v = 0.5 * ((s[0] + s[1]) + (s[2] + s[3])) * (1 + 0x1p-16) + 0x1p-140;
       float f = (float)v;
       if ((double)f < v) f = nextafterf(f, INFINITY);
       o = ord_f32(f);
     }
-    key[r] = o;
+    keys[wave * (64 * RPL) + r * 64 + lane] = o;
   }
+  __syncthreads();
+  if (wave != 0) return;
+  uint32_t key[RW];
+#pragma unroll
+  for (int r = 0; r < RW; ++r) key[r] = keys[r * 64 + lane];
   uint32_t p = 0;  // the KP-th smallest bound: smallest t with #(key <= t) >= KP
-  for (int b = 31; b >= 0; --b) {
-    const uint32_t probe = p | ((1u << b) - 1u);
+  for (int bit = 31; bit >= 0; --bit) {
+    const uint32_t probe = p | ((1u << bit) - 1u);
     int c = 0;
 #pragma unroll
-    for (int r = 0; r < RPL; ++r) c += __popcll(__ballot(key[r] <= probe));
-    if (c < a.KP) p |= (1u << b);
+    for (int r = 0; r < RW; ++r) c += __popcll(__ballot(key[r] <= probe));
+    if (c < a.KP) p |= (1u << bit);
   }
   const float t = unord_f32(p);
   if (lane == 0 && t < MFMA_KEY_LIMIT) {
@@ -436,7 +459,8 @@ __global__ __launch_bounds__(256) void seed_sample_kernel(SampleSeedArgs a) {
 hipError_t launch_seed_sample(const SampleSeedArgs& a, hipStream_t st) {
   if (a.m <= 0 || a.L <= 0) return hipSuccess;
   if (a.KP < 1 || a.KP > SEED_SAMPLE_ROWS) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(seed_sample_kernel, dim3((unsigned)((a.m + 3) / 4)), dim3(256), 0, st, a);
+  if ((a.ldx & 3) || (a.ld32 & 3)) return hipErrorInvalidValue;  // float4 rows
+  hipLaunchKernelGGL(seed_sample_kernel, dim3((unsigned)a.m), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
