@@ -286,9 +286,9 @@ int mfma_splits(int64_t nqb, int64_t n_pad, int cus, int64_t rows_tile, int wg_p
 
 // Sample of the seed pass (ntiles = 0: no seed).  Every group of every part
 // must receive at least one row block (parts * SEED_GROUPS / R tiles), the
-// groups must number at least KP, and the sample stays at about 1/16 of the
-// operand's tiles, at most 1024 tiles; operands of fewer than 8 samples' worth
-// of tiles start cold.
+// groups number about 4 KP (at least KP), the sample is a fraction of the
+// operand's tiles that grows with KP / n (at most 4096 tiles), and operands of
+// fewer than two samples' worth of tiles start cold.
 SeedPlan plan_seed(int mk, int ks, bool resid, bool has_theta, int flags, int64_t n_pad, int rows_tile, int KP) {
   SeedPlan p;
   if (resid || has_theta || (flags & FUEL_AMD_FLAG_NO_SEED)) return p;
@@ -296,14 +296,21 @@ SeedPlan plan_seed(int mk, int ks, bool resid, bool has_theta, int flags, int64_
   // the candidate appends the seed saves (1M x 128: scan -4.6 ms, seed +13 ms)
   if (mk == 1 && ks > 4) return p;
   const int R = rows_tile / 32;
-  p.parts = std::max(4, (KP + 2 * SEED_GROUPS - 1) / (2 * SEED_GROUPS));
+  // about 4 KP group minima, so that the KP-th of them sits near the sample's
+  // KP-th key rather than near its largest group minimum
+  p.parts = std::min(32, std::max(4, (4 * KP + 2 * SEED_GROUPS - 1) / (2 * SEED_GROUPS)));
   const int64_t tiles = n_pad / rows_tile;
   const int64_t need = int64_t(p.parts) * SEED_GROUPS / R;
-  if (tiles < 8 * need) {
+  if (tiles < 2 * need) {
     p.parts = 0;
     return p;
   }
-  p.ntiles = std::min<int64_t>(1024, std::max<int64_t>(need, tiles / 16));
+  // Sample fraction: a query still appends about KP / f rows after the seed and
+  // the seed reads f n rows, so f grows as sqrt(KP / n): 1/16 at KP = 40 and
+  // 10^6 rows (where it was measured best), at most 1/2.
+  const double f = std::min(0.5, std::sqrt(25000.0 * double(KP) / double(n_pad)) / 16.0);
+  p.ntiles = std::min<int64_t>(4096, std::max<int64_t>(need, (int64_t)(f * double(tiles))));
+  p.ntiles = std::min(p.ntiles, tiles);
   p.stride = (tiles / p.ntiles) * rows_tile;
   return p;
 }
