@@ -43,7 +43,8 @@
 namespace fa {
 namespace {
 
-constexpr int KP_LIMIT = 128;  // widest kept list of the MFMA path
+constexpr int KP_LIMIT = 128;  // widest kept list of the MFMA path's first level
+constexpr int KP_CHAIN = 256;  // ... and of the pass from the certified threshold
 constexpr int kChainLanes = 4;  // certification chains of different problems run concurrently
 using Clock = std::chrono::steady_clock;
 
@@ -668,7 +669,7 @@ int32_t finish(fuel_amd_batch* b) {
         } else {
           rc = knn_mfma(&shell, &shell.ctx, static_cast<const float*>(pr.data), b->h_fail[i], pr.ld, pr.k,
                         b->exclude, resid ? 1 : pp.kind, pr.out_idx, static_cast<float*>(pr.out_dist), st,
-                        at<int64_t>(b->keep, pp.o_fl), 0, resid ? 2 : 1, KP_LIMIT,
+                        at<int64_t>(b->keep, pp.o_fl), 0, resid ? 2 : 1, resid ? KP_LIMIT : KP_CHAIN,
                         resid ? nullptr : at<float>(b->keep, pp.o_ft));
         }
         exact += shell.ctx.stats.exact_queries - before;

@@ -32,6 +32,11 @@ constexpr int kMaxLists = 256;                          // resid_lists never exc
 // segments), 256 for the E4M3 copy, whose bound band is wider (two 128-entry
 // segments, still whole tiles of appends).
 constexpr int kp_max(int kind) { return kind == 3 ? 256 : 128; }
+// A pass from the certified threshold keeps up to 256 for every kind: only the
+// rows of the query's bound band append, and a wide band is what sent the query
+// there.  (A level without a threshold, the f32 pass after a residual one,
+// keeps its kind's first-level limit.)
+constexpr int KP_CHAIN = 256;
 constexpr size_t kFailRowsKept = size_t(1) << 20;  // rows of failed queries recorded per call
 
 // Candidates kept per query: widen * k (default 4; 16 for the E4M3 copy, whose
@@ -723,7 +728,7 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
         // (||q - x||^2 / 2) are not those of the f32 form
         const float* theta = (next_kind == kind || !resid) ? ra.fail_theta : nullptr;
         rc = knn_mfma(ix, ctx, Qb, nfail, ldq, k, exclude, next_kind, ra.out_idx, ra.out_dist, st, ra.fail_list,
-                      ra.qbase, level + 1, kp_max(next_kind), theta);
+                      ra.qbase, level + 1, theta ? KP_CHAIN : kp_max(next_kind), theta);
       } else {
         ctx->stats.exact_queries += nfail;  // queries answered by the exact scan
         rc = knn_exact_rows(ix, ctx, Qb, ldq, ra.fail_list, nfail, ra.qbase, k, exclude, ra.out_idx, ra.out_dist,
