@@ -206,16 +206,64 @@ __device__ __forceinline__ int merge_select(uint64_t* __restrict__ slot, int KP,
 // up to them.  The scan body and the launch planning both use these.
 constexpr int scan_wave_loads(int ks, int R, int W) { return (32 * R * (2 * ks + 1) + 64 * W - 1) / (64 * W); }
 constexpr int scan_tile_bytes(int ks, int R, int W) { return scan_wave_loads(ks, R, W) * 64 * W * 16; }
-// LDS of one workgroup: two tile buffers, their C inputs (64 floats each) and
+// LDS of one workgroup: nb tile buffers, their C inputs (64 floats each) and
 // the published thresholds (64 words per wave and buffer), plus the persistent
 // form's work item.
-constexpr int scan_lds_bytes(int ks, int R, int W, bool batch) {
-  return 2 * scan_tile_bytes(ks, R, W) + 2 * 64 * 4 + 2 * W * 64 * 4 + (batch ? 16 : 0);
+constexpr int scan_lds_bytes(int ks, int R, int W, bool batch, int nb = 2) {
+  return nb * scan_tile_bytes(ks, R, W) + nb * 64 * 4 + nb * W * 64 * 4 + (batch ? 16 : 0);
 }
-// the E4M3 operand's row scales: 64 words per buffer
-constexpr int scan_lds_extra(int kind) { return kind == 2 ? 2 * 64 * 4 : 0; }
+// per buffer, 64 words of the E4M3 operand's row scales and (residual operand)
+// 64 copies of the tile's list
+constexpr int scan_lds_extra(int kind, int nb = 2, bool resid = false) {
+  return nb * 64 * 4 * ((kind == 2 ? 1 : 0) + (resid ? 1 : 0));
+}
 constexpr int kLdsPerCu = 160 * 1024;  // gfx950
 constexpr int kLdsGranule = 512;       // allocation granularity per workgroup
+
+// Shape of a K step count: query groups per wave (A), 32-row blocks per tile
+// (R), two waves per SIMD or one.  Plain bf16 shapes of 14 and 16 steps take
+// 32-row tiles so that two waves per SIMD fit (their query fragments take 128
+// registers); beyond 16 steps one query group per wave.
+// The residual variants keep 64-row tiles and one wave per SIMD from 14 steps
+// (their list-change rebuild of the fragments leaves no room for two).
+constexpr int scan_groups(int ks) { return ks <= 16 ? 2 : 1; }
+// (E4M3, kind 2: 64-row tiles; two waves per SIMD up to 4 32-byte steps, whose
+// hi and lo fragments then fit without spills.)
+constexpr bool scan_two_waves(int kind, int ks, bool resid) {
+  return kind == 0 ? ks <= (resid ? 12 : 16) : (kind == 1 ? ks <= 8 : ks <= 4);
+}
+// Shapes whose fragments fit 128 registers run four workgroups (four waves per
+// SIMD): at few K steps the selection, not the MFMA, bounds the scan, and more
+// waves hide its latency.  They take 32-row tiles (half the accumulators) so
+// that the 128 registers hold them without spills.
+constexpr bool scan_four_wg(int kind, int ks) { return kind == 0 ? ks <= 4 : (kind == 1 && ks <= 2); }
+constexpr int scan_row_blocks(int kind, int ks, bool resid) {
+  return scan_four_wg(kind, ks) ? 1 : ((kind == 0 && !resid ? ks <= 12 : ks <= 16) ? 2 : 1);
+}
+
+// Tile buffers of a shape: three (the copy of tile t + 2 leaves when tile t
+// starts, so two copies overlap the reads of one) when the larger LDS share
+// leaves as many workgroups resident as two buffers do (the resident count by
+// registers is 4 for one-wave workgroups, whose 270-390 registers leave one
+// wave per SIMD, and the launch bound otherwise); else two (the copy of tile
+// t + 2 leaves when tile t's buffer is free).  Fewer workgroups with deeper
+// copies measured slower (one-wave bf16 residual scan of 10M x 96: 0.53 ms with
+// four workgroups per CU and two buffers, 0.66 ms with three and three).
+constexpr int scan_wgs_by_lds(int lds) { return kLdsPerCu / ((lds + kLdsGranule - 1) / kLdsGranule * kLdsGranule); }
+constexpr int scan_buffers(int kind, int ks, int R, int W, bool resid) {
+  const int per = W == 1 ? 4 : (scan_four_wg(kind, ks) ? 4 : (scan_two_waves(kind, ks, resid) ? 2 : 1));
+  const int n2 = std::min(per, scan_wgs_by_lds(scan_lds_bytes(ks, R, W, true, 2) + scan_lds_extra(kind, 2, resid)));
+  const int n3 = std::min(per, scan_wgs_by_lds(scan_lds_bytes(ks, R, W, true, 3) + scan_lds_extra(kind, 3, resid)));
+  return n3 >= n2 && n3 > 0 ? 3 : 2;
+}
+// s_waitcnt vmcnt(n) with the export and LDS counters left alone (gfx9 encoding:
+// vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8).
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt has 6 bits");
+  __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
+}
+
 
 // Tile staging: lane i of copy j of wave w fills LDS chunk p = (j W + w) 64 + i
 // = row (2 KS + 1) + c of the tile; the pad chunk (c == 2 KS) and the round-up
@@ -261,10 +309,19 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
   constexpr int WAVE_LOADS = scan_wave_loads(KS, R, W);  // 64-chunk loads per wave per tile
   constexpr int TILE_BYTES = scan_tile_bytes(KS, R, W);  // rounded up to whole wave loads
   static_assert(WAVE_LOADS * 64 * W >= BN * ROWC, "tile copy covers the tile");
-  __shared__ __attribute__((aligned(16))) char tile[2][TILE_BYTES];
-  __shared__ __attribute__((aligned(16))) float tile_hx[2][64];      // one dword per lane of wave 0
-  __shared__ __attribute__((aligned(16))) uint32_t thr_lds[2][W][64];  // published thresholds, per wave
-  __shared__ __attribute__((aligned(16))) uint32_t tile_sc[2][KIND == 2 ? 64 : 1];  // E8M0 row scales (E4M3)
+  constexpr int NB = scan_buffers(KIND, KS, R, W, RESID);
+  // vector memory loads one stage() issues in every wave: the tile chunks and
+  // the thresholds (wave 0 adds the C inputs, E4M3 row scales and list id, so
+  // this is the smallest count of any wave)
+  constexpr int STAGE_OPS = WAVE_LOADS + 1;
+  __shared__ __attribute__((aligned(16))) char tile[NB][TILE_BYTES];
+  __shared__ __attribute__((aligned(16))) float tile_hx[NB][64];      // one dword per lane of wave 0
+  __shared__ __attribute__((aligned(16))) uint32_t thr_lds[NB][W][64];  // published thresholds, per wave
+  __shared__ __attribute__((aligned(16))) uint32_t tile_sc[NB][KIND == 2 ? 64 : 1];  // E8M0 row scales (E4M3)
+  // residual operand: the tile's list, copied with the tile, so that reading it
+  // waits on the LDS counter only (a vector load here would wait for the copy
+  // of the next tile too)
+  __shared__ __attribute__((aligned(16))) uint32_t tile_li[NB][RESID ? 64 : 1];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -318,6 +375,7 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
   const uint32_t lds_hx = (uint32_t)(uintptr_t)&tile_hx[0][0];
   const uint32_t lds_thr = (uint32_t)(uintptr_t)&thr_lds[0][0][0];
   const uint32_t lds_sc = (uint32_t)(uintptr_t)&tile_sc[0][0];
+  const uint32_t lds_li = (uint32_t)(uintptr_t)&tile_li[0][0];
   const uint32_t* gthr_src = a.gthr + q0 + (lane < QPW ? lane : QPW - 1);
   auto stage = [&](int b, int64_t t) {
     const char* base = xop + size_t(t) * kp_bytes;
@@ -332,21 +390,44 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
         const int w = lane < BN / 4 ? lane : BN / 4 - 1;
         lds_dma4(__builtin_amdgcn_readfirstlane(lds_sc + b * 64 * 4), reinterpret_cast<const uint32_t*>(a.xsc + t) + w);
       }
+      if constexpr (RESID)  // every lane copies the same word: the list of the tile's 64-row block
+        lds_dma4(__builtin_amdgcn_readfirstlane(lds_li + b * 64 * 4),
+                 reinterpret_cast<const uint32_t*>(a.tile_list) + (t >> 6));
     }
     // the thresholds other splits published for this wave's queries ride along
     lds_dma4(__builtin_amdgcn_readfirstlane(lds_thr + (b * W + wave) * 256), gthr_src);
   };
 
+  // Loads of one wave complete in the order they were issued, so when at most
+  // STAGE_OPS loads are outstanding, every stage before the last one has
+  // landed, whatever stores (appends, merges) are still pending: those only add
+  // to the count.  (Wave 0 may wait for a few chunks of the last stage too.)
+  auto wait_all_but_last_stage = [&]() { wait_vmcnt<STAGE_OPS>(); };
   if (r0 < r1) stage(0, r0);
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  if (r0 + BN < r1) stage(1, r0 + BN);
+  if constexpr (NB == 3) {
+    if (r0 + BN < r1) {
+      stage(1, r0 + BN);
+      wait_all_but_last_stage();
+    } else {
+      __builtin_amdgcn_s_waitcnt(0);
+    }
+    __syncthreads();
+  } else {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (r0 + BN < r1) stage(1, r0 + BN);
+  }
   int buf = 0;
   for (int64_t t = r0; t < r1; t += BN) {
+    // three buffers: the one of tile t - BN is free (every wave passed the last
+    // barrier), and takes tile t + 2 BN now
+    if constexpr (NB == 3) {
+      if (t + 2 * BN < r1) stage(buf == 0 ? 2 : buf - 1, t + 2 * BN);
+    }
     const char* tb = tile[buf];
     const float* hb = tile_hx[buf];
     if constexpr (RESID) {
-      const int jl = __builtin_amdgcn_readfirstlane(a.tile_list[t >> 6]);
+      const int jl = __builtin_amdgcn_readfirstlane((int)tile_li[buf][0]);
       if (jl != cur_list) {
         // B operand of this list: bf16(fl32(q - c_j)) of the lane's 8 coordinates
         // per K step (16 s + 8 half ...), zero beyond d; padding query slots
@@ -515,15 +596,22 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
       if (appended) MERGE_WHERE(g, cnt[g] > csub - 16 * R);
     }
     // The next tile (and the thresholds that came with it) have landed for
-    // every wave; this tile's buffer is free again and takes the tile after next.
-    __builtin_amdgcn_s_waitcnt(0);
+    // every wave; with two buffers this tile's one is free again and takes the
+    // tile after next.
+    const int nbuf = buf + 1 == NB ? 0 : buf + 1;
+    if (NB == 3 && t + 2 * BN < r1)
+      wait_all_but_last_stage();  // the stage of tile t + 2 BN may still be in flight
+    else
+      __builtin_amdgcn_s_waitcnt(0);
     if (t + BN < r1) {
 #pragma unroll
-      for (int g = 0; g < A; ++g) thr[g] = fminf(thr[g], unord_f32(thr_lds[buf ^ 1][wave][g * 32 + col]));
+      for (int g = 0; g < A; ++g) thr[g] = fminf(thr[g], unord_f32(thr_lds[nbuf][wave][g * 32 + col]));
     }
     __syncthreads();
-    if (t + 2 * BN < r1) stage(buf, t + 2 * BN);
-    buf ^= 1;
+    if constexpr (NB == 2) {
+      if (t + 2 * BN < r1) stage(buf, t + 2 * BN);
+    }
+    buf = nbuf;
   }
 #pragma unroll
   for (int g = 0; g < A; ++g) MERGE_WHERE(g, cnt[g] > 0);
@@ -608,27 +696,6 @@ __global__ __launch_bounds__(256, 4) void scan_mfma_batch_kernel_4wg(ScanBatchAr
   scan_mfma_batch_body<KIND, KS, A, R, V, RESID>(b);
 }
 
-// Shape of a K step count: query groups per wave (A), 32-row blocks per tile
-// (R), two waves per SIMD or one.  Plain bf16 shapes of 14 and 16 steps take
-// 32-row tiles so that two waves per SIMD fit (their query fragments take 128
-// registers); beyond 16 steps one query group per wave.
-// The residual variants keep 64-row tiles and one wave per SIMD from 14 steps
-// (their list-change rebuild of the fragments leaves no room for two).
-constexpr int scan_groups(int ks) { return ks <= 16 ? 2 : 1; }
-// (E4M3, kind 2: 64-row tiles; two waves per SIMD up to 4 32-byte steps, whose
-// hi and lo fragments then fit without spills.)
-constexpr bool scan_two_waves(int kind, int ks, bool resid) {
-  return kind == 0 ? ks <= (resid ? 12 : 16) : (kind == 1 ? ks <= 8 : ks <= 4);
-}
-// Shapes whose fragments fit 128 registers run four workgroups (four waves per
-// SIMD): at few K steps the selection, not the MFMA, bounds the scan, and more
-// waves hide its latency.  They take 32-row tiles (half the accumulators) so
-// that the 128 registers hold them without spills.
-constexpr bool scan_four_wg(int kind, int ks) { return kind == 0 ? ks <= 4 : (kind == 1 && ks <= 2); }
-constexpr int scan_row_blocks(int kind, int ks, bool resid) {
-  return scan_four_wg(kind, ks) ? 1 : ((kind == 0 && !resid ? ks <= 12 : ks <= 16) ? 2 : 1);
-}
-
 // K step buckets: bf16 covers 16 coordinates per step, f32 covers 8.  Up to 16
 // steps a wave holds 2 query groups and the tile has 64 rows; beyond, 1 query
 // group and 32-row tiles keep the query fragments at <= 256 registers and the
@@ -659,9 +726,9 @@ int mfma_rows_per_tile(int kind, int ksteps, bool resid) { return 32 * scan_row_
 int mfma_workgroups_per_cu(int kind, int ksteps, int waves, bool resid) {
   const int per = scan_four_wg(kind, ksteps) ? 4 : (scan_two_waves(kind, ksteps, resid) ? 2 : 1);
   const int by_waves = waves == 1 ? 4 * std::min(per, 2) : per;
-  const int lds = scan_lds_bytes(ksteps, scan_row_blocks(kind, ksteps, resid), waves == 1 ? 1 : 4, false) +
-                  scan_lds_extra(kind);
-  const int by_lds = kLdsPerCu / ((lds + kLdsGranule - 1) / kLdsGranule * kLdsGranule);
+  const int R = scan_row_blocks(kind, ksteps, resid), W = waves == 1 ? 1 : 4;
+  const int nb = scan_buffers(kind, ksteps, R, W, resid);
+  const int by_lds = scan_wgs_by_lds(scan_lds_bytes(ksteps, R, W, false, nb) + scan_lds_extra(kind, nb, resid));
   return std::max(1, std::min(by_waves, by_lds));
 }
 bool mfma_one_wave_supported(int kind, int ksteps, int V, bool resid) {
