@@ -300,17 +300,78 @@ int32_t fuel_amd_timing_read(fuel_amd_index* ix, double* kernel_ms, int64_t* lau
 
 int32_t fuel_amd_last_stats(const fuel_amd_index* ix, fuel_amd_stats* out) {
   if (!ix || !out) return fail(FUEL_AMD_ERR_INVALID, "NULL argument");
-  *out = ix->ctx.stats;
+  auto* mix = const_cast<fuel_amd_index*>(ix);
+  std::lock_guard<std::mutex> lock(mix->stats_mu);
+  *out = ix->last_stats;
   return FUEL_AMD_OK;
 }
 
 int32_t fuel_amd_last_fallback_rows(const fuel_amd_index* ix, int64_t* rows, int64_t cap, int64_t* count) {
   if (!ix || !count || (cap > 0 && !rows) || cap < 0) return fail(FUEL_AMD_ERR_INVALID, "bad argument");
-  const auto& v = ix->ctx.fail_rows;
+  auto* mix = const_cast<fuel_amd_index*>(ix);
+  std::lock_guard<std::mutex> lock(mix->stats_mu);
+  const auto& v = ix->last_fail_rows;
   *count = (int64_t)v.size();
   for (int64_t i = 0; i < cap && i < (int64_t)v.size(); ++i) rows[i] = v[i];
   return FUEL_AMD_OK;
 }
+
+namespace {
+
+// The context a kNN call runs on: the index's own (with its lock held) when
+// free, otherwise an idle lane (made when none is idle), returned on release.
+struct CallCtx {
+  fuel_amd_index* ix = nullptr;
+  std::unique_lock<std::mutex> own;
+  fuel_amd_index::CallLane* lane = nullptr;
+  fa::QueryCtx* ctx = nullptr;
+  hipStream_t stream = nullptr;
+  fa::DevBuf *upload = nullptr, *hq = nullptr, *hout_idx = nullptr, *hout_dist = nullptr;
+
+  int32_t acquire(fuel_amd_index* index) {
+    ix = index;
+    own = std::unique_lock<std::mutex>(ix->mu, std::try_to_lock);
+    if (own.owns_lock()) {
+      ctx = &ix->ctx;
+      stream = ix->stream;
+      upload = &ix->upload;
+      hq = &ix->hq;
+      hout_idx = &ix->hout_idx;
+      hout_dist = &ix->hout_dist;
+      return FUEL_AMD_OK;
+    }
+    {
+      std::lock_guard<std::mutex> lock(ix->lanes_mu);
+      if (!ix->lanes_idle.empty()) {
+        lane = ix->lanes_idle.back();
+        ix->lanes_idle.pop_back();
+      }
+    }
+    if (!lane) {
+      auto L = std::make_unique<fuel_amd_index::CallLane>();
+      hipError_t e = hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking);
+      if (e != hipSuccess) return hip_fail(e, "call lane stream");
+      lane = L.get();
+      std::lock_guard<std::mutex> lock(ix->lanes_mu);
+      ix->lanes.push_back(std::move(L));
+    }
+    ctx = &lane->ctx;
+    stream = lane->stream;
+    upload = &lane->upload;
+    hq = &lane->hq;
+    hout_idx = &lane->hout_idx;
+    hout_dist = &lane->hout_dist;
+    return FUEL_AMD_OK;
+  }
+  ~CallCtx() {
+    if (lane) {
+      std::lock_guard<std::mutex> lock(ix->lanes_mu);
+      ix->lanes_idle.push_back(lane);
+    }
+  }
+};
+
+}  // namespace
 
 // ---- kNN --------------------------------------------------------------------
 
@@ -321,11 +382,13 @@ int32_t fuel_amd_knn_device(fuel_amd_index* ix, const void* d_query, int64_t m, 
   if (m < 0 || k < 0) return fail(FUEL_AMD_ERR_INVALID, "m and k must be non-negative");
   if (exclude_self < 0 || exclude_self > 2) return fail(FUEL_AMD_ERR_INVALID, "bad exclude_self");
   if (ldq < ix->d) return fail(FUEL_AMD_ERR_INVALID, "ldq must be >= d");
-  std::lock_guard<std::mutex> lock(ix->mu);
   int32_t rc = check_device(ix->device);
   if (rc) return rc;
-  hipStream_t st = stream ? (hipStream_t)stream : ix->stream;
-  return knn_run(ix, &ix->ctx, d_query, m, ldq, k, exclude_self, d_out_idx, d_out_dist, st);
+  CallCtx cc;
+  rc = cc.acquire(ix);
+  if (rc) return rc;
+  hipStream_t st = stream ? (hipStream_t)stream : cc.stream;
+  return knn_run(ix, cc.ctx, d_query, m, ldq, k, exclude_self, d_out_idx, d_out_dist, st);
 }
 
 int32_t fuel_amd_knn(fuel_amd_index* ix, const void* query, int64_t m, int32_t k, int32_t exclude_self,
@@ -336,23 +399,25 @@ int32_t fuel_amd_knn(fuel_amd_index* ix, const void* query, int64_t m, int32_t k
   if (exclude_self < 0 || exclude_self > 2) return fail(FUEL_AMD_ERR_INVALID, "bad exclude_self");
   if (m == 0 || k == 0) return FUEL_AMD_OK;
   if (!query || !out_idx || !out_dist) return fail(FUEL_AMD_ERR_INVALID, "NULL buffer");
-  std::lock_guard<std::mutex> lock(ix->mu);
   int32_t rc = check_device(ix->device);
   if (rc) return rc;
+  CallCtx cc;
+  rc = cc.acquire(ix);
+  if (rc) return rc;
   const size_t es = ix->dtype == FUEL_AMD_F32 ? 4 : 8;
-  hipStream_t st = ix->stream;
-  hipError_t e = ix->hq.ensure(size_t(m) * ix->ld * es);
-  if (e == hipSuccess) e = ix->hout_idx.ensure(size_t(m) * k * 8);
-  if (e == hipSuccess) e = ix->hout_dist.ensure(size_t(m) * k * es);
+  hipStream_t st = cc.stream;
+  hipError_t e = cc.hq->ensure(size_t(m) * ix->ld * es);
+  if (e == hipSuccess) e = cc.hout_idx->ensure(size_t(m) * k * 8);
+  if (e == hipSuccess) e = cc.hout_dist->ensure(size_t(m) * k * es);
   if (e != hipSuccess) return hip_fail(e, "query buffers");
   if (ix->d > 0) {
-    e = fa::upload_rows(ix->hq.p, ix->ld, query, m, ix->d, es, ix->upload, st);
+    e = fa::upload_rows(cc.hq->p, ix->ld, query, m, ix->d, es, *cc.upload, st);
     if (e != hipSuccess) return hip_fail(e, "query upload");
   }
-  rc = knn_run(ix, &ix->ctx, ix->hq.p, m, ix->ld, k, exclude_self, (int64_t*)ix->hout_idx.p, ix->hout_dist.p, st);
+  rc = knn_run(ix, cc.ctx, cc.hq->p, m, ix->ld, k, exclude_self, (int64_t*)cc.hout_idx->p, cc.hout_dist->p, st);
   if (rc) return rc;
-  e = hipMemcpyAsync(out_idx, ix->hout_idx.p, size_t(m) * k * 8, hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(out_dist, ix->hout_dist.p, size_t(m) * k * es, hipMemcpyDeviceToHost, st);
+  e = hipMemcpyAsync(out_idx, cc.hout_idx->p, size_t(m) * k * 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(out_dist, cc.hout_dist->p, size_t(m) * k * es, hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return hip_fail(e, "result download");
   return FUEL_AMD_OK;

@@ -4,6 +4,7 @@
 
 #include <atomic>
 #include <cstdint>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -62,6 +63,7 @@ struct QueryCtx {
   fuel_amd_stats stats{};
   std::vector<int64_t> fail_rows;  // query rows whose first-level certificate failed (last call)
   bool timed = false;  // record kernel timing events (the index's own context)
+  bool publish = true;  // knn_run copies stats and fail rows to the index's last-call record
   void release() {
     for (auto& w : lv) {
       w.Qop.release();
@@ -127,6 +129,22 @@ struct fuel_amd_index {
   std::mutex auto_mu;
   int64_t auto_probe_rows = 0, auto_probe_fail = 0;
   fa::QueryCtx ctx;  // the synchronous entry points' query workspaces and stats
+  // Concurrent kNN calls (fuel's SearchIndex is shared by many threads): the
+  // index's own context, stream and staging serve one call at a time under
+  // `mu`; a kNN call that finds them busy takes a lane of its own (context,
+  // stream, staging), made on first need and kept for later calls.
+  struct CallLane {
+    fa::QueryCtx ctx;
+    hipStream_t stream = nullptr;
+    fa::DevBuf upload, hq, hout_idx, hout_dist;
+  };
+  std::mutex lanes_mu;
+  std::vector<std::unique_ptr<CallLane>> lanes;  // every lane made
+  std::vector<CallLane*> lanes_idle;
+  // the last finished kNN call's statistics and failed rows (any context)
+  std::mutex stats_mu;
+  fuel_amd_stats last_stats{};
+  std::vector<int64_t> last_fail_rows;
   // profiling: event pairs around the dominant kernel of each call
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;
@@ -138,6 +156,20 @@ struct fuel_amd_index {
     hout_idx.release();
     hout_dist.release();
     ctx.release();
+    for (auto& L : lanes) {
+      if (L->stream) {
+        (void)hipStreamSynchronize(L->stream);
+        (void)hipStreamDestroy(L->stream);
+        L->stream = nullptr;
+      }
+      L->ctx.release();
+      L->upload.release();
+      L->hq.release();
+      L->hout_idx.release();
+      L->hout_dist.release();
+    }
+    lanes_idle.clear();
+    lanes.clear();
     ws_a.release();
     ws_b.release();
     ws_c.release();

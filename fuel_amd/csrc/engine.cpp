@@ -499,6 +499,11 @@ int32_t knn_run(fuel_amd_index* ix, QueryCtx* ctx, const void* d_query, int64_t 
   }
   ctx->stats.ms_total =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (ctx->publish) {
+    std::lock_guard<std::mutex> lock(ix->stats_mu);
+    ix->last_stats = ctx->stats;
+    ix->last_fail_rows = ctx->fail_rows;
+  }
   return rc;
 }
 

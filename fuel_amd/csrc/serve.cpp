@@ -260,6 +260,7 @@ int32_t fuel_amd_server_create(fuel_amd_index* ix, const fuel_amd_server_opts* o
   const int nw = o.workers > 0 ? o.workers : 2;
   for (int i = 0; i < nw; ++i) {
     auto w = std::make_unique<Worker>();
+    w->ctx.publish = false;  // served calls leave the index's last-call record alone
     hipError_t e = hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
       fuel_amd_server_destroy(s.release());

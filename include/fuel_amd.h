@@ -20,8 +20,12 @@
  *     device pointers and a hipStream_t (as void*) and enqueue their work on
  *     it; fuel_amd_knn_device in an MFMA mode also waits for the stream once
  *     per query chunk (see there).
- *   - an index is immutable after creation; calls on one index from several
- *     threads are serialized internally.
+ *   - an index is immutable after creation.  kNN calls (fuel_amd_knn,
+ *     fuel_amd_knn_device) on one index from several threads run concurrently,
+ *     each on a context and stream of its own (fuel's SearchIndex is shared by
+ *     rayon's worker threads); the other entry points on one index are
+ *     serialized internally.  fuel_amd_last_stats reports the last kNN call to
+ *     finish.
  */
 #ifndef FUEL_AMD_H
 #define FUEL_AMD_H
