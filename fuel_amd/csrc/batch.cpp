@@ -86,6 +86,12 @@ struct Bucket {
   int64_t range_start[SCAN_XCDS + 1] = {};  // item ranges of about equal work, one per XCD
   size_t pool_bytes = 0;
   DevBuf probs_dev, starts_dev;
+  // grouped per-problem stages (one launch per group): resets, operand copies
+  // (plain operands), query preparation, seed selection, rerank
+  DevBuf group_dev;
+  const FillEntry* fills = nullptr;
+  int nfill = 0, sel_vmax = 4;
+  GroupTab g_db{}, g_q{}, g_sel{}, g_rr{};
 };
 
 }  // namespace
@@ -140,6 +146,12 @@ template <typename T>
 T* at(const DevBuf& b, size_t off) {
   return reinterpret_cast<T*>(static_cast<char*>(b.p) + off);
 }
+
+PrepDbArgs prep_db_args(fuel_amd_batch* b, int i);
+PrepQArgs prep_q_args(fuel_amd_batch* b, int i);
+SeedArgs seed_args(fuel_amd_batch* b, int i);
+RerankArgs rerank_args(fuel_amd_batch* b, int i, int V);
+int32_t build_group_tables(fuel_amd_batch* b, Bucket& B);
 
 int32_t plan_batch(fuel_amd_batch* b, const std::map<int, Cfg>& cfg_by_d) {
   const int cus = device_cu_count(b->device);
@@ -367,8 +379,86 @@ int32_t plan_batch(fuel_amd_batch* b, const std::map<int, Cfg>& cfg_by_d) {
                        b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) return hip_fail(e, "batch tables");
+    const int32_t rc = build_group_tables(b, B);
+    if (rc) return rc;
   }
   b->h_fail.assign(np, 0);
+  return FUEL_AMD_OK;
+}
+
+// Argument tables of a group's grouped stages, in one device buffer: fill
+// entries, then per stage the problems' arguments and their block starts.
+int32_t build_group_tables(fuel_amd_batch* b, Bucket& B) {
+  std::vector<FillEntry> fills;
+  std::vector<PrepDbArgs> dbs;
+  std::vector<PrepQArgs> qs;
+  std::vector<SeedArgs> sels;
+  std::vector<RerankArgs> rrs;
+  std::vector<int64_t> db_start{0}, q_start{0}, sel_start{0}, rr_start{0};
+  int vmax = 1;
+  for (int i : B.members) {
+    const auto& pp = b->plan[i];
+    const auto& pr = b->probs[i];
+    const PrepDbArgs pa = prep_db_args(b, i);
+    if (pp.kind != 2) {
+      fills.push_back({reinterpret_cast<uint32_t*>(pa.consts), 16, 0u, 0u});
+      fills.push_back({reinterpret_cast<uint32_t*>(pa.flags), 2, 1u, 0u});
+      dbs.push_back(pa);
+      db_start.push_back(db_start.back() + (pa.n_pad + 63) / 64);
+    } else {
+      fills.push_back({reinterpret_cast<uint32_t*>(pa.consts), 32, 0u, 0u});  // constants, zero flags
+      fills.push_back({at<uint32_t>(b->pool, pp.o_lconsts), int64_t(16) * b->parts[i]->L, 0u, 0u});
+    }
+    fills.push_back({reinterpret_cast<uint32_t*>(b->fail_counts.as<int>() + i), 1, 0u, 0u});
+    const PrepQArgs qa = prep_q_args(b, i);
+    qs.push_back(qa);
+    q_start.push_back(q_start.back() + (qa.m_pad + 3) / 4);
+    if (pp.seed.ntiles > 0) {
+      sels.push_back(seed_args(b, i));
+      sel_start.push_back(sel_start.back() + pr.n);
+      vmax = std::max(vmax, (pp.seed.parts * 2 * SEED_GROUPS + 63) / 64);
+    }
+    rrs.push_back(rerank_args(b, i, B.V));
+    rr_start.push_back(rr_start.back() + pr.n);
+  }
+  fills.push_back({b->counter.as<uint32_t>(), int64_t(SCAN_XCDS) * SCAN_COUNTER_STRIDE, 0u, 0u});
+  if (vmax > 16) return fail(FUEL_AMD_ERR_DEVICE, "internal error: seed minima beyond the selection's registers");
+  std::vector<char> host;
+  auto put = [&](const void* src, size_t bytes) {
+    const size_t off = align256(host.size());
+    host.resize(off + bytes);
+    if (bytes) std::memcpy(host.data() + off, src, bytes);
+    return off;
+  };
+  const size_t o_fill = put(fills.data(), fills.size() * sizeof(FillEntry));
+  const size_t o_db = put(dbs.data(), dbs.size() * sizeof(PrepDbArgs));
+  const size_t o_dbs = put(db_start.data(), db_start.size() * 8);
+  const size_t o_q = put(qs.data(), qs.size() * sizeof(PrepQArgs));
+  const size_t o_qs = put(q_start.data(), q_start.size() * 8);
+  const size_t o_sel = put(sels.data(), sels.size() * sizeof(SeedArgs));
+  const size_t o_sels = put(sel_start.data(), sel_start.size() * 8);
+  const size_t o_rr = put(rrs.data(), rrs.size() * sizeof(RerankArgs));
+  const size_t o_rrs = put(rr_start.data(), rr_start.size() * 8);
+  hipError_t e = B.group_dev.ensure(std::max<size_t>(host.size(), 256));
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(B.group_dev.p, host.data(), host.size(), hipMemcpyHostToDevice, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  if (e != hipSuccess) return hip_fail(e, "batch group tables");
+  auto tab = [&](size_t o_tab, size_t o_start, size_t np, const std::vector<int64_t>& start) {
+    GroupTab g{};
+    g.tab = at<void>(B.group_dev, o_tab);
+    g.start = at<int64_t>(B.group_dev, o_start);
+    g.np = (int)np;
+    g.nblocks = start.back();
+    return g;
+  };
+  B.fills = at<FillEntry>(B.group_dev, o_fill);
+  B.nfill = (int)fills.size();
+  B.g_db = tab(o_db, o_dbs, dbs.size(), db_start);
+  B.g_q = tab(o_q, o_qs, qs.size(), q_start);
+  B.g_sel = tab(o_sel, o_sels, sels.size(), sel_start);
+  B.g_rr = tab(o_rr, o_rrs, rrs.size(), rr_start);
+  B.sel_vmax = vmax;
   return FUEL_AMD_OK;
 }
 
@@ -391,19 +481,23 @@ PrepDbArgs prep_db_args(fuel_amd_batch* b, int i) {
 }
 
 // Operand copy of problem i (and its bound constants, reset first).
-hipError_t enqueue_operand(fuel_amd_batch* b, int i, hipStream_t st) {
+// (reset: the bound constants are reset here first; the grouped pass resets
+// them in its fill launch)
+hipError_t enqueue_operand(fuel_amd_batch* b, int i, hipStream_t st, bool reset) {
   const auto& pp = b->plan[i];
   const auto& pr = b->probs[i];
   const PrepDbArgs pa = prep_db_args(b, i);
   if (pp.kind != 2) {
-    hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t*>(pa.consts), 16, 0u, st);
-    if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(pa.flags), 2, 1u, st);
+    hipError_t e = hipSuccess;
+    if (reset) e = launch_fill_u32(reinterpret_cast<uint32_t*>(pa.consts), 16, 0u, st);
+    if (reset && e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(pa.flags), 2, 1u, st);
     if (e == hipSuccess) e = launch_prep_db(pa, st);
     return e;
   }
   const auto& part = *b->parts[i];
-  hipError_t e = launch_fill_u32(reinterpret_cast<uint32_t*>(pa.consts), 32, 0u, st);  // constants, zero flags
-  if (e == hipSuccess) e = launch_fill_u32(at<uint32_t>(b->pool, pp.o_lconsts), 16 * part.L, 0u, st);
+  hipError_t e = hipSuccess;
+  if (reset) e = launch_fill_u32(reinterpret_cast<uint32_t*>(pa.consts), 32, 0u, st);  // constants, zero flags
+  if (reset && e == hipSuccess) e = launch_fill_u32(at<uint32_t>(b->pool, pp.o_lconsts), 16 * part.L, 0u, st);
   PrepResidArgs r{};
   r.X = static_cast<const float*>(pr.data);
   r.ldx = pr.ld;
@@ -423,91 +517,158 @@ hipError_t enqueue_operand(fuel_amd_batch* b, int i, hipStream_t st) {
 }
 
 // Level 0 of one group on stream st (no host synchronisation: capturable).
+PrepQArgs prep_q_args(fuel_amd_batch* b, int i) {
+  const auto& pp = b->plan[i];
+  const auto& pr = b->probs[i];
+  PrepQArgs qa{};
+  qa.Q = static_cast<const float*>(pr.data);
+  qa.ldq = pr.ld;
+  qa.m = pr.n;
+  qa.m_pad = pp.nq_pad;
+  qa.d = pr.d;
+  qa.Kp = pp.Kp;
+  qa.bf16 = pp.kind != 1;
+  qa.Qop = pp.kind == 2 ? nullptr : at<void>(b->pool, pp.o_qop);
+  qa.Q32 = at<float>(b->pool, pp.o_q32);
+  qa.ld32 = pr.ld;
+  qa.qn2 = at<double>(b->pool, pp.o_qn);
+  qa.eq2 = qa.qn2 + pr.n;
+  qa.qt2 = qa.eq2 + pr.n;
+  qa.qint = at<int>(b->pool, pp.o_qint);
+  qa.gthr = at<uint32_t>(b->pool, pp.o_gthr);
+  return qa;
+}
+
+SeedArgs seed_args(fuel_amd_batch* b, int i) {
+  const auto& pp = b->plan[i];
+  const auto& pr = b->probs[i];
+  SeedArgs sd{};
+  sd.Xop = at<void>(b->pool, pp.o_xop);
+  sd.hx = at<float>(b->pool, pp.o_hx);
+  sd.Kp = pp.Kp;
+  sd.Qop = at<void>(b->pool, pp.o_qop);
+  sd.nq_pad = pp.nq_pad;
+  sd.nqb = (int)pp.nqb;
+  sd.stride = pp.seed.stride;
+  sd.ntiles = pp.seed.ntiles;
+  sd.parts = pp.seed.parts;
+  sd.mins = at<float>(b->pool, pp.o_seed);
+  sd.m = pr.n;
+  sd.KP = pp.KP;
+  sd.gthr = at<uint32_t>(b->pool, pp.o_gthr);
+  return sd;
+}
+
+RerankArgs rerank_args(fuel_amd_batch* b, int i, int V) {
+  const auto& pp = b->plan[i];
+  const auto& pr = b->probs[i];
+  RerankArgs ra{};
+  ra.slots = at<uint64_t>(b->pool, pp.o_slots);
+  ra.lcnts = at<int>(b->pool, pp.o_lcnts);
+  ra.nsplit = pp.nsplit;
+  ra.nq = pr.n;
+  ra.nq_pad = pp.nq_pad;
+  ra.slot_ns = 64 * V;
+  ra.KP = pp.KP;
+  ra.X = static_cast<const float*>(pr.data);
+  ra.ldx = pr.ld;
+  ra.Q32 = at<float>(b->pool, pp.o_q32);
+  ra.ld32 = pr.ld;
+  ra.d = pr.d;
+  ra.metric = b->metric;
+  ra.arith = ARITH_AVX;
+  ra.bf16 = pp.kind != 1;
+  ra.resid = pp.kind == 2 ? 1 : 0;
+  ra.rperm = pp.kind == 2 ? b->parts[i]->rperm.as<uint32_t>() : nullptr;
+  ra.qn2 = at<double>(b->pool, pp.o_qn);
+  ra.eq2 = ra.qn2 + pr.n;
+  ra.qt2 = ra.eq2 + pr.n;
+  ra.qint = at<int>(b->pool, pp.o_qint);
+  ra.consts = at<unsigned long long>(b->pool, pp.o_consts);
+  ra.flags = at<int>(b->pool, pp.o_consts + 64);
+  ra.KK = pp.KK;
+  ra.k = pr.k;
+  ra.mode = b->exclude;
+  ra.qbase = 0;
+  ra.rows = nullptr;
+  ra.out_idx = pr.out_idx;
+  ra.out_dist = static_cast<float*>(pr.out_dist);
+  ra.fail_count = b->fail_counts.as<int>() + i;
+  ra.fail_list = at<int64_t>(b->keep, pp.o_fl);
+  ra.gthr = at<uint32_t>(b->pool, pp.o_gthr);
+  ra.fail_theta = at<float>(b->keep, pp.o_ft);
+  return ra;
+}
+
+// Residual problem i after its query preparation: query-to-centroid norms, the
+// per-(query, list) terms and proven start, and the sampled threshold.
+hipError_t enqueue_resid_queries(fuel_amd_batch* b, int i, hipStream_t st) {
+  const auto& pp = b->plan[i];
+  const auto& pr = b->probs[i];
+  const auto& part = *b->parts[i];
+  const PrepQArgs qa = prep_q_args(b, i);
+  KmeansPairwiseArgs<float> pw{};
+  pw.P1 = qa.Q32;
+  pw.ld1 = pr.ld;
+  pw.m = pr.n;
+  pw.P2 = part.cents.as<float>();
+  pw.ld2 = pr.d;
+  pw.n = part.L;
+  pw.d = pr.d;
+  pw.out = at<float>(b->pool, pp.o_qs2);
+  pw.ldo = part.L;
+  hipError_t e = launch_kmeans_pairwise<float>(pw, st);
+  QListArgs la{};
+  la.s2 = pw.out;
+  la.m = pr.n;
+  la.nq_pad = pp.nq_pad;
+  la.d = pr.d;
+  la.L = part.L;
+  la.lconsts = at<unsigned long long>(b->pool, pp.o_lconsts);
+  la.qtab = at<float>(b->pool, pp.o_qtab);
+  la.lcount = part.lcount.as<uint32_t>();
+  la.KP = pp.KP;
+  la.gthr = qa.gthr;
+  if (e == hipSuccess) e = launch_prep_qlist(la, st);
+  // the sampled threshold: exact distances to the nearest list's first rows
+  SampleSeedArgs ss{};
+  ss.s2 = pw.out;
+  ss.m = pr.n;
+  ss.L = part.L;
+  ss.KP = pp.KP;
+  ss.lcount = la.lcount;
+  ss.loff = part.loff.as<int64_t>();
+  ss.rperm = part.rperm.as<uint32_t>();
+  ss.X = static_cast<const float*>(pr.data);
+  ss.ldx = pr.ld;
+  ss.Q32 = qa.Q32;
+  ss.ld32 = pr.ld;
+  ss.d = pr.d;
+  ss.gthr = la.gthr;
+  if (e == hipSuccess) e = launch_seed_sample(ss, st);
+  return e;
+}
+
+// Level 0 of one group on stream st (no host synchronisation: capturable).
+// Stage by stage over the group's problems: resets (bound constants, flags,
+// residual list constants, certification counters and the scan's work
+// counters: one fill launch), operand copies and query preparation (one
+// launch each; residual operands per problem), seed passes (per problem) and
+// their selection (one launch), the persistent scan, the reranks (one launch).
 hipError_t enqueue_bucket(fuel_amd_batch* b, const Bucket& B, hipStream_t st) {
-  hipError_t e = hipSuccess;
+  hipError_t e = launch_fill_multi(B.fills, B.nfill, st);
+  if (e == hipSuccess) e = launch_prep_db_group(B.g_db, st);
+  for (int i : B.members)
+    if (e == hipSuccess && b->plan[i].kind == 2) e = enqueue_operand(b, i, st, false);
+  if (e == hipSuccess) e = launch_prep_queries_group(B.g_q, st);
   for (int i : B.members) {
-    const auto& pp = b->plan[i];
-    const auto& pr = b->probs[i];
-    e = enqueue_operand(b, i, st);
-    PrepQArgs qa{};
-    qa.Q = static_cast<const float*>(pr.data);
-    qa.ldq = pr.ld;
-    qa.m = pr.n;
-    qa.m_pad = pp.nq_pad;
-    qa.d = pr.d;
-    qa.Kp = pp.Kp;
-    qa.bf16 = pp.kind != 1;
-    qa.Qop = pp.kind == 2 ? nullptr : at<void>(b->pool, pp.o_qop);
-    qa.Q32 = at<float>(b->pool, pp.o_q32);
-    qa.ld32 = pr.ld;
-    qa.qn2 = at<double>(b->pool, pp.o_qn);
-    qa.eq2 = qa.qn2 + pr.n;
-    qa.qt2 = qa.eq2 + pr.n;
-    qa.qint = at<int>(b->pool, pp.o_qint);
-    qa.gthr = at<uint32_t>(b->pool, pp.o_gthr);
-    if (e == hipSuccess) e = launch_prep_queries(qa, st);
-    if (pp.seed.ntiles > 0 && e == hipSuccess) {
-      SeedArgs sd{};
-      sd.Xop = at<void>(b->pool, pp.o_xop);
-      sd.hx = at<float>(b->pool, pp.o_hx);
-      sd.Kp = pp.Kp;
-      sd.Qop = qa.Qop;
-      sd.nq_pad = pp.nq_pad;
-      sd.nqb = (int)pp.nqb;
-      sd.stride = pp.seed.stride;
-      sd.ntiles = pp.seed.ntiles;
-      sd.parts = pp.seed.parts;
-      sd.mins = at<float>(b->pool, pp.o_seed);
-      sd.m = pr.n;
-      sd.KP = pp.KP;
-      sd.gthr = qa.gthr;
-      e = launch_seed_mfma(sd, pp.kind, pp.ks, st);
-    }
-    if (pp.kind == 2 && e == hipSuccess) {
-      KmeansPairwiseArgs<float> pw{};
-      pw.P1 = qa.Q32;
-      pw.ld1 = pr.ld;
-      pw.m = pr.n;
-      pw.P2 = b->parts[i]->cents.as<float>();
-      pw.ld2 = pr.d;
-      pw.n = b->parts[i]->L;
-      pw.d = pr.d;
-      pw.out = at<float>(b->pool, pp.o_qs2);
-      pw.ldo = b->parts[i]->L;
-      e = launch_kmeans_pairwise<float>(pw, st);
-      QListArgs la{};
-      la.s2 = pw.out;
-      la.m = pr.n;
-      la.nq_pad = pp.nq_pad;
-      la.d = pr.d;
-      la.L = b->parts[i]->L;
-      la.lconsts = at<unsigned long long>(b->pool, pp.o_lconsts);
-      la.qtab = at<float>(b->pool, pp.o_qtab);
-      la.lcount = b->parts[i]->lcount.as<uint32_t>();
-      la.KP = pp.KP;
-      la.gthr = at<uint32_t>(b->pool, pp.o_gthr);
-      if (e == hipSuccess) e = launch_prep_qlist(la, st);
-      // the sampled threshold: exact distances to the nearest list's first rows
-      SampleSeedArgs ss{};
-      ss.s2 = pw.out;
-      ss.m = pr.n;
-      ss.L = b->parts[i]->L;
-      ss.KP = pp.KP;
-      ss.lcount = la.lcount;
-      ss.loff = b->parts[i]->loff.as<int64_t>();
-      ss.rperm = b->parts[i]->rperm.as<uint32_t>();
-      ss.X = static_cast<const float*>(pr.data);
-      ss.ldx = pr.ld;
-      ss.Q32 = qa.Q32;
-      ss.ld32 = pr.ld;
-      ss.d = pr.d;
-      ss.gthr = la.gthr;
-      if (e == hipSuccess) e = launch_seed_sample(ss, st);
-    }
-    if (e == hipSuccess) e = launch_fill_u32(reinterpret_cast<uint32_t*>(b->fail_counts.as<int>() + i), 1, 0u, st);
     if (e != hipSuccess) return e;
+    const auto& pp = b->plan[i];
+    if (pp.seed.ntiles > 0) e = launch_seed_mfma(seed_args(b, i), pp.kind, pp.ks, st, false);
+    if (pp.kind == 2 && e == hipSuccess) e = enqueue_resid_queries(b, i, st);
   }
-  e = launch_fill_u32(b->counter.as<uint32_t>(), SCAN_XCDS * SCAN_COUNTER_STRIDE, 0u, st);
+  if (e == hipSuccess) e = launch_seed_select_group(B.g_sel, B.sel_vmax, st);
+  if (e != hipSuccess) return e;
   ScanBatchArgs sb{};
   sb.probs = B.probs_dev.as<ScanMfmaArgs>();
   sb.item_start = B.starts_dev.as<int64_t>();
@@ -517,46 +678,7 @@ hipError_t enqueue_bucket(fuel_amd_batch* b, const Bucket& B, hipStream_t st) {
   for (int x = 0; x <= SCAN_XCDS; ++x) sb.range_start[x] = B.range_start[x];
   if (e == hipSuccess)
     e = launch_scan_mfma_batch(sb, B.cfg.kind == 2 ? 0 : B.cfg.kind, B.ks, B.V, B.grid, B.cfg.kind == 2, st);
-  for (int i : B.members) {
-    if (e != hipSuccess) return e;
-    const auto& pp = b->plan[i];
-    const auto& pr = b->probs[i];
-    RerankArgs ra{};
-    ra.slots = at<uint64_t>(b->pool, pp.o_slots);
-    ra.lcnts = at<int>(b->pool, pp.o_lcnts);
-    ra.nsplit = pp.nsplit;
-    ra.nq = pr.n;
-    ra.nq_pad = pp.nq_pad;
-    ra.slot_ns = 64 * B.V;
-    ra.KP = pp.KP;
-    ra.X = static_cast<const float*>(pr.data);
-    ra.ldx = pr.ld;
-    ra.Q32 = at<float>(b->pool, pp.o_q32);
-    ra.ld32 = pr.ld;
-    ra.d = pr.d;
-    ra.metric = b->metric;
-    ra.bf16 = pp.kind != 1;
-    ra.resid = pp.kind == 2 ? 1 : 0;
-    ra.rperm = pp.kind == 2 ? b->parts[i]->rperm.as<uint32_t>() : nullptr;
-    ra.qn2 = at<double>(b->pool, pp.o_qn);
-    ra.eq2 = ra.qn2 + pr.n;
-    ra.qt2 = ra.eq2 + pr.n;
-    ra.qint = at<int>(b->pool, pp.o_qint);
-    ra.consts = at<unsigned long long>(b->pool, pp.o_consts);
-    ra.flags = at<int>(b->pool, pp.o_consts + 64);
-    ra.KK = pp.KK;
-    ra.k = pr.k;
-    ra.mode = b->exclude;
-    ra.qbase = 0;
-    ra.rows = nullptr;
-    ra.out_idx = pr.out_idx;
-    ra.out_dist = static_cast<float*>(pr.out_dist);
-    ra.fail_count = b->fail_counts.as<int>() + i;
-    ra.fail_list = at<int64_t>(b->keep, pp.o_fl);
-    ra.gthr = at<uint32_t>(b->pool, pp.o_gthr);
-    ra.fail_theta = at<float>(b->keep, pp.o_ft);
-    e = launch_rerank(ra, B.V, st);
-  }
+  if (e == hipSuccess) e = launch_rerank_group(B.g_rr, B.V, ARITH_AVX, st);
   return e;
 }
 
@@ -593,7 +715,7 @@ int32_t drop_unusable(fuel_amd_batch* b, bool& changed) {
   for (const auto& B : b->buckets) {
     for (int i : B.members) {
       const PrepDbArgs pa = prep_db_args(b, i);
-      hipError_t e = enqueue_operand(b, i, b->stream);
+      hipError_t e = enqueue_operand(b, i, b->stream, true);
       if (e == hipSuccess)
         e = hipMemcpyAsync(&x2[i], pa.consts, sizeof(unsigned long long), hipMemcpyDeviceToHost, b->stream);
       if (e != hipSuccess) return hip_fail(e, "batch operand check");

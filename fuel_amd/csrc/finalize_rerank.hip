@@ -216,12 +216,27 @@ __device__ __forceinline__ void sort_prefix(T (&x)[V], int count) {
   }
 }
 
+// The n-th smallest (from 1) high word of the entries in registers [0, nr) of
+// the wave: the smallest t with #(high word <= t) >= n, by a 32-step search.
+template <int V>
+__device__ __forceinline__ uint32_t wave_kth_hi(const uint64_t (&y)[V], int nr, int n) {
+  uint32_t p = 0;
+  for (int b = 31; b >= 0; --b) {
+    const uint32_t probe = p | ((1u << b) - 1u);
+    int c = 0;
+#pragma unroll
+    for (int v = 0; v < V; ++v)
+      if (v < nr) c += __popcll(__ballot(uint32_t(y[v] >> 32) <= probe));
+    if (c < n) p |= (1u << b);
+  }
+  return p;
+}
+
 template <int V, int ARITH>
-__global__ __launch_bounds__(64) void rerank_kernel(RerankArgs a) {
+__device__ __forceinline__ void rerank_body(const RerankArgs& a, const int64_t qs) {
   using C = Comp<float>;
   using T = uint64_t;
   constexpr int NS = 64 * V;
-  const int64_t qs = blockIdx.x;
   if (qs >= a.nq) return;
   const int lane = __lane_id();
   const int KP = a.KP;
@@ -304,14 +319,27 @@ __global__ __launch_bounds__(64) void rerank_kernel(RerankArgs a) {
   // above it).  At or above KEY_LIMIT no real row was excluded.
   const float T_C = total > KP ? C::key(wave_get<T, V>(x, KP - 1)) : unord_f32(a.gthr[qs]);
 
-  // 2. exact rerank
+  // 2. exact rerank, 64 candidates at a time in the order of their approximate
+  //    keys.  Once the rounds done hold KK exact keys, kk' (the KK-th of them)
+  //    bounds the final KK-th exact key from above; a later candidate whose own
+  //    key~ = s already certifies against kk' (the bound of step 3 with T = s:
+  //    its restated distance orders strictly after kk') cannot enter the top KK
+  //    and is not gathered.  The top KK, kk and everything after are unchanged.
   const float* qrow = a.Q32 + qs * a.ld32;
+  const BoundTerms bt = bound_terms(a, qs);
   T y[V];
+  float kkp = INFINITY;
 #pragma unroll
   for (int v = 0; v < V; ++v) {
     const int e = v * 64 + lane;
     y[v] = C::MAX;
-    if (e < ncand) {
+    if (v * 64 >= ncand) continue;
+    bool eval = e < ncand;
+    if (eval && kkp < INFINITY) {
+      const float s = C::key(x[v]);
+      if (s < MFMA_KEY_LIMIT && (a.resid ? certify_resid(a, bt, s, kkp) : certify(a, bt, s, kkp))) eval = false;
+    }
+    if (eval) {
       uint32_t idx = C::idx(x[v]);
       if (a.resid) idx = a.rperm[idx];  // padded position -> row (padding rows never survive)
       if (idx != RPERM_NONE) {
@@ -320,16 +348,18 @@ __global__ __launch_bounds__(64) void rerank_kernel(RerankArgs a) {
         y[v] = C::make(key, idx);
       }
     }
+    if (!(kkp < INFINITY) && (v + 1) * 64 >= a.KK && (v + 1) * 64 < ncand) {
+      const uint32_t h = wave_kth_hi<V>(y, v + 1, a.KK);  // every entry of rounds 0..v was evaluated
+      if (h != 0xFFFFFFFFu) kkp = unord_f32(h);
+    }
   }
   sort_prefix<T, V>(y, ncand);
 
   // 3. certification
   bool cert = true;
   float kk = INFINITY;
-  BoundTerms bt;
   if (T_C < MFMA_KEY_LIMIT) {
     kk = ncand >= a.KK ? C::key(wave_get<T, V>(y, a.KK - 1)) : INFINITY;
-    bt = bound_terms(a, qs);
     cert = a.resid ? certify_resid(a, bt, T_C, kk) : certify(a, bt, T_C, kk);
   }
   const int64_t orow = a.rows ? a.rows[qs] : qs;
@@ -345,6 +375,36 @@ __global__ __launch_bounds__(64) void rerank_kernel(RerankArgs a) {
       if (a.fail_theta) a.fail_theta[pos] = theta;
     }
   }
+}
+
+template <int V, int ARITH>
+__global__ __launch_bounds__(64) void rerank_kernel(RerankArgs a) {
+  rerank_body<V, ARITH>(a, blockIdx.x);
+}
+
+// Grouped form: one wave per query of every problem of the table.
+template <int V, int ARITH>
+__global__ __launch_bounds__(64) void rerank_group_kernel(GroupTab g) {
+  const int p = group_problem(g, blockIdx.x);
+  const RerankArgs a = static_cast<const RerankArgs*>(g.tab)[p];
+  rerank_body<V, ARITH>(a, int64_t(blockIdx.x) - g.start[p]);
+}
+
+hipError_t launch_rerank_group(const GroupTab& g, int V, int arith, hipStream_t st) {
+  if (g.np <= 0 || g.nblocks <= 0) return hipSuccess;
+  const bool fb = arith == ARITH_FALLBACK;
+  const dim3 grid((unsigned)g.nblocks);
+  if (V == 4 && !fb)
+    hipLaunchKernelGGL((rerank_group_kernel<4, ARITH_AVX>), grid, dim3(64), 0, st, g);
+  else if (V == 8 && !fb)
+    hipLaunchKernelGGL((rerank_group_kernel<8, ARITH_AVX>), grid, dim3(64), 0, st, g);
+  else if (V == 4)
+    hipLaunchKernelGGL((rerank_group_kernel<4, ARITH_FALLBACK>), grid, dim3(64), 0, st, g);
+  else if (V == 8)
+    hipLaunchKernelGGL((rerank_group_kernel<8, ARITH_FALLBACK>), grid, dim3(64), 0, st, g);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
 }
 
 hipError_t launch_rerank(const RerankArgs& a, int V, hipStream_t st) {

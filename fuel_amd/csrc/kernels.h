@@ -479,6 +479,41 @@ struct ScanBatchArgs {
 
 hipError_t launch_prep_db(const PrepDbArgs& a, hipStream_t st);
 hipError_t launch_prep_queries(const PrepQArgs& a, hipStream_t st);
+
+// Grouped launches: one grid over the np problems of a device argument table
+// (tab[p] of the kernel's own argument type); workgroup b belongs to problem p
+// with start[p] <= b < start[p + 1] (start[0] = 0, start[np] = nblocks), so
+// small problems share one launch instead of one each.
+struct GroupTab {
+  const void* tab;
+  const int64_t* start;
+  int np;
+  int64_t nblocks;
+};
+__device__ __forceinline__ int group_problem(const GroupTab& g, int64_t b) {
+  int lo = 0, hi = g.np;  // start[lo] <= b < start[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (g.start[mid] <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+// count u32s at p set to v, one table entry per workgroup
+struct FillEntry {
+  uint32_t* p;
+  int64_t count;
+  uint32_t v;
+  uint32_t pad;
+};
+hipError_t launch_fill_multi(const FillEntry* tab, int n, hipStream_t st);
+hipError_t launch_prep_db_group(const GroupTab& g, hipStream_t st);       // PrepDbArgs, (n_pad + 63) / 64 each
+hipError_t launch_prep_queries_group(const GroupTab& g, hipStream_t st);  // PrepQArgs, (m_pad + 3) / 4 each
+hipError_t launch_rerank_group(const GroupTab& g, int V, int arith, hipStream_t st);  // RerankArgs, nq each
+// SeedArgs, m each; vmax = 64-entry registers for the largest parts * 2 * SEED_GROUPS
+hipError_t launch_seed_select_group(const GroupTab& g, int vmax, hipStream_t st);
 // kind 0: bf16 MFMA, 1: f32 MFMA, 2: E4M3 MFMA (residual operand only, with
 // its row scales in a.xsc).  Returns hipErrorInvalidValue for an unsupported K
 // step count.
@@ -525,7 +560,9 @@ struct SeedArgs {
   int KP;
   uint32_t* gthr;  // [nq_pad] shared thresholds, lowered by the select pass
 };
-hipError_t launch_seed_mfma(const SeedArgs& s, int kind, int ksteps, hipStream_t st);
+// The sample pass, then (select) the per-query selection; without it the
+// caller runs launch_seed_select_group over the problems' SeedArgs.
+hipError_t launch_seed_mfma(const SeedArgs& s, int kind, int ksteps, hipStream_t st, bool select = true);
 
 // Bytes addressable from each pointer of a ScanMfmaArgs.
 struct ScanExtents {

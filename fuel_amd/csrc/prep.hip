@@ -68,10 +68,11 @@ __device__ __forceinline__ RowStats prep_row(const float* __restrict__ src, int 
   return r;
 }
 
-// One workgroup per 64 rows (16 per wave); the index-wide maxima and flags are
-// reduced in the workgroup and published with one atomic each (per-row
-// atomics on the same few addresses serialise).
-__global__ __launch_bounds__(256) void prep_db_kernel(PrepDbArgs a) {
+// One workgroup per 64 rows (16 per wave; `blk` is the workgroup's block of
+// the problem); the index-wide maxima and flags are reduced in the workgroup
+// and published with one atomic each (per-row atomics on the same few
+// addresses serialise).
+__device__ __forceinline__ void prep_db_body(const PrepDbArgs& a, const int64_t blk) {
   __shared__ double red[4][5];
   __shared__ int redf[4][2];
   const int wave = threadIdx.x >> 6;
@@ -81,7 +82,7 @@ __global__ __launch_bounds__(256) void prep_db_kernel(PrepDbArgs a) {
   double m[5] = {0, 0, 0, 0, 0};
   bool allint = true, allnonneg = true;
   for (int i = 0; i < 16; ++i) {
-    const int64_t row = int64_t(blockIdx.x) * 64 + wave * 16 + i;
+    const int64_t row = blk * 64 + wave * 16 + i;
     if (row >= a.n_pad) break;
     char* dst = reinterpret_cast<char*>(a.Xop) + size_t(row) * a.Kp * es;
     if (row >= a.n) {
@@ -124,8 +125,8 @@ __global__ __launch_bounds__(256) void prep_db_kernel(PrepDbArgs a) {
   }
 }
 
-__global__ __launch_bounds__(256) void prep_query_kernel(PrepQArgs a) {
-  const int64_t row = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+__device__ __forceinline__ void prep_query_body(const PrepQArgs& a, const int64_t blk) {
+  const int64_t row = blk * 4 + (threadIdx.x >> 6);
   if (row >= a.m_pad) return;
   const int lane = __lane_id();
   const size_t es = a.bf16 ? 2 : 4;
@@ -154,6 +155,45 @@ __global__ __launch_bounds__(256) void prep_query_kernel(PrepQArgs a) {
     a.qt2[row] = r.t2;
     a.qint[row] = r.isint ? 1 : 0;
   }
+}
+
+__global__ __launch_bounds__(256) void prep_db_kernel(PrepDbArgs a) { prep_db_body(a, blockIdx.x); }
+__global__ __launch_bounds__(256) void prep_query_kernel(PrepQArgs a) { prep_query_body(a, blockIdx.x); }
+
+// Grouped forms: one grid over the problems of a table (group_problem).
+__global__ __launch_bounds__(256) void prep_db_group_kernel(GroupTab g) {
+  const int p = group_problem(g, blockIdx.x);
+  const PrepDbArgs a = static_cast<const PrepDbArgs*>(g.tab)[p];
+  prep_db_body(a, int64_t(blockIdx.x) - g.start[p]);
+}
+__global__ __launch_bounds__(256) void prep_query_group_kernel(GroupTab g) {
+  const int p = group_problem(g, blockIdx.x);
+  const PrepQArgs a = static_cast<const PrepQArgs*>(g.tab)[p];
+  prep_query_body(a, int64_t(blockIdx.x) - g.start[p]);
+}
+
+// One workgroup per table entry.
+__global__ __launch_bounds__(256) void fill_multi_kernel(const FillEntry* tab) {
+  const FillEntry f = tab[blockIdx.x];
+  for (int64_t i = threadIdx.x; i < f.count; i += 256) f.p[i] = f.v;
+}
+
+hipError_t launch_fill_multi(const FillEntry* tab, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_multi_kernel, dim3((unsigned)n), dim3(256), 0, st, tab);
+  return hipGetLastError();
+}
+
+hipError_t launch_prep_db_group(const GroupTab& g, hipStream_t st) {
+  if (g.np <= 0 || g.nblocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(prep_db_group_kernel, dim3((unsigned)g.nblocks), dim3(256), 0, st, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_prep_queries_group(const GroupTab& g, hipStream_t st) {
+  if (g.np <= 0 || g.nblocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(prep_query_group_kernel, dim3((unsigned)g.nblocks), dim3(256), 0, st, g);
+  return hipGetLastError();
 }
 
 hipError_t launch_prep_db(const PrepDbArgs& a, hipStream_t st) {

@@ -860,13 +860,13 @@ __global__ __launch_bounds__(256, 2) void seed_mfma_kernel(SeedArgs a) {
 // group minima (a 32-step ballot search on the order-preserving bits), one ulp
 // up so that the row attaining it still passes the scan's strict test, then
 // the shared threshold drops to it.
-__global__ __launch_bounds__(64) void seed_select_kernel(SeedArgs a) {
-  const int64_t q = blockIdx.x;
+// VMAX registers of 64 hold up to 64 VMAX minima.
+template <int VMAX>
+__device__ __forceinline__ void seed_select_body(const SeedArgs& a, const int64_t q) {
   if (q >= a.m) return;
   const int lane = __lane_id();
   const int total = a.parts * 2 * SEED_GROUPS;
   const float* v = a.mins + size_t(q) * total;
-  constexpr int VMAX = 16;  // up to 1024 minima
   uint32_t x[VMAX];
 #pragma unroll
   for (int r = 0; r < VMAX; ++r) {
@@ -887,6 +887,30 @@ __global__ __launch_bounds__(64) void seed_select_kernel(SeedArgs a) {
   if (lane == 0) atomicMin(&a.gthr[q], ord_f32(nextafterf(t, INFINITY)));
 }
 
+__global__ __launch_bounds__(64) void seed_select_kernel(SeedArgs a) { seed_select_body<16>(a, blockIdx.x); }
+
+// Grouped form: one wave per real query of every problem of the table.
+template <int VMAX>
+__global__ __launch_bounds__(64) void seed_select_group_kernel(GroupTab g) {
+  const int p = group_problem(g, blockIdx.x);
+  const SeedArgs a = static_cast<const SeedArgs*>(g.tab)[p];
+  seed_select_body<VMAX>(a, int64_t(blockIdx.x) - g.start[p]);
+}
+
+hipError_t launch_seed_select_group(const GroupTab& g, int vmax, hipStream_t st) {
+  if (g.np <= 0 || g.nblocks <= 0) return hipSuccess;
+  const dim3 grid((unsigned)g.nblocks);
+  if (vmax <= 4)
+    hipLaunchKernelGGL(seed_select_group_kernel<4>, grid, dim3(64), 0, st, g);
+  else if (vmax <= 8)
+    hipLaunchKernelGGL(seed_select_group_kernel<8>, grid, dim3(64), 0, st, g);
+  else if (vmax <= 16)
+    hipLaunchKernelGGL(seed_select_group_kernel<16>, grid, dim3(64), 0, st, g);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
 template <int KIND, int KS>
 static hipError_t launch_seed_ks(const SeedArgs& s, hipStream_t st) {
   constexpr int A = scan_groups(KS);
@@ -895,7 +919,7 @@ static hipError_t launch_seed_ks(const SeedArgs& s, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_seed_mfma(const SeedArgs& s, int kind, int ks, hipStream_t st) {
+hipError_t launch_seed_mfma(const SeedArgs& s, int kind, int ks, hipStream_t st, bool select) {
   if (s.ntiles <= 0 || s.m <= 0) return hipSuccess;
   if (s.parts * 2 * SEED_GROUPS > 1024 || s.parts * 2 * SEED_GROUPS < s.KP) return hipErrorInvalidValue;
   hipError_t e = hipErrorInvalidValue;
@@ -925,7 +949,7 @@ hipError_t launch_seed_mfma(const SeedArgs& s, int kind, int ks, hipStream_t st)
         return hipErrorInvalidValue;
     }
   }
-  if (e != hipSuccess) return e;
+  if (e != hipSuccess || !select) return e;
   hipLaunchKernelGGL(seed_select_kernel, dim3((unsigned)s.m), dim3(64), 0, st, s);
   return hipGetLastError();
 }
