@@ -91,7 +91,8 @@ struct Bucket {
   DevBuf group_dev;
   const FillEntry* fills = nullptr;
   int nfill = 0, sel_vmax = 4;
-  GroupTab g_db{}, g_q{}, g_sel{}, g_rr{};
+  GroupTab g_db{}, g_q{}, g_seed{}, g_sel{}, g_rr{};
+  GroupTab g_resid{}, g_qlist{}, g_theta{}, g_sample{};  // residual problems
 };
 
 }  // namespace
@@ -148,6 +149,9 @@ T* at(const DevBuf& b, size_t off) {
 }
 
 PrepDbArgs prep_db_args(fuel_amd_batch* b, int i);
+PrepResidArgs prep_resid_args(fuel_amd_batch* b, int i);
+QListArgs qlist_args(fuel_amd_batch* b, int i);
+SampleSeedArgs sample_args(fuel_amd_batch* b, int i);
 PrepQArgs prep_q_args(fuel_amd_batch* b, int i);
 SeedArgs seed_args(fuel_amd_batch* b, int i);
 RerankArgs rerank_args(fuel_amd_batch* b, int i, int V);
@@ -394,7 +398,11 @@ int32_t build_group_tables(fuel_amd_batch* b, Bucket& B) {
   std::vector<PrepQArgs> qs;
   std::vector<SeedArgs> sels;
   std::vector<RerankArgs> rrs;
-  std::vector<int64_t> db_start{0}, q_start{0}, sel_start{0}, rr_start{0};
+  std::vector<PrepResidArgs> rps;
+  std::vector<QListArgs> qls;
+  std::vector<SampleSeedArgs> sms;
+  std::vector<int64_t> db_start{0}, q_start{0}, seed_start{0}, sel_start{0}, rr_start{0};
+  std::vector<int64_t> rp_start{0}, ql_start{0}, th_start{0}, sm_start{0};
   int vmax = 1;
   for (int i : B.members) {
     const auto& pp = b->plan[i];
@@ -408,15 +416,34 @@ int32_t build_group_tables(fuel_amd_batch* b, Bucket& B) {
     } else {
       fills.push_back({reinterpret_cast<uint32_t*>(pa.consts), 32, 0u, 0u});  // constants, zero flags
       fills.push_back({at<uint32_t>(b->pool, pp.o_lconsts), int64_t(16) * b->parts[i]->L, 0u, 0u});
+      const PrepResidArgs ra = prep_resid_args(b, i);
+      if (ra.n_rpad % 64 != 0) return fail(FUEL_AMD_ERR_DEVICE, "internal error: residual rows not in whole blocks");
+      rps.push_back(ra);
+      rp_start.push_back(rp_start.back() + ra.n_rpad / 64);
+      const QListArgs la = qlist_args(b, i);
+      qls.push_back(la);
+      ql_start.push_back(ql_start.back() + (int64_t(la.L) * la.nq_pad + 255) / 256);
+      th_start.push_back(th_start.back() + (la.nq_pad + 3) / 4);
+      const SampleSeedArgs sa = sample_args(b, i);
+      if (sa.KP < 1 || sa.KP > SEED_SAMPLE_ROWS || (sa.ldx & 3) || (sa.ld32 & 3))
+        return fail(FUEL_AMD_ERR_DEVICE, "internal error: sampled seed outside its limits");
+      sms.push_back(sa);
+      sm_start.push_back(sm_start.back() + pr.n);
     }
     fills.push_back({reinterpret_cast<uint32_t*>(b->fail_counts.as<int>() + i), 1, 0u, 0u});
     const PrepQArgs qa = prep_q_args(b, i);
     qs.push_back(qa);
     q_start.push_back(q_start.back() + (qa.m_pad + 3) / 4);
     if (pp.seed.ntiles > 0) {
-      sels.push_back(seed_args(b, i));
+      const SeedArgs sd = seed_args(b, i);
+      const int total = sd.parts * 2 * SEED_GROUPS;
+      if (total > 1024 || total < sd.KP || sd.stride * (sd.ntiles - 1) + mfma_rows_per_tile(pp.kind, pp.ks, false) >
+                                               pp.n_pad)
+        return fail(FUEL_AMD_ERR_DEVICE, "internal error: seed plan outside its buffers");
+      sels.push_back(sd);
+      seed_start.push_back(seed_start.back() + int64_t(sd.nqb) * sd.parts);
       sel_start.push_back(sel_start.back() + pr.n);
-      vmax = std::max(vmax, (pp.seed.parts * 2 * SEED_GROUPS + 63) / 64);
+      vmax = std::max(vmax, (total + 63) / 64);
     }
     rrs.push_back(rerank_args(b, i, B.V));
     rr_start.push_back(rr_start.back() + pr.n);
@@ -436,7 +463,15 @@ int32_t build_group_tables(fuel_amd_batch* b, Bucket& B) {
   const size_t o_q = put(qs.data(), qs.size() * sizeof(PrepQArgs));
   const size_t o_qs = put(q_start.data(), q_start.size() * 8);
   const size_t o_sel = put(sels.data(), sels.size() * sizeof(SeedArgs));
+  const size_t o_seeds = put(seed_start.data(), seed_start.size() * 8);
   const size_t o_sels = put(sel_start.data(), sel_start.size() * 8);
+  const size_t o_rp = put(rps.data(), rps.size() * sizeof(PrepResidArgs));
+  const size_t o_rps = put(rp_start.data(), rp_start.size() * 8);
+  const size_t o_ql = put(qls.data(), qls.size() * sizeof(QListArgs));
+  const size_t o_qls = put(ql_start.data(), ql_start.size() * 8);
+  const size_t o_ths = put(th_start.data(), th_start.size() * 8);
+  const size_t o_sm = put(sms.data(), sms.size() * sizeof(SampleSeedArgs));
+  const size_t o_sms = put(sm_start.data(), sm_start.size() * 8);
   const size_t o_rr = put(rrs.data(), rrs.size() * sizeof(RerankArgs));
   const size_t o_rrs = put(rr_start.data(), rr_start.size() * 8);
   hipError_t e = B.group_dev.ensure(std::max<size_t>(host.size(), 256));
@@ -456,7 +491,12 @@ int32_t build_group_tables(fuel_amd_batch* b, Bucket& B) {
   B.nfill = (int)fills.size();
   B.g_db = tab(o_db, o_dbs, dbs.size(), db_start);
   B.g_q = tab(o_q, o_qs, qs.size(), q_start);
+  B.g_seed = tab(o_sel, o_seeds, sels.size(), seed_start);
   B.g_sel = tab(o_sel, o_sels, sels.size(), sel_start);
+  B.g_resid = tab(o_rp, o_rps, rps.size(), rp_start);
+  B.g_qlist = tab(o_ql, o_qls, qls.size(), ql_start);
+  B.g_theta = tab(o_ql, o_ths, qls.size(), th_start);
+  B.g_sample = tab(o_sm, o_sms, sms.size(), sm_start);
   B.g_rr = tab(o_rr, o_rrs, rrs.size(), rr_start);
   B.sel_vmax = vmax;
   return FUEL_AMD_OK;
@@ -481,6 +521,27 @@ PrepDbArgs prep_db_args(fuel_amd_batch* b, int i) {
 }
 
 // Operand copy of problem i (and its bound constants, reset first).
+PrepResidArgs prep_resid_args(fuel_amd_batch* b, int i) {
+  const auto& pp = b->plan[i];
+  const auto& pr = b->probs[i];
+  const auto& part = *b->parts[i];
+  PrepResidArgs r{};
+  r.X = static_cast<const float*>(pr.data);
+  r.ldx = pr.ld;
+  r.d = pr.d;
+  r.Kp = pp.Kp;
+  r.n_rpad = part.n_rpad;
+  r.rperm = part.rperm.as<uint32_t>();
+  r.tile_list = part.tile_list.as<int>();
+  r.cents = part.cents.as<float>();
+  r.ldc = pr.d;
+  r.Xop = at<uint16_t>(b->pool, pp.o_xop);
+  r.hx = at<float>(b->pool, pp.o_hx);
+  r.lconsts = at<unsigned long long>(b->pool, pp.o_lconsts);
+  r.gconsts = at<unsigned long long>(b->pool, pp.o_consts);
+  return r;
+}
+
 // (reset: the bound constants are reset here first; the grouped pass resets
 // them in its fill launch)
 hipError_t enqueue_operand(fuel_amd_batch* b, int i, hipStream_t st, bool reset) {
@@ -498,25 +559,11 @@ hipError_t enqueue_operand(fuel_amd_batch* b, int i, hipStream_t st, bool reset)
   hipError_t e = hipSuccess;
   if (reset) e = launch_fill_u32(reinterpret_cast<uint32_t*>(pa.consts), 32, 0u, st);  // constants, zero flags
   if (reset && e == hipSuccess) e = launch_fill_u32(at<uint32_t>(b->pool, pp.o_lconsts), 16 * part.L, 0u, st);
-  PrepResidArgs r{};
-  r.X = static_cast<const float*>(pr.data);
-  r.ldx = pr.ld;
-  r.d = pr.d;
-  r.Kp = pp.Kp;
-  r.n_rpad = part.n_rpad;
-  r.rperm = part.rperm.as<uint32_t>();
-  r.tile_list = part.tile_list.as<int>();
-  r.cents = part.cents.as<float>();
-  r.ldc = pr.d;
-  r.Xop = at<uint16_t>(b->pool, pp.o_xop);
-  r.hx = at<float>(b->pool, pp.o_hx);
-  r.lconsts = at<unsigned long long>(b->pool, pp.o_lconsts);
-  r.gconsts = pa.consts;
+  const PrepResidArgs r = prep_resid_args(b, i);
   if (e == hipSuccess) e = launch_prep_resid(r, st);
   return e;
 }
 
-// Level 0 of one group on stream st (no host synchronisation: capturable).
 PrepQArgs prep_q_args(fuel_amd_batch* b, int i) {
   const auto& pp = b->plan[i];
   const auto& pr = b->probs[i];
@@ -600,15 +647,15 @@ RerankArgs rerank_args(fuel_amd_batch* b, int i, int V) {
   return ra;
 }
 
-// Residual problem i after its query preparation: query-to-centroid norms, the
-// per-(query, list) terms and proven start, and the sampled threshold.
-hipError_t enqueue_resid_queries(fuel_amd_batch* b, int i, hipStream_t st) {
+// Residual problem i: query-to-centroid norms (per problem, after the query
+// preparation), then through the grouped launches the per-(query, list) terms
+// and proven start, and the sampled threshold.
+KmeansPairwiseArgs<float> centroid_args(fuel_amd_batch* b, int i) {
   const auto& pp = b->plan[i];
   const auto& pr = b->probs[i];
   const auto& part = *b->parts[i];
-  const PrepQArgs qa = prep_q_args(b, i);
   KmeansPairwiseArgs<float> pw{};
-  pw.P1 = qa.Q32;
+  pw.P1 = at<float>(b->pool, pp.o_q32);
   pw.ld1 = pr.ld;
   pw.m = pr.n;
   pw.P2 = part.cents.as<float>();
@@ -617,9 +664,15 @@ hipError_t enqueue_resid_queries(fuel_amd_batch* b, int i, hipStream_t st) {
   pw.d = pr.d;
   pw.out = at<float>(b->pool, pp.o_qs2);
   pw.ldo = part.L;
-  hipError_t e = launch_kmeans_pairwise<float>(pw, st);
+  return pw;
+}
+
+QListArgs qlist_args(fuel_amd_batch* b, int i) {
+  const auto& pp = b->plan[i];
+  const auto& pr = b->probs[i];
+  const auto& part = *b->parts[i];
   QListArgs la{};
-  la.s2 = pw.out;
+  la.s2 = at<float>(b->pool, pp.o_qs2);
   la.m = pr.n;
   la.nq_pad = pp.nq_pad;
   la.d = pr.d;
@@ -628,25 +681,30 @@ hipError_t enqueue_resid_queries(fuel_amd_batch* b, int i, hipStream_t st) {
   la.qtab = at<float>(b->pool, pp.o_qtab);
   la.lcount = part.lcount.as<uint32_t>();
   la.KP = pp.KP;
-  la.gthr = qa.gthr;
-  if (e == hipSuccess) e = launch_prep_qlist(la, st);
-  // the sampled threshold: exact distances to the nearest list's first rows
+  la.gthr = at<uint32_t>(b->pool, pp.o_gthr);
+  return la;
+}
+
+// the sampled threshold: exact distances to the nearest list's first rows
+SampleSeedArgs sample_args(fuel_amd_batch* b, int i) {
+  const auto& pp = b->plan[i];
+  const auto& pr = b->probs[i];
+  const auto& part = *b->parts[i];
   SampleSeedArgs ss{};
-  ss.s2 = pw.out;
+  ss.s2 = at<float>(b->pool, pp.o_qs2);
   ss.m = pr.n;
   ss.L = part.L;
   ss.KP = pp.KP;
-  ss.lcount = la.lcount;
+  ss.lcount = part.lcount.as<uint32_t>();
   ss.loff = part.loff.as<int64_t>();
   ss.rperm = part.rperm.as<uint32_t>();
   ss.X = static_cast<const float*>(pr.data);
   ss.ldx = pr.ld;
-  ss.Q32 = qa.Q32;
+  ss.Q32 = at<float>(b->pool, pp.o_q32);
   ss.ld32 = pr.ld;
   ss.d = pr.d;
-  ss.gthr = la.gthr;
-  if (e == hipSuccess) e = launch_seed_sample(ss, st);
-  return e;
+  ss.gthr = at<uint32_t>(b->pool, pp.o_gthr);
+  return ss;
 }
 
 // Level 0 of one group on stream st (no host synchronisation: capturable).
@@ -656,18 +714,17 @@ hipError_t enqueue_resid_queries(fuel_amd_batch* b, int i, hipStream_t st) {
 // launch each; residual operands per problem), seed passes (per problem) and
 // their selection (one launch), the persistent scan, the reranks (one launch).
 hipError_t enqueue_bucket(fuel_amd_batch* b, const Bucket& B, hipStream_t st) {
+  const int mk = B.cfg.kind == 2 ? 0 : B.cfg.kind;
   hipError_t e = launch_fill_multi(B.fills, B.nfill, st);
   if (e == hipSuccess) e = launch_prep_db_group(B.g_db, st);
-  for (int i : B.members)
-    if (e == hipSuccess && b->plan[i].kind == 2) e = enqueue_operand(b, i, st, false);
+  if (e == hipSuccess) e = launch_prep_resid_group(B.g_resid, st);
   if (e == hipSuccess) e = launch_prep_queries_group(B.g_q, st);
-  for (int i : B.members) {
-    if (e != hipSuccess) return e;
-    const auto& pp = b->plan[i];
-    if (pp.seed.ntiles > 0) e = launch_seed_mfma(seed_args(b, i), pp.kind, pp.ks, st, false);
-    if (pp.kind == 2 && e == hipSuccess) e = enqueue_resid_queries(b, i, st);
-  }
+  if (e == hipSuccess) e = launch_seed_mfma_group(B.g_seed, mk, B.ks, st);
   if (e == hipSuccess) e = launch_seed_select_group(B.g_sel, B.sel_vmax, st);
+  for (int i : B.members)
+    if (e == hipSuccess && b->plan[i].kind == 2) e = launch_kmeans_pairwise<float>(centroid_args(b, i), st);
+  if (e == hipSuccess) e = launch_prep_qlist_group(B.g_qlist, B.g_theta, st);
+  if (e == hipSuccess) e = launch_seed_sample_group(B.g_sample, st);
   if (e != hipSuccess) return e;
   ScanBatchArgs sb{};
   sb.probs = B.probs_dev.as<ScanMfmaArgs>();

@@ -15,6 +15,28 @@ namespace fa {
 //   2 the Python linear-scan path (k, drop self: python/search.rs:485-488).
 enum ExcludeMode { EXCL_NONE = 0, EXCL_FOR_EACH_KNN = 1, EXCL_PY_LINEAR = 2 };
 
+// Grouped launches: one grid over the np problems of a device argument table
+// (tab[p] of the kernel's own argument type); workgroup b belongs to problem p
+// with start[p] <= b < start[p + 1] (start[0] = 0, start[np] = nblocks), so
+// small problems share one launch instead of one each.
+struct GroupTab {
+  const void* tab;
+  const int64_t* start;
+  int np;
+  int64_t nblocks;
+};
+__device__ __forceinline__ int group_problem(const GroupTab& g, int64_t b) {
+  int lo = 0, hi = g.np;  // start[lo] <= b < start[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (g.start[mid] <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
 template <typename F>
 struct ScanExactArgs {
   const F* X;
@@ -370,6 +392,13 @@ struct QListArgs {
   uint32_t* gthr;          // [nq_pad], initialised by the query preparation
 };
 hipError_t launch_prep_qlist(const QListArgs& a, hipStream_t st);
+// Grouped forms over tables of PrepResidArgs (n_rpad / 64 workgroups each, bf16
+// operand), QListArgs ((L nq_pad + 255) / 256 for the terms, then (nq_pad + 3)
+// / 4 for the proven start, two tables of the same arguments) and
+// SampleSeedArgs (m each).
+hipError_t launch_prep_resid_group(const GroupTab& g, hipStream_t st);
+hipError_t launch_prep_qlist_group(const GroupTab& terms, const GroupTab& theta, hipStream_t st);
+hipError_t launch_seed_sample_group(const GroupTab& g, hipStream_t st);
 // Sampled threshold of a residual scan, one wave per real query: the nearest
 // list with at least KP rows (smallest s2, ties to the smaller index), its
 // first min(rows, SEED_SAMPLE_ROWS) rows, each row's ||q - x||^2 / 2 in f64
@@ -480,27 +509,6 @@ struct ScanBatchArgs {
 hipError_t launch_prep_db(const PrepDbArgs& a, hipStream_t st);
 hipError_t launch_prep_queries(const PrepQArgs& a, hipStream_t st);
 
-// Grouped launches: one grid over the np problems of a device argument table
-// (tab[p] of the kernel's own argument type); workgroup b belongs to problem p
-// with start[p] <= b < start[p + 1] (start[0] = 0, start[np] = nblocks), so
-// small problems share one launch instead of one each.
-struct GroupTab {
-  const void* tab;
-  const int64_t* start;
-  int np;
-  int64_t nblocks;
-};
-__device__ __forceinline__ int group_problem(const GroupTab& g, int64_t b) {
-  int lo = 0, hi = g.np;  // start[lo] <= b < start[hi]
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (g.start[mid] <= b)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  return lo;
-}
 // count u32s at p set to v, one table entry per workgroup
 struct FillEntry {
   uint32_t* p;
@@ -563,6 +571,9 @@ struct SeedArgs {
 // The sample pass, then (select) the per-query selection; without it the
 // caller runs launch_seed_select_group over the problems' SeedArgs.
 hipError_t launch_seed_mfma(const SeedArgs& s, int kind, int ksteps, hipStream_t st, bool select = true);
+// The sample passes of a table of SeedArgs (nqb * parts workgroups each; every
+// entry already checked by its builder), one launch.
+hipError_t launch_seed_mfma_group(const GroupTab& g, int kind, int ksteps, hipStream_t st);
 
 // Bytes addressable from each pointer of a ScanMfmaArgs.
 struct ScanExtents {

@@ -162,11 +162,10 @@ __device__ __forceinline__ float wave_max(float v) {
 // fl32(x - c_j) 2^-e_p, e_p = fp8_scale_exp of the row's largest |coordinate|,
 // and xsc[p] the E8M0 code of 2^e_p; r~ is the dequantized row.
 template <bool FP8>
-__global__ __launch_bounds__(256) void prep_resid_kernel(PrepResidArgs a) {
+__device__ __forceinline__ void prep_resid_body(const PrepResidArgs& a, const int64_t blk) {
   __shared__ double red[4][6];
   const int wave = threadIdx.x >> 6;
   const int lane = __lane_id();
-  const int64_t blk = blockIdx.x;
   const int j = a.tile_list[blk];
   const float* cj = a.cents + size_t(j) * a.ldc;
   const double slack = 1.0 + (a.d + 6) * 0x1p-52;
@@ -245,6 +244,22 @@ __global__ __launch_bounds__(256) void prep_resid_kernel(PrepResidArgs a) {
   }
 }
 
+template <bool FP8>
+__global__ __launch_bounds__(256) void prep_resid_kernel(PrepResidArgs a) {
+  prep_resid_body<FP8>(a, blockIdx.x);
+}
+__global__ __launch_bounds__(256) void prep_resid_group_kernel(GroupTab g) {
+  const int p = group_problem(g, blockIdx.x);
+  const PrepResidArgs a = static_cast<const PrepResidArgs*>(g.tab)[p];
+  prep_resid_body<false>(a, int64_t(blockIdx.x) - g.start[p]);
+}
+
+hipError_t launch_prep_resid_group(const GroupTab& g, hipStream_t st) {
+  if (g.np <= 0 || g.nblocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(prep_resid_group_kernel, dim3((unsigned)g.nblocks), dim3(256), 0, st, g);
+  return hipGetLastError();
+}
+
 hipError_t launch_prep_resid(const PrepResidArgs& a, hipStream_t st) {
   if (a.n_rpad == 0) return hipSuccess;
   if (a.n_rpad % 64 != 0) return hipErrorInvalidValue;  // lists are whole 64-row blocks
@@ -278,8 +293,8 @@ hipError_t launch_prep_resid(const PrepResidArgs& a, hipStream_t st) {
 // (2^-23 covers the f32 subtraction; the last term a clamped scale and flushed
 // subnormal differences).  The MFMA sums 2d products against hi and lo, with
 // sum |a||b| <= RT (||hi|| + ||lo||) <= RT (N + EA)(1 + 2^-3).
-__global__ __launch_bounds__(256) void prep_qlist_kernel(QListArgs a) {
-  const int64_t w = int64_t(blockIdx.x) * 256 + threadIdx.x;
+__device__ __forceinline__ void prep_qlist_body(const QListArgs& a, const int64_t blk) {
+  const int64_t w = blk * 256 + threadIdx.x;
   if (w >= int64_t(a.L) * a.nq_pad) return;
   const int j = (int)(w / a.nq_pad);
   const int64_t q = w - int64_t(j) * a.nq_pad;
@@ -315,8 +330,8 @@ __global__ __launch_bounds__(256) void prep_qlist_kernel(QListArgs a) {
 }
 
 // One wave per query slot, lanes over the lists (see QListArgs).
-__global__ __launch_bounds__(256) void seed_theta_kernel(QListArgs a) {
-  const int64_t q = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+__device__ __forceinline__ void seed_theta_body(const QListArgs& a, const int64_t blk) {
+  const int64_t q = blk * 4 + (threadIdx.x >> 6);
   if (q >= a.nq_pad) return;
   const int lane = __lane_id();
   const float* s2 = a.s2 + (q < a.m ? q : a.m - 1) * a.L;
@@ -339,6 +354,19 @@ __global__ __launch_bounds__(256) void seed_theta_kernel(QListArgs a) {
     const uint32_t o = ord_f32(t);
     if (o < a.gthr[q]) a.gthr[q] = o;
   }
+}
+
+__global__ __launch_bounds__(256) void prep_qlist_kernel(QListArgs a) { prep_qlist_body(a, blockIdx.x); }
+__global__ __launch_bounds__(256) void seed_theta_kernel(QListArgs a) { seed_theta_body(a, blockIdx.x); }
+__global__ __launch_bounds__(256) void prep_qlist_group_kernel(GroupTab g) {
+  const int p = group_problem(g, blockIdx.x);
+  const QListArgs a = static_cast<const QListArgs*>(g.tab)[p];
+  prep_qlist_body(a, int64_t(blockIdx.x) - g.start[p]);
+}
+__global__ __launch_bounds__(256) void seed_theta_group_kernel(GroupTab g) {
+  const int p = group_problem(g, blockIdx.x);
+  const QListArgs a = static_cast<const QListArgs*>(g.tab)[p];
+  seed_theta_body(a, int64_t(blockIdx.x) - g.start[p]);
 }
 
 __global__ __launch_bounds__(256) void prep_qfrag_fp8_kernel(QFragArgs a) {
@@ -376,11 +404,10 @@ hipError_t launch_prep_qfrag_fp8(const QFragArgs& a, hipStream_t st) {
 // (the same answer), wave w bounds rows w * 128 + lane and + 64 of its sample
 // (float4 loads, f64 sums in four chains), the bounds meet in LDS and wave 0
 // selects the KP-th smallest.
-__global__ __launch_bounds__(256) void seed_sample_kernel(SampleSeedArgs a) {
+__device__ __forceinline__ void seed_sample_body(const SampleSeedArgs& a, const int64_t q) {
   constexpr int RPL = SEED_SAMPLE_ROWS / 256;  // sample rows per thread
   constexpr int RW = SEED_SAMPLE_ROWS / 64;    // sample rows per lane of wave 0's select
   __shared__ uint32_t keys[SEED_SAMPLE_ROWS];
-  const int64_t q = blockIdx.x;
   const int lane = __lane_id();
   const int wave = threadIdx.x >> 6;
   const float* s2 = a.s2 + q * a.L;
@@ -454,6 +481,27 @@ __global__ __launch_bounds__(256) void seed_sample_kernel(SampleSeedArgs a) {
     const uint32_t o = ord_f32(nextafterf(t, INFINITY));
     if (o < a.gthr[q]) a.gthr[q] = o;
   }
+}
+
+__global__ __launch_bounds__(256) void seed_sample_kernel(SampleSeedArgs a) { seed_sample_body(a, blockIdx.x); }
+__global__ __launch_bounds__(256) void seed_sample_group_kernel(GroupTab g) {
+  const int p = group_problem(g, blockIdx.x);
+  const SampleSeedArgs a = static_cast<const SampleSeedArgs*>(g.tab)[p];
+  seed_sample_body(a, int64_t(blockIdx.x) - g.start[p]);
+}
+
+hipError_t launch_seed_sample_group(const GroupTab& g, hipStream_t st) {
+  if (g.np <= 0 || g.nblocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(seed_sample_group_kernel, dim3((unsigned)g.nblocks), dim3(256), 0, st, g);
+  return hipGetLastError();
+}
+
+hipError_t launch_prep_qlist_group(const GroupTab& terms, const GroupTab& theta, hipStream_t st) {
+  if (terms.np > 0 && terms.nblocks > 0)
+    hipLaunchKernelGGL(prep_qlist_group_kernel, dim3((unsigned)terms.nblocks), dim3(256), 0, st, terms);
+  if (theta.np > 0 && theta.nblocks > 0)
+    hipLaunchKernelGGL(seed_theta_group_kernel, dim3((unsigned)theta.nblocks), dim3(256), 0, st, theta);
+  return hipGetLastError();
 }
 
 hipError_t launch_seed_sample(const SampleSeedArgs& a, hipStream_t st) {

@@ -206,11 +206,15 @@ __device__ __forceinline__ int merge_select(uint64_t* __restrict__ slot, int KP,
 // up to them.  The scan body and the launch planning both use these.
 constexpr int scan_wave_loads(int ks, int R, int W) { return (32 * R * (2 * ks + 1) + 64 * W - 1) / (64 * W); }
 constexpr int scan_tile_bytes(int ks, int R, int W) { return scan_wave_loads(ks, R, W) * 64 * W * 16; }
-// LDS of one workgroup: nb tile buffers, their C inputs (64 floats each) and
-// the published thresholds (64 words per wave and buffer), plus the persistent
-// form's work item.
+// C inputs of a tile: one float per row, at least 64 per buffer, copied 64 at a
+// time by the first (32 R + 63) / 64 waves.
+constexpr int scan_hx_words(int R) { return 32 * R < 64 ? 64 : 32 * R; }
+constexpr int scan_hx_waves(int R) { return (32 * R + 63) / 64; }
+// LDS of one workgroup: nb tile buffers, their C inputs and the published
+// thresholds (64 words per wave and buffer), plus the persistent form's work
+// item.
 constexpr int scan_lds_bytes(int ks, int R, int W, bool batch, int nb = 2) {
-  return nb * scan_tile_bytes(ks, R, W) + nb * 64 * 4 + nb * W * 64 * 4 + (batch ? 16 : 0);
+  return nb * scan_tile_bytes(ks, R, W) + nb * scan_hx_words(R) * 4 + nb * W * 64 * 4 + (batch ? 16 : 0);
 }
 // per buffer, 64 words of the E4M3 operand's row scales and (residual operand)
 // 64 copies of the tile's list
@@ -234,11 +238,19 @@ constexpr bool scan_two_waves(int kind, int ks, bool resid) {
 }
 // Shapes whose fragments fit 128 registers run four workgroups (four waves per
 // SIMD): at few K steps the selection, not the MFMA, bounds the scan, and more
-// waves hide its latency.  They take 32-row tiles (half the accumulators) so
-// that the 128 registers hold them without spills.
+// waves hide its latency.  They keep the accumulators of one 32-row block at a
+// time, so that the 128 registers hold them without spills, and their tiles
+// hold 4 such blocks up to 3 K steps, else 2 (whose LDS then still leaves four
+// workgroups resident; residual: 2, one list per 64-row block), so that the
+// barrier and the copy issue of a tile are shared by several blocks' MFMAs and
+// selection.
 constexpr bool scan_four_wg(int kind, int ks) { return kind == 0 ? ks <= 4 : (kind == 1 && ks <= 2); }
 constexpr int scan_row_blocks(int kind, int ks, bool resid) {
-  return scan_four_wg(kind, ks) ? 1 : ((kind == 0 && !resid ? ks <= 12 : ks <= 16) ? 2 : 1);
+  return scan_four_wg(kind, ks) ? (resid || ks > 3 ? 2 : 4) : ((kind == 0 && !resid ? ks <= 12 : ks <= 16) ? 2 : 1);
+}
+// Row blocks whose accumulators are live at once (R for the other shapes).
+constexpr int scan_row_chunk(int kind, int ks, bool resid) {
+  return scan_four_wg(kind, ks) ? 1 : scan_row_blocks(kind, ks, resid);
 }
 
 // Tile buffers of a shape: three (the copy of tile t + 2 leaves when tile t
@@ -310,12 +322,17 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
   constexpr int TILE_BYTES = scan_tile_bytes(KS, R, W);  // rounded up to whole wave loads
   static_assert(WAVE_LOADS * 64 * W >= BN * ROWC, "tile copy covers the tile");
   constexpr int NB = scan_buffers(KIND, KS, R, W, RESID);
+  constexpr int RC = scan_row_chunk(KIND, KS, RESID);
+  static_assert(R % RC == 0, "whole chunks of row blocks");
   // vector memory loads one stage() issues in every wave: the tile chunks and
-  // the thresholds (wave 0 adds the C inputs, E4M3 row scales and list id, so
-  // this is the smallest count of any wave)
+  // the thresholds (the first HXW waves add the C inputs, wave 0 the E4M3 row
+  // scales and list id, so this is the smallest count of any wave)
   constexpr int STAGE_OPS = WAVE_LOADS + 1;
+  constexpr int HXN = scan_hx_words(R), HXW = scan_hx_waves(R);
+  static_assert(HXW <= W, "every C input has a wave that copies it");
+  static_assert(!RESID || BN <= 64, "a residual tile lies in one 64-row block, hence in one list");
   __shared__ __attribute__((aligned(16))) char tile[NB][TILE_BYTES];
-  __shared__ __attribute__((aligned(16))) float tile_hx[NB][64];      // one dword per lane of wave 0
+  __shared__ __attribute__((aligned(16))) float tile_hx[NB][HXN];  // one dword per lane of the first HXW waves
   __shared__ __attribute__((aligned(16))) uint32_t thr_lds[NB][W][64];  // published thresholds, per wave
   __shared__ __attribute__((aligned(16))) uint32_t tile_sc[NB][KIND == 2 ? 64 : 1];  // E8M0 row scales (E4M3)
   // residual operand: the tile's list, copied with the tile, so that reading it
@@ -382,9 +399,11 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
 #pragma unroll
     for (int j = 0; j < WAVE_LOADS; ++j)
       lds_dma16(__builtin_amdgcn_readfirstlane(lds_tile + b * TILE_BYTES + (j * W + wave) * 1024), base + soff[j]);
+    if (wave < HXW) {
+      const int r = wave * 64 + lane < BN ? wave * 64 + lane : BN - 1;
+      lds_dma4(__builtin_amdgcn_readfirstlane(lds_hx + (b * HXN + wave * 64) * 4), a.hx + t + r);
+    }
     if (wave == 0) {
-      const int r = lane < BN ? lane : BN - 1;
-      lds_dma4(__builtin_amdgcn_readfirstlane(lds_hx + b * 64 * 4), a.hx + t + r);
       if constexpr (KIND == 2) {
         // the BN scale bytes of the tile, a word per lane (lanes past BN/4 repeat the last)
         const int w = lane < BN / 4 ? lane : BN / 4 - 1;
@@ -486,10 +505,16 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
         }
       }
     }
+    // Row blocks c0 .. c0 + RC - 1 of the tile at a time (RC = R but for the
+    // shapes of few K steps, whose tiles hold more row blocks than they keep
+    // accumulators for).
+#pragma unroll 1
+    for (int c0 = 0; c0 < R; c0 += RC) {
     // C input: hxn of the 16 rows this lane holds, shared by the query groups
-    floatx16 acc[A][R];
+    floatx16 acc[A][RC];
 #pragma unroll
-    for (int rb = 0; rb < R; ++rb) {
+    for (int rc = 0; rc < RC; ++rc) {
+      const int rb = c0 + rc;
       floatx16 h;
 #pragma unroll
       for (int q4 = 0; q4 < 4; ++q4) {
@@ -503,34 +528,34 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
 #pragma unroll
         for (int g = 0; g < A; ++g)
 #pragma unroll
-          for (int r = 0; r < 16; ++r) acc[g][rb][r] = h[r] + qt[g];
+          for (int r = 0; r < 16; ++r) acc[g][rc][r] = h[r] + qt[g];
       } else {
 #pragma unroll
-        for (int g = 0; g < A; ++g) acc[g][rb] = h;
+        for (int g = 0; g < A; ++g) acc[g][rc] = h;
       }
     }
     if constexpr (KIND == 2) {
       const uint8_t* scb = reinterpret_cast<const uint8_t*>(&tile_sc[buf][0]);
-      int sa[R];
+      int sa[RC];
 #pragma unroll
-      for (int rb = 0; rb < R; ++rb) sa[rb] = scb[rb * 32 + col];
+      for (int rc = 0; rc < RC; ++rc) sa[rc] = scb[(c0 + rc) * 32 + col];
 #pragma unroll
       for (int p = 0; p < KS / 2; ++p) {
-        intx8 a8[R];
+        intx8 a8[RC];
 #pragma unroll
-        for (int rb = 0; rb < R; ++rb) {
-          const char* rp = tb + (rb * 32 + col) * ROWB + p * 64 + half * 16;
+        for (int rc = 0; rc < RC; ++rc) {
+          const char* rp = tb + ((c0 + rc) * 32 + col) * ROWB + p * 64 + half * 16;
           const uint4 x0 = *reinterpret_cast<const uint4*>(rp);
           const uint4 x1 = *reinterpret_cast<const uint4*>(rp + 32);
-          a8[rb] = intx8{(int)x0.x, (int)x0.y, (int)x0.z, (int)x0.w, (int)x1.x, (int)x1.y, (int)x1.z, (int)x1.w};
+          a8[rc] = intx8{(int)x0.x, (int)x0.y, (int)x0.z, (int)x0.w, (int)x1.x, (int)x1.y, (int)x1.z, (int)x1.w};
         }
 #pragma unroll
         for (int g = 0; g < A; ++g) {
           const uint4 h0 = qf[g][2 * p], h1 = qf[g][2 * p + 1];
           const intx8 bh{(int)h0.x, (int)h0.y, (int)h0.z, (int)h0.w, (int)h1.x, (int)h1.y, (int)h1.z, (int)h1.w};
 #pragma unroll
-          for (int rb = 0; rb < R; ++rb)
-            acc[g][rb] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a8[rb], bh, acc[g][rb], 0, 0, 0, sa[rb], 0,
+          for (int rc = 0; rc < RC; ++rc)
+            acc[g][rc] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a8[rc], bh, acc[g][rc], 0, 0, 0, sa[rc], 0,
                                                                          qsh[g]);
         }
 #pragma unroll
@@ -538,22 +563,22 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
           const uint4 l0 = qlo[g][2 * p], l1 = qlo[g][2 * p + 1];
           const intx8 bl{(int)l0.x, (int)l0.y, (int)l0.z, (int)l0.w, (int)l1.x, (int)l1.y, (int)l1.z, (int)l1.w};
 #pragma unroll
-          for (int rb = 0; rb < R; ++rb)
-            acc[g][rb] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a8[rb], bl, acc[g][rb], 0, 0, 0, sa[rb], 0,
+          for (int rc = 0; rc < RC; ++rc)
+            acc[g][rc] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a8[rc], bl, acc[g][rc], 0, 0, 0, sa[rc], 0,
                                                                          qsl[g]);
         }
       }
     } else {
 #pragma unroll
       for (int s = 0; s < KS; ++s) {
-        uint4 af[R];
+        uint4 af[RC];
 #pragma unroll
-        for (int rb = 0; rb < R; ++rb)
-          af[rb] = *reinterpret_cast<const uint4*>(tb + (rb * 32 + col) * ROWB + s * 32 + half * 16);
+        for (int rc = 0; rc < RC; ++rc)
+          af[rc] = *reinterpret_cast<const uint4*>(tb + ((c0 + rc) * 32 + col) * ROWB + s * 32 + half * 16);
 #pragma unroll
         for (int g = 0; g < A; ++g)
 #pragma unroll
-          for (int rb = 0; rb < R; ++rb) Mfma<KIND>::step(acc[g][rb], af[rb], qf[g][s]);
+          for (int rc = 0; rc < RC; ++rc) Mfma<KIND>::step(acc[g][rc], af[rc], qf[g][s]);
       }
     }
 
@@ -562,17 +587,18 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
       const float nt = -thr[g];
       bool appended = false;
 #pragma unroll
-      for (int rb = 0; rb < R; ++rb) {
-        float mx = acc[g][rb][0];
+      for (int rc = 0; rc < RC; ++rc) {
+        const int rb = c0 + rc;
+        float mx = acc[g][rc][0];
 #pragma unroll
-        for (int r = 1; r < 16; ++r) mx = fmaxf(mx, acc[g][rb][r]);
+        for (int r = 1; r < 16; ++r) mx = fmaxf(mx, acc[g][rc][r]);
         if (__ballot(mx > nt)) {
           appended = true;
           uint64_t* sl = a.slots + scan_slot_index(split, a.nq_pad, q0 + g * 32 + col, NS, KP + half * csub);
           float w[16];
 #pragma unroll
           for (int r = 0; r < 16; ++r)
-            w[r] = __uint_as_float((__float_as_uint(acc[g][rb][r]) & ~15u) | uint32_t(r));
+            w[r] = __uint_as_float((__float_as_uint(acc[g][rc][r]) & ~15u) | uint32_t(r));
           float mt = w[0];
 #pragma unroll
           for (int r = 1; r < 16; ++r) mt = fmaxf(mt, w[r]);
@@ -592,8 +618,9 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
           }
         }
       }
-      // a merge leaves room for a whole tile of appends (16 R per lane)
-      if (appended) MERGE_WHERE(g, cnt[g] > csub - 16 * R);
+      // a merge leaves room for the next chunk's appends (16 RC per lane)
+      if (appended) MERGE_WHERE(g, cnt[g] > csub - 16 * RC);
+    }
     }
     // The next tile (and the thresholds that came with it) have landed for
     // every wave; with two buffers this tile's one is free again and takes the
@@ -749,8 +776,10 @@ __device__ __forceinline__ void seed_mfma_body(const SeedArgs& a, const int qb, 
   constexpr int ROWB = (2 * KS + 1) * 16;
   constexpr int WAVE_LOADS = scan_wave_loads(KS, R, W);
   constexpr int TILE_BYTES = scan_tile_bytes(KS, R, W);
+  constexpr int HXN = scan_hx_words(R), HXW = scan_hx_waves(R);
+  static_assert(HXW <= W, "every C input has a wave that copies it");
   __shared__ __attribute__((aligned(16))) char tile[2][TILE_BYTES];
-  __shared__ __attribute__((aligned(16))) float tile_hx[2][64];
+  __shared__ __attribute__((aligned(16))) float tile_hx[2][HXN];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -782,9 +811,9 @@ __device__ __forceinline__ void seed_mfma_body(const SeedArgs& a, const int qb, 
 #pragma unroll
     for (int j = 0; j < WAVE_LOADS; ++j)
       lds_dma16(__builtin_amdgcn_readfirstlane(lds_tile + b * TILE_BYTES + (j * W + wave) * 1024), base + soff[j]);
-    if (wave == 0) {
-      const int r = lane < BN ? lane : BN - 1;
-      lds_dma4(__builtin_amdgcn_readfirstlane(lds_hx + b * 64 * 4), a.hx + t + r);
+    if (wave < HXW) {
+      const int r = wave * 64 + lane < BN ? wave * 64 + lane : BN - 1;
+      lds_dma4(__builtin_amdgcn_readfirstlane(lds_hx + (b * HXN + wave * 64) * 4), a.hx + t + r);
     }
   };
   const int64_t step = a.parts;
@@ -911,23 +940,35 @@ hipError_t launch_seed_select_group(const GroupTab& g, int vmax, hipStream_t st)
   return hipGetLastError();
 }
 
+// Grouped form: workgroup b of problem p is query block b % nqb of part b / nqb.
+template <int KIND, int KS, int A, int R>
+__global__ __launch_bounds__(256, 2) void seed_mfma_group_kernel(GroupTab g) {
+  const int p = group_problem(g, blockIdx.x);
+  const SeedArgs a = static_cast<const SeedArgs*>(g.tab)[p];
+  const int64_t lb = int64_t(blockIdx.x) - g.start[p];
+  seed_mfma_body<KIND, KS, A, R>(a, (int)(lb % a.nqb), (int)(lb / a.nqb));
+}
+
+// One problem (s) or a grouped launch (g).
 template <int KIND, int KS>
-static hipError_t launch_seed_ks(const SeedArgs& s, hipStream_t st) {
+static hipError_t launch_seed_ks(const SeedArgs* s, const GroupTab* g, hipStream_t st) {
   constexpr int A = scan_groups(KS);
   constexpr int R = scan_row_blocks(KIND, KS, false);
-  hipLaunchKernelGGL((seed_mfma_kernel<KIND, KS, A, R>), dim3((unsigned)(s.nqb * s.parts)), dim3(256), 0, st, s);
+  if (g)
+    hipLaunchKernelGGL((seed_mfma_group_kernel<KIND, KS, A, R>), dim3((unsigned)g->nblocks), dim3(256), 0, st, *g);
+  else
+    hipLaunchKernelGGL((seed_mfma_kernel<KIND, KS, A, R>), dim3((unsigned)(s->nqb * s->parts)), dim3(256), 0, st,
+                       *s);
   return hipGetLastError();
 }
 
-hipError_t launch_seed_mfma(const SeedArgs& s, int kind, int ks, hipStream_t st, bool select) {
-  if (s.ntiles <= 0 || s.m <= 0) return hipSuccess;
-  if (s.parts * 2 * SEED_GROUPS > 1024 || s.parts * 2 * SEED_GROUPS < s.KP) return hipErrorInvalidValue;
+static hipError_t launch_seed_pass(const SeedArgs* s, const GroupTab* g, int kind, int ks, hipStream_t st) {
   hipError_t e = hipErrorInvalidValue;
   if (kind == 0) {
     switch (ks) {
 #define KS_CASE(K) \
   case K:          \
-    e = launch_seed_ks<0, K>(s, st); \
+    e = launch_seed_ks<0, K>(s, g, st); \
     break;
       KS_CASE(1) KS_CASE(2) KS_CASE(3) KS_CASE(4) KS_CASE(5) KS_CASE(6) KS_CASE(7) KS_CASE(8) KS_CASE(9)
       KS_CASE(10) KS_CASE(12) KS_CASE(14) KS_CASE(16) KS_CASE(20) KS_CASE(24) KS_CASE(28) KS_CASE(32)
@@ -940,7 +981,7 @@ hipError_t launch_seed_mfma(const SeedArgs& s, int kind, int ks, hipStream_t st,
     switch (ks) {
 #define KS_CASE(K) \
   case K:          \
-    e = launch_seed_ks<1, K>(s, st); \
+    e = launch_seed_ks<1, K>(s, g, st); \
     break;
       KS_CASE(1) KS_CASE(2) KS_CASE(3) KS_CASE(4) KS_CASE(6) KS_CASE(8) KS_CASE(12) KS_CASE(16) KS_CASE(20)
       KS_CASE(24) KS_CASE(32)
@@ -949,9 +990,21 @@ hipError_t launch_seed_mfma(const SeedArgs& s, int kind, int ks, hipStream_t st,
         return hipErrorInvalidValue;
     }
   }
+  return e;
+}
+
+hipError_t launch_seed_mfma(const SeedArgs& s, int kind, int ks, hipStream_t st, bool select) {
+  if (s.ntiles <= 0 || s.m <= 0) return hipSuccess;
+  if (s.parts * 2 * SEED_GROUPS > 1024 || s.parts * 2 * SEED_GROUPS < s.KP) return hipErrorInvalidValue;
+  const hipError_t e = launch_seed_pass(&s, nullptr, kind, ks, st);
   if (e != hipSuccess || !select) return e;
   hipLaunchKernelGGL(seed_select_kernel, dim3((unsigned)s.m), dim3(64), 0, st, s);
   return hipGetLastError();
+}
+
+hipError_t launch_seed_mfma_group(const GroupTab& g, int kind, int ks, hipStream_t st) {
+  if (g.np <= 0 || g.nblocks <= 0) return hipSuccess;
+  return launch_seed_pass(nullptr, &g, kind, ks, st);
 }
 
 template <int KIND, int KS, bool RESID>
@@ -1115,9 +1168,13 @@ void check_shape(int kind, int ks, int W, bool resid, Violations& v) {
           v.add("staged chunk differs from the chunk read", src[byte / 16], (long long)row * kp_bytes + s * 32 + h * 16);
         }
       }
-  // C inputs: lane r < BN of wave 0 copies row t + r; reads of float4 at
-  // rb * 32 + q4 * 8 + half * 4 stay inside the 64 floats of a buffer
-  if ((R - 1) * 32 + 3 * 8 + 4 + 4 > 64) v.add("C-input read", (R - 1) * 32 + 32, 64);
+  // C inputs: lane l of wave w < HXW copies row t + 64 w + l (rows past BN
+  // repeat the last) into word 64 w + l of the buffer's HXN; reads of float4 at
+  // rb * 32 + q4 * 8 + half * 4 stay inside the words copied from rows < BN
+  const int HXN = scan_hx_words(R), HXW = scan_hx_waves(R);
+  if (HXW > W) v.add("C-input copying waves", HXW, W);
+  if (64 * HXW > HXN) v.add("C-input copy destination", 64 * HXW, HXN);
+  if ((R - 1) * 32 + 3 * 8 + 4 + 4 > std::min(HXN, 64 * std::min(HXW, W))) v.add("C-input read", (R - 1) * 32 + 32, HXN);
   // the thresholds ride in 64 words per wave and buffer; a wave reads g * 32 + col
   if (32 * scan_groups(ks) > 64) v.add("threshold read", 32 * scan_groups(ks), 64);
 }
@@ -1152,14 +1209,15 @@ int64_t scan_check_launch(const ScanMfmaArgs& a, int kind, int ks, int V, int ns
     }
   }
   if (v.count == 0) {
-    const int A = scan_groups(ks), R = scan_row_blocks(kind, ks, resid);
+    const int A = scan_groups(ks), R = scan_row_blocks(kind, ks, resid), RC = scan_row_chunk(kind, ks, resid);
     const int NS = 64 * V, BN = 32 * R, QPB = W * 32 * A;
     const size_t kp_bytes = size_t(ks) * 32;
     const size_t es = kind == 0 ? 2 : (kind == 1 ? 4 : 1);
     if (size_t(a.Kp) * es != kp_bytes) v.add("operand row bytes", (long long)a.Kp * es, (long long)kp_bytes);
     if (a.KP < 1 || 2 * a.KP > NS) v.add("kept candidates KP", a.KP, NS / 2);
-    // a merge leaves cnt <= Csub - 16 R before a tile, whose lanes then append at most 16 R
-    if (a.Csub < 16 * R) v.add("append segment shorter than one tile's appends", a.Csub, 16 * R);
+    // a merge leaves cnt <= Csub - 16 RC before a chunk of RC row blocks, whose
+    // lanes then append at most 16 RC
+    if (a.Csub < 16 * RC) v.add("append segment shorter than one chunk's appends", a.Csub, 16 * RC);
     if (a.KP + 2 * a.Csub > NS) v.add("kept list + append segments", a.KP + 2 * a.Csub, NS);
     if (a.rows_per_split <= 0 || a.rows_per_split % BN) v.add("rows per split, whole tiles", a.rows_per_split, BN);
     if (a.n_pad <= 0 || a.n_pad % BN) v.add("operand rows, whole tiles", a.n_pad, BN);
