@@ -245,12 +245,18 @@ constexpr bool scan_two_waves(int kind, int ks, bool resid) {
 // barrier and the copy issue of a tile are shared by several blocks' MFMAs and
 // selection.
 constexpr bool scan_four_wg(int kind, int ks) { return kind == 0 ? ks <= 4 : (kind == 1 && ks <= 2); }
+// Plain bf16 shapes of 5-8 K steps take 128-row tiles in two chunks of two
+// blocks (one barrier per 128 rows; two tile buffers then still leave two
+// workgroups resident).
+constexpr bool scan_wide_tile(int kind, int ks, bool resid) { return kind == 0 && !resid && ks >= 5 && ks <= 8; }
 constexpr int scan_row_blocks(int kind, int ks, bool resid) {
-  return scan_four_wg(kind, ks) ? (resid || ks > 3 ? 2 : 4) : ((kind == 0 && !resid ? ks <= 12 : ks <= 16) ? 2 : 1);
+  return scan_four_wg(kind, ks)            ? (resid || ks > 3 ? 2 : 4)
+         : scan_wide_tile(kind, ks, resid) ? 4
+                                           : ((kind == 0 && !resid ? ks <= 12 : ks <= 16) ? 2 : 1);
 }
 // Row blocks whose accumulators are live at once (R for the other shapes).
 constexpr int scan_row_chunk(int kind, int ks, bool resid) {
-  return scan_four_wg(kind, ks) ? 1 : scan_row_blocks(kind, ks, resid);
+  return scan_four_wg(kind, ks) ? 1 : (scan_wide_tile(kind, ks, resid) ? 2 : scan_row_blocks(kind, ks, resid));
 }
 
 // Tile buffers of a shape: three (the copy of tile t + 2 leaves when tile t
@@ -777,6 +783,7 @@ __device__ __forceinline__ void seed_mfma_body(const SeedArgs& a, const int qb, 
   constexpr int WAVE_LOADS = scan_wave_loads(KS, R, W);
   constexpr int TILE_BYTES = scan_tile_bytes(KS, R, W);
   constexpr int HXN = scan_hx_words(R), HXW = scan_hx_waves(R);
+  constexpr int RC = scan_row_chunk(KIND, KS, false);
   static_assert(HXW <= W, "every C input has a wave that copies it");
   __shared__ __attribute__((aligned(16))) char tile[2][TILE_BYTES];
   __shared__ __attribute__((aligned(16))) float tile_hx[2][HXN];
@@ -826,46 +833,54 @@ __device__ __forceinline__ void seed_mfma_body(const SeedArgs& a, const int qb, 
   for (; i < a.ntiles; i += step) {
     const char* tb = tile[buf];
     const float* hb = tile_hx[buf];
-    floatx16 acc[A][R];
+    // chunks of RC row blocks, as the scan takes them; block c0 + rc of the
+    // tile lands in group register rc, and the registers rotate by RC per chunk
+    // (so by R per tile)
+#pragma unroll 1
+    for (int c0 = 0; c0 < R; c0 += RC) {
+      floatx16 acc[A][RC];
 #pragma unroll
-    for (int rb = 0; rb < R; ++rb) {
-      floatx16 h;
+      for (int rc = 0; rc < RC; ++rc) {
+        floatx16 h;
 #pragma unroll
-      for (int q4 = 0; q4 < 4; ++q4) {
-        const float4 h4 = *reinterpret_cast<const float4*>(hb + rb * 32 + q4 * 8 + half * 4);
-        h[q4 * 4 + 0] = h4.x;
-        h[q4 * 4 + 1] = h4.y;
-        h[q4 * 4 + 2] = h4.z;
-        h[q4 * 4 + 3] = h4.w;
+        for (int q4 = 0; q4 < 4; ++q4) {
+          const float4 h4 = *reinterpret_cast<const float4*>(hb + (c0 + rc) * 32 + q4 * 8 + half * 4);
+          h[q4 * 4 + 0] = h4.x;
+          h[q4 * 4 + 1] = h4.y;
+          h[q4 * 4 + 2] = h4.z;
+          h[q4 * 4 + 3] = h4.w;
+        }
+#pragma unroll
+        for (int g = 0; g < A; ++g) acc[g][rc] = h;
       }
 #pragma unroll
-      for (int g = 0; g < A; ++g) acc[g][rb] = h;
-    }
+      for (int s = 0; s < KS; ++s) {
+        uint4 af[RC];
 #pragma unroll
-    for (int s = 0; s < KS; ++s) {
-      uint4 af[R];
+        for (int rc = 0; rc < RC; ++rc)
+          af[rc] = *reinterpret_cast<const uint4*>(tb + ((c0 + rc) * 32 + col) * ROWB + s * 32 + half * 16);
 #pragma unroll
-      for (int rb = 0; rb < R; ++rb)
-        af[rb] = *reinterpret_cast<const uint4*>(tb + (rb * 32 + col) * ROWB + s * 32 + half * 16);
+        for (int g = 0; g < A; ++g)
+#pragma unroll
+          for (int rc = 0; rc < RC; ++rc) Mfma<KIND>::step(acc[g][rc], af[rc], qf[g][s]);
+      }
 #pragma unroll
       for (int g = 0; g < A; ++g)
 #pragma unroll
-        for (int rb = 0; rb < R; ++rb) Mfma<KIND>::step(acc[g][rb], af[rb], qf[g][s]);
-    }
+        for (int rc = 0; rc < RC; ++rc) {
+          float mx = acc[g][rc][0];
 #pragma unroll
-    for (int g = 0; g < A; ++g) {
+          for (int r = 1; r < 16; ++r) mx = fmaxf(mx, acc[g][rc][r]);
+          best[g][rc] = fmaxf(best[g][rc], mx);
+        }
 #pragma unroll
-      for (int rb = 0; rb < R; ++rb) {
-        float mx = acc[g][rb][0];
+      for (int g = 0; g < A; ++g) {
+        float rot[G];
 #pragma unroll
-        for (int r = 1; r < 16; ++r) mx = fmaxf(mx, acc[g][rb][r]);
-        best[g][rb] = fmaxf(best[g][rb], mx);
+        for (int j = 0; j < G; ++j) rot[j] = best[g][(j + RC) % G];
+#pragma unroll
+        for (int j = 0; j < G; ++j) best[g][j] = rot[j];
       }
-      float rot[G];
-#pragma unroll
-      for (int j = 0; j < G; ++j) rot[j] = best[g][(j + R) % G];
-#pragma unroll
-      for (int j = 0; j < G; ++j) best[g][j] = rot[j];
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
