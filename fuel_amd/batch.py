@@ -6,7 +6,8 @@ searched, self dropped, truncated to k).  Problems of one dimension share one
 persistent MFMA scan; the whole level-0 pass is a captured hipGraph replayed
 by every ``run``; the autotuner (``mode="auto"``) picks, per dimension, the
 MFMA precision, candidate widening and split size by timing them on a sample
-of that dimension's problems.  Results equal ``SearchIndex.knn`` /
+of that dimension's problems, then its four leading choices on all of them.
+Results equal ``SearchIndex.knn`` /
 ``fuel_amd_knn`` on each problem alone.
 """
 import ctypes

@@ -393,7 +393,8 @@ typedef struct fuel_amd_problem {
 
 typedef struct fuel_amd_batch_opts {
   int32_t mode;     /* FUEL_AMD_MODE_AUTO: the autotuner picks mode, widening and split size per
-                       dimension by timing them on a sample of that dimension's problems;
+                       dimension by timing them on a sample of that dimension's problems, then
+                       its four leading choices on all of them;
                        FUEL_AMD_MODE_BF16 / _FP32 / _EXACT: fixed for every problem */
   int32_t widen;    /* candidate widening with a fixed mode (0: 4) */
   int32_t device;
