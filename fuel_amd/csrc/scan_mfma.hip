@@ -514,7 +514,9 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
     // Row blocks c0 .. c0 + RC - 1 of the tile at a time (RC = R but for the
     // shapes of few K steps, whose tiles hold more row blocks than they keep
     // accumulators for).
-#pragma unroll 1
+    // (unrolled where a chunk holds several blocks: 0.7 % on the 8-step scan;
+    // rolled for the four-workgroup shapes, whose 128 registers it would strain)
+#pragma unroll(RC > 1 ? R / RC : 1)
     for (int c0 = 0; c0 < R; c0 += RC) {
     // C input: hxn of the 16 rows this lane holds, shared by the query groups
     floatx16 acc[A][RC];
