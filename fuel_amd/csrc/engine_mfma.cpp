@@ -311,9 +311,10 @@ SeedPlan plan_seed(int mk, int ks, bool resid, bool has_theta, int flags, int64_
     return p;
   }
   // Sample fraction: a query still appends about KP / f rows after the seed and
-  // the seed reads f n rows, so f grows as sqrt(KP / n): 1/16 at KP = 40 and
-  // 10^6 rows (where it was measured best), at most 1/4.
-  const double f = std::min(0.25, std::sqrt(25000.0 * double(KP) / double(n_pad)) / 16.0);
+  // the seed reads f n rows, so f grows as sqrt(KP / n): 1/11.3 at KP = 40 and
+  // 10^6 rows (measured against 1/16 and 1/22.6 on one box: 26.4 against 26.9
+  // and 26.6 ms per step), at most 1/4.
+  const double f = std::min(0.25, std::sqrt(50000.0 * double(KP) / double(n_pad)) / 16.0);
   p.ntiles = std::min<int64_t>(4096, std::max<int64_t>(need, (int64_t)(f * double(tiles))));
   p.ntiles = std::min(p.ntiles, tiles);
   p.stride = (tiles / p.ntiles) * rows_tile;
