@@ -584,8 +584,8 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
     if (e != hipSuccess) return hip_fail(e, "query preparation");
 
     // Seeded thresholds for a cold start of the plain operands: the KP-th
-    // smallest group minimum over an evenly strided sample of about 1/16 of the
-    // tiles (at most 1024) lowers each query's shared threshold before the scan,
+    // smallest group minimum over an evenly strided sample of the tiles
+    // (plan_seed) lowers each query's shared threshold before the scan,
     // so its splits append few rows that a tighter threshold would drop.
     {
       const SeedPlan sp = plan_seed(mk, ks, resid, d_theta != nullptr, ix->opts.flags, n_pad, pl.rows_tile, KP);

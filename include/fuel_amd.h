@@ -92,7 +92,7 @@ typedef struct fuel_amd_index fuel_amd_index;
 #define FUEL_AMD_FLAG_FALLBACK_ARITH 1
 /* FUEL_AMD_FLAG_NO_SEED: start the plain MFMA scans (FP32 / BF16) at no
  * threshold instead of the seeded one (the KP-th smallest of per-row-group key
- * minima over a ~1/16 row sample, computed on the matrix cores first), and the
+ * minima over a ~1/11 row sample, computed on the matrix cores first), and the
  * residual scans (BF16R / FP8R) at their list bound instead of the sampled one
  * (the KP-th smallest exact distance to the first rows of the query's nearest
  * list).  Answers are the same either way; the seeds only save candidate
