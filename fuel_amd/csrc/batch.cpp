@@ -1183,13 +1183,17 @@ int32_t fuel_amd_batch_describe(fuel_amd_batch* b, char* text, int32_t len) {
   for (size_t g = 0; g < b->buckets.size(); ++g) {
     const auto& B = b->buckets[g];
     int64_t rows = 0;
-    for (int i : B.members) rows += b->probs[i].n;
-    char buf[320];
+    int seeded = 0;  // problems whose level 0 starts from a seeded threshold
+    for (int i : B.members) {
+      rows += b->probs[i].n;
+      seeded += (b->plan[i].seed.ntiles > 0 || b->plan[i].kind == 2) ? 1 : 0;
+    }
+    char buf[360];
     std::snprintf(buf, sizeof(buf),
                   "%s{\"d\": %d, \"problems\": %zu, \"rows\": %lld, \"mode\": \"%s\", \"widen\": %d, \"fill\": %d, "
-                  "\"k_steps\": %d, \"work_items\": %lld, \"workgroups\": %d}",
+                  "\"k_steps\": %d, \"work_items\": %lld, \"workgroups\": %d, \"seeded\": %d}",
                   g ? ", " : "", B.d, B.members.size(), (long long)rows, kind_name(B.cfg.kind), B.cfg.widen,
-                  B.cfg.fill, B.ks, (long long)B.nitems, B.grid);
+                  B.cfg.fill, B.ks, (long long)B.nitems, B.grid, seeded);
     s += buf;
   }
   int64_t nexact = 0;

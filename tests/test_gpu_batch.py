@@ -64,6 +64,26 @@ def test_graph_replays_with_eager_work_between(gpu, oracle, problems, mode):
     eager.close()
 
 
+@pytest.mark.parametrize("mode", ["bf16", "fp32", "bf16r"])
+def test_seeded_problems(gpu, oracle, mode):
+    # problems large enough that level 0 starts from a seeded threshold (the
+    # grouped seed passes and their selection; the residual form's sampled
+    # seed), beside small ones that start cold, in one group per dimension
+    from fuel_amd import BatchKnn
+    probs = _problems(53, 2, 9000, 12000, [32]) + _problems(54, 2, 9000, 11000, [64]) + \
+        _problems(55, 3, 300, 2000, [32, 64]) + _problems(56, 1, 8500, 9000, [8])
+    b = BatchKnn(probs, 15, mode=mode)
+    b.run()
+    _check(oracle, probs, b.results(), 15)
+    info = b.describe()
+    seeded = {g["d"]: g["seeded"] for g in info["groups"]}
+    # (the f32 form seeds up to 4 K steps only: d <= 32)
+    assert seeded[32] >= 2 and (mode == "fp32" or seeded[64] >= 2), info["groups"]
+    b.run()  # a replay: same answers
+    _check(oracle, probs, b.results(), 15)
+    b.close()
+
+
 def test_autotuned_plan(gpu, oracle):
     from fuel_amd import BatchKnn
     probs = _problems(52, 10, 300, 3000, [8, 64, 256])
