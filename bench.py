@@ -52,7 +52,7 @@ def parse():
     p.add_argument("--c3-full", type=int, default=1,
                    help="headline line: GPU LOF of the full 70,000 x 784 workload against the oracle (0: skip)")
     p.add_argument("--c4-line", type=int, default=1,
-                   help="headline line: the 512 batched problems (plan, passes, parity on the smallest third)")
+                   help="headline line: the 512 batched problems (plan, passes, parity on the smallest third and on the largest problem of at most 25,000 rows of each dimension)")
     p.add_argument("--c4-steps", type=int, default=3, help="timed passes of the batched problems")
     p.add_argument("--tune-cache", default=None,
                    help="file keeping the batch autotuner's choices across runs (read, then appended to)")
@@ -364,8 +364,9 @@ def c3_full(threads):
 
 def c4_run(args, threads, steps, warmup):
     """The 512 batched uneven self-kNN problems: plan (autotuned), graph
-    replays timed, answers of the smallest third checked against the oracle and
-    a hash of every problem's answer."""
+    replays timed, answers of the smallest third and of each dimension's largest
+    problem of at most 25,000 rows checked against the oracle, and a hash of
+    every problem's answer."""
     import hashlib
 
     import torch
@@ -398,6 +399,12 @@ def c4_run(args, threads, steps, warmup):
         h.update(gd.tobytes())
     order = sorted(range(len(probs)), key=lambda i: probs[i].shape[0])
     checked = order[:len(probs) // 3]  # the smallest third, in full
+    # and per dimension the largest problem of at most 25,000 rows (seeded
+    # thresholds start at about 8,000 rows; the oracle takes seconds for these)
+    for dd in sorted({X.shape[1] for X in probs}):
+        cand = [i for i in order if probs[i].shape[1] == dd and probs[i].shape[0] <= 25000]
+        if cand and cand[-1] not in checked:
+            checked.append(cand[-1])
     t0 = time.perf_counter()
     same = True
     for i in checked:
@@ -416,7 +423,8 @@ def c4_run(args, threads, steps, warmup):
            "roofline": {"bound": "mfma", "achieved": round(achieved / 1e12, 2), "peak": PEAKS["bf16"][0] / 1e12,
                         "unit": "TFLOP/s", "frac": round(achieved / PEAKS["bf16"][0], 4),
                         "note": "2 n^2 d per problem over the whole pass; bf16 dense peak"},
-           "answers_sha256": h.hexdigest(), "checked_problems": len(checked), "identical": same}
+           "answers_sha256": h.hexdigest(), "checked_problems": len(checked),
+           "checked_max_rows": int(max(probs[i].shape[0] for i in checked)), "identical": same}
     b.close()
     return out, probs, info
 
