@@ -412,8 +412,9 @@ int32_t fuel_amd_batch_create(const fuel_amd_problem* problems, int32_t count, i
 /* One pass over every problem; synchronous.  Calls on one batch from several
  * threads run one after another. */
 int32_t fuel_amd_batch_run(fuel_amd_batch* batch);
-/* The plan (per dimension: mode, widening, split size, work items, tuning times)
- * and the last run's statistics as JSON text (truncated to len - 1 bytes). */
+/* The plan (per dimension: mode, widening, split size, work items, the problems
+ * that start from seeded thresholds, tuning times) and the last run's statistics
+ * as JSON text (truncated to len - 1 bytes). */
 int32_t fuel_amd_batch_describe(fuel_amd_batch* batch, char* text, int32_t len);
 int32_t fuel_amd_batch_destroy(fuel_amd_batch* batch);
 
