@@ -246,17 +246,21 @@ constexpr bool scan_two_waves(int kind, int ks, bool resid) {
 // selection.
 constexpr bool scan_four_wg(int kind, int ks) { return kind == 0 ? ks <= 4 : (kind == 1 && ks <= 2); }
 // Plain bf16 shapes of 5-8 K steps take 128-row tiles in two chunks of two
-// blocks (one barrier per 128 rows; two tile buffers then still leave two
-// workgroups resident).
-constexpr bool scan_wide_tile(int kind, int ks, bool resid) { return kind == 0 && !resid && ks >= 5 && ks <= 8; }
+// blocks, those of 14-16 steps 64-row tiles in two chunks of one (one barrier
+// per tile; two tile buffers then still leave two workgroups resident).
+constexpr bool scan_wide_tile(int kind, int ks, bool resid) {
+  return kind == 0 && !resid && ((ks >= 5 && ks <= 8) || (ks >= 14 && ks <= 16));
+}
 constexpr int scan_row_blocks(int kind, int ks, bool resid) {
   return scan_four_wg(kind, ks)            ? (resid || ks > 3 ? 2 : 4)
-         : scan_wide_tile(kind, ks, resid) ? 4
+         : scan_wide_tile(kind, ks, resid) ? (ks <= 8 ? 4 : 2)
                                            : ((kind == 0 && !resid ? ks <= 12 : ks <= 16) ? 2 : 1);
 }
 // Row blocks whose accumulators are live at once (R for the other shapes).
 constexpr int scan_row_chunk(int kind, int ks, bool resid) {
-  return scan_four_wg(kind, ks) ? 1 : (scan_wide_tile(kind, ks, resid) ? 2 : scan_row_blocks(kind, ks, resid));
+  return scan_four_wg(kind, ks)              ? 1
+         : scan_wide_tile(kind, ks, resid) ? (ks <= 8 ? 2 : 1)
+                                             : scan_row_blocks(kind, ks, resid);
 }
 
 // Tile buffers of a shape: three (the copy of tile t + 2 leaves when tile t
@@ -514,9 +518,10 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
     // Row blocks c0 .. c0 + RC - 1 of the tile at a time (RC = R but for the
     // shapes of few K steps, whose tiles hold more row blocks than they keep
     // accumulators for).
-    // (unrolled where a chunk holds several blocks: 0.7 % on the 8-step scan;
-    // rolled for the four-workgroup shapes, whose 128 registers it would strain)
-#pragma unroll(RC > 1 ? R / RC : 1)
+    // (unrolled but for the four-workgroup shapes, whose 128 registers it would
+    // strain: same-box A/B, 0.7 % on the 8-step scan, 2 % on the batched d = 256
+    // problems, whose 64-row tiles were slower than 32-row ones when rolled)
+#pragma unroll(scan_four_wg(KIND, KS) ? 1 : R / RC)
     for (int c0 = 0; c0 < R; c0 += RC) {
     // C input: hxn of the 16 rows this lane holds, shared by the query groups
     floatx16 acc[A][RC];
