@@ -518,10 +518,10 @@ __device__ __forceinline__ void scan_mfma_body(const ScanMfmaArgs& a, const int 
     // Row blocks c0 .. c0 + RC - 1 of the tile at a time (RC = R but for the
     // shapes of few K steps, whose tiles hold more row blocks than they keep
     // accumulators for).
-    // (unrolled but for the four-workgroup shapes, whose 128 registers it would
-    // strain: same-box A/B, 0.7 % on the 8-step scan, 2 % on the batched d = 256
-    // problems, whose 64-row tiles were slower than 32-row ones when rolled)
-#pragma unroll(scan_four_wg(KIND, KS) ? 1 : R / RC)
+    // (unrolled: same-box A/B, 0.7 % on the 8-step scan, 2 % on the batched
+    // d = 256 problems, whose 64-row tiles were slower than 32-row ones when
+    // rolled, and 1 % on the whole batched pass for the four-workgroup shapes)
+#pragma unroll(R / RC)
     for (int c0 = 0; c0 < R; c0 += RC) {
     // C input: hxn of the 16 rows this lane holds, shared by the query groups
     floatx16 acc[A][RC];
