@@ -91,6 +91,7 @@ struct Bucket {
   DevBuf group_dev;
   const FillEntry* fills = nullptr;
   int nfill = 0, sel_vmax = 4;
+  bool thread_rows = false;  // operand copies and query preparation one row per thread
   GroupTab g_db{}, g_q{}, g_seed{}, g_sel{}, g_rr{};
   GroupTab g_resid{}, g_qlist{}, g_theta{}, g_sample{};  // residual problems
 };
@@ -404,6 +405,10 @@ int32_t build_group_tables(fuel_amd_batch* b, Bucket& B) {
   std::vector<int64_t> db_start{0}, q_start{0}, seed_start{0}, sel_start{0}, rr_start{0};
   std::vector<int64_t> rp_start{0}, ql_start{0}, th_start{0}, sm_start{0};
   int vmax = 1;
+  // rows of at most 64 operand columns: one row per thread (the batch's rows
+  // are 32-byte aligned with pitches a multiple of 8, and so are its copies)
+  B.thread_rows = !B.members.empty() && b->plan[B.members[0]].Kp <= PREP_THREAD_ROWS_MAX_KP;
+  const int64_t db_rows = B.thread_rows ? 256 : 64, q_rows = B.thread_rows ? 256 : 4;
   for (int i : B.members) {
     const auto& pp = b->plan[i];
     const auto& pr = b->probs[i];
@@ -412,7 +417,7 @@ int32_t build_group_tables(fuel_amd_batch* b, Bucket& B) {
       fills.push_back({reinterpret_cast<uint32_t*>(pa.consts), 16, 0u, 0u});
       fills.push_back({reinterpret_cast<uint32_t*>(pa.flags), 2, 1u, 0u});
       dbs.push_back(pa);
-      db_start.push_back(db_start.back() + (pa.n_pad + 63) / 64);
+      db_start.push_back(db_start.back() + (pa.n_pad + db_rows - 1) / db_rows);
     } else {
       fills.push_back({reinterpret_cast<uint32_t*>(pa.consts), 32, 0u, 0u});  // constants, zero flags
       fills.push_back({at<uint32_t>(b->pool, pp.o_lconsts), int64_t(16) * b->parts[i]->L, 0u, 0u});
@@ -433,7 +438,7 @@ int32_t build_group_tables(fuel_amd_batch* b, Bucket& B) {
     fills.push_back({reinterpret_cast<uint32_t*>(b->fail_counts.as<int>() + i), 1, 0u, 0u});
     const PrepQArgs qa = prep_q_args(b, i);
     qs.push_back(qa);
-    q_start.push_back(q_start.back() + (qa.m_pad + 3) / 4);
+    q_start.push_back(q_start.back() + (qa.m_pad + q_rows - 1) / q_rows);
     if (pp.seed.ntiles > 0) {
       const SeedArgs sd = seed_args(b, i);
       const int total = sd.parts * 2 * SEED_GROUPS;
@@ -716,9 +721,9 @@ SampleSeedArgs sample_args(fuel_amd_batch* b, int i) {
 hipError_t enqueue_bucket(fuel_amd_batch* b, const Bucket& B, hipStream_t st) {
   const int mk = B.cfg.kind == 2 ? 0 : B.cfg.kind;
   hipError_t e = launch_fill_multi(B.fills, B.nfill, st);
-  if (e == hipSuccess) e = launch_prep_db_group(B.g_db, st);
+  if (e == hipSuccess) e = launch_prep_db_group(B.g_db, B.thread_rows, st);
   if (e == hipSuccess) e = launch_prep_resid_group(B.g_resid, st);
-  if (e == hipSuccess) e = launch_prep_queries_group(B.g_q, st);
+  if (e == hipSuccess) e = launch_prep_queries_group(B.g_q, B.thread_rows, st);
   if (e == hipSuccess) e = launch_seed_mfma_group(B.g_seed, mk, B.ks, st);
   if (e == hipSuccess) e = launch_seed_select_group(B.g_sel, B.sel_vmax, st);
   for (int i : B.members)

@@ -517,8 +517,13 @@ struct FillEntry {
   uint32_t pad;
 };
 hipError_t launch_fill_multi(const FillEntry* tab, int n, hipStream_t st);
-hipError_t launch_prep_db_group(const GroupTab& g, hipStream_t st);       // PrepDbArgs, (n_pad + 63) / 64 each
-hipError_t launch_prep_queries_group(const GroupTab& g, hipStream_t st);  // PrepQArgs, (m_pad + 3) / 4 each
+// PrepDbArgs, (n_pad + 63) / 64 workgroups each, and PrepQArgs, (m_pad + 3) / 4
+// each; thread_rows: one row per thread instead of one per wave, (n_pad + 255)
+// / 256 and (m_pad + 255) / 256 workgroups each, for Kp <= 64 with rows 16-byte
+// aligned and pitches (ldx, ldq, ld32) multiples of 4
+constexpr int PREP_THREAD_ROWS_MAX_KP = 64;
+hipError_t launch_prep_db_group(const GroupTab& g, bool thread_rows, hipStream_t st);
+hipError_t launch_prep_queries_group(const GroupTab& g, bool thread_rows, hipStream_t st);
 hipError_t launch_rerank_group(const GroupTab& g, int V, int arith, hipStream_t st);  // RerankArgs, nq each
 // SeedArgs, m each; vmax = 64-entry registers for the largest parts * 2 * SEED_GROUPS
 hipError_t launch_seed_select_group(const GroupTab& g, int vmax, hipStream_t st);

@@ -68,6 +68,152 @@ __device__ __forceinline__ RowStats prep_row(const float* __restrict__ src, int 
   return r;
 }
 
+// One row per thread, for operand rows of at most 64 columns (there one wave
+// per row leaves most lanes idle): the row's Kp values in steps of 4 (source
+// rows 16-byte aligned, pitches multiples of 4), the sums in f64 in column
+// order (any order: the bound's slack covers the f64 error of a d-term sum).
+__device__ __forceinline__ RowStats prep_row_thread(const float* __restrict__ src, int d, int Kp, int bf16,
+                                                    void* dst) {
+  double s2 = 0.0, e2 = 0.0, t2 = 0.0;
+  bool isint = true, nonneg = true;
+  for (int c0 = 0; c0 < Kp; c0 += 4) {
+    float v[4];
+    if (c0 + 4 <= d) {
+      const float4 f = *reinterpret_cast<const float4*>(src + c0);
+      v[0] = f.x;
+      v[1] = f.y;
+      v[2] = f.z;
+      v[3] = f.w;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) v[e] = c0 + e < d ? src[c0 + e] : 0.0f;
+    }
+    float vt[4];
+    if (bf16) {
+      uint16_t h[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        h[e] = bf16_rne(v[e]);
+        vt[e] = bf16_val(h[e]);
+      }
+      if (dst)
+        *reinterpret_cast<uint2*>(static_cast<uint16_t*>(dst) + c0) =
+            make_uint2(uint32_t(h[0]) | (uint32_t(h[1]) << 16), uint32_t(h[2]) | (uint32_t(h[3]) << 16));
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) vt[e] = v[e];
+      if (dst) *reinterpret_cast<float4*>(static_cast<float*>(dst) + c0) = make_float4(v[0], v[1], v[2], v[3]);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const double dv = (double)v[e], dt = (double)vt[e];
+      s2 += dv * dv;
+      e2 += (dt - dv) * (dt - dv);
+      t2 += dt * dt;
+      isint = isint && (v[e] == rintf(v[e])) && fabsf(v[e]) <= 256.0f;
+      nonneg = nonneg && v[e] >= 0.0f;
+    }
+  }
+  RowStats r;
+  r.s2 = s2;
+  r.e2 = e2;
+  r.t2 = t2;
+  r.isint = isint;
+  r.nonneg = nonneg;
+  return r;
+}
+
+__device__ __forceinline__ double wave_max_d(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m));
+  return v;
+}
+
+// The thread-per-row form of prep_db_body: workgroup `blk` prepares rows
+// 256 blk + threadIdx.x.
+__device__ __forceinline__ void prep_db_thread_body(const PrepDbArgs& a, const int64_t blk) {
+  __shared__ double red[4][5];
+  __shared__ int redf[4][2];
+  const int wave = threadIdx.x >> 6;
+  const int lane = __lane_id();
+  const size_t es = a.bf16 ? 2 : 4;
+  const double slack = 1.0 + (a.d + 4) * 0x1p-52;
+  double m[5] = {0, 0, 0, 0, 0};
+  bool allint = true, allnonneg = true;
+  const int64_t row = blk * 256 + threadIdx.x;
+  if (row < a.n_pad) {
+    char* dst = reinterpret_cast<char*>(a.Xop) + size_t(row) * a.Kp * es;
+    if (row >= a.n) {
+      for (int c = 0; c < a.Kp * (int)es; c += 16) *reinterpret_cast<uint4*>(dst + c) = make_uint4(0, 0, 0, 0);
+      a.hx[row] = MFMA_PAD_HXN;
+    } else {
+      const RowStats r = prep_row_thread(a.X + row * a.ldx, a.d, a.Kp, a.bf16, dst);
+      const double v = r.s2 * 0.5;
+      const float h = (float)v;
+      a.hx[row] = -h;
+      m[0] = r.s2 * slack;
+      m[1] = r.e2 * slack;
+      m[2] = r.t2 * slack;
+      m[3] = fabs((double)h - v) + (r.isint ? 0.0 : v * (a.d + 4) * 0x1p-53);
+      m[4] = (double)h;
+      allint = r.isint;
+      allnonneg = r.nonneg;
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 5; ++t) m[t] = wave_max_d(m[t]);
+  allint = __all(allint);
+  allnonneg = __all(allnonneg);
+  if (lane == 0) {
+    for (int t = 0; t < 5; ++t) red[wave][t] = m[t];
+    redf[wave][0] = allint ? 1 : 0;
+    redf[wave][1] = allnonneg ? 1 : 0;
+  }
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    const int t = threadIdx.x;
+    const double v = fmax(fmax(red[0][t], red[1][t]), fmax(red[2][t], red[3][t]));
+    if (v > 0.0) atomic_max_nonneg(&a.consts[t], v);
+  } else if (threadIdx.x < 7) {
+    const int f = threadIdx.x - 5;
+    if (!(redf[0][f] && redf[1][f] && redf[2][f] && redf[3][f])) atomicAnd(&a.flags[f], 0);
+  }
+}
+
+// The thread-per-row form of prep_query_body: rows 256 blk + threadIdx.x.
+__device__ __forceinline__ void prep_query_thread_body(const PrepQArgs& a, const int64_t blk) {
+  const int64_t row = blk * 256 + threadIdx.x;
+  if (row >= a.m_pad) return;
+  const size_t es = a.bf16 ? 2 : 4;
+  char* dst = a.Qop ? reinterpret_cast<char*>(a.Qop) + size_t(row) * a.Kp * es : nullptr;
+  if (a.gthr)
+    a.gthr[row] = row >= a.m ? ord_f32(-MFMA_KEY_LIMIT) : (a.theta ? ord_f32(a.theta[row]) : ord_f32(INFINITY));
+  if (row >= a.m) {
+    if (dst)
+      for (int c = 0; c < a.Kp * (int)es; c += 16) *reinterpret_cast<uint4*>(dst + c) = make_uint4(0, 0, 0, 0);
+    return;
+  }
+  const float* src = a.Q + (a.rows ? a.rows[row] : row) * a.ldq;
+  float* q32 = a.Q32 + row * a.ld32;
+  for (int c0 = 0; c0 < a.ld32; c0 += 4) {
+    float4 f;
+    if (c0 + 4 <= a.d) {
+      f = *reinterpret_cast<const float4*>(src + c0);
+    } else {
+      f.x = c0 < a.d ? src[c0] : 0.0f;
+      f.y = c0 + 1 < a.d ? src[c0 + 1] : 0.0f;
+      f.z = c0 + 2 < a.d ? src[c0 + 2] : 0.0f;
+      f.w = c0 + 3 < a.d ? src[c0 + 3] : 0.0f;
+    }
+    *reinterpret_cast<float4*>(q32 + c0) = f;
+  }
+  const RowStats r = prep_row_thread(src, a.d, a.Kp, a.bf16, dst);
+  a.qn2[row] = r.s2;
+  a.eq2[row] = r.e2;
+  a.qt2[row] = r.t2;
+  a.qint[row] = r.isint ? 1 : 0;
+}
+
 // One workgroup per 64 rows (16 per wave; `blk` is the workgroup's block of
 // the problem); the index-wide maxima and flags are reduced in the workgroup
 // and published with one atomic each (per-row atomics on the same few
@@ -172,6 +318,17 @@ __global__ __launch_bounds__(256) void prep_query_group_kernel(GroupTab g) {
   prep_query_body(a, int64_t(blockIdx.x) - g.start[p]);
 }
 
+__global__ __launch_bounds__(256) void prep_db_group_thread_kernel(GroupTab g) {
+  const int p = group_problem(g, blockIdx.x);
+  const PrepDbArgs a = static_cast<const PrepDbArgs*>(g.tab)[p];
+  prep_db_thread_body(a, int64_t(blockIdx.x) - g.start[p]);
+}
+__global__ __launch_bounds__(256) void prep_query_group_thread_kernel(GroupTab g) {
+  const int p = group_problem(g, blockIdx.x);
+  const PrepQArgs a = static_cast<const PrepQArgs*>(g.tab)[p];
+  prep_query_thread_body(a, int64_t(blockIdx.x) - g.start[p]);
+}
+
 // One workgroup per table entry.
 __global__ __launch_bounds__(256) void fill_multi_kernel(const FillEntry* tab) {
   const FillEntry f = tab[blockIdx.x];
@@ -184,15 +341,21 @@ hipError_t launch_fill_multi(const FillEntry* tab, int n, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_prep_db_group(const GroupTab& g, hipStream_t st) {
+hipError_t launch_prep_db_group(const GroupTab& g, bool thread_rows, hipStream_t st) {
   if (g.np <= 0 || g.nblocks <= 0) return hipSuccess;
-  hipLaunchKernelGGL(prep_db_group_kernel, dim3((unsigned)g.nblocks), dim3(256), 0, st, g);
+  if (thread_rows)
+    hipLaunchKernelGGL(prep_db_group_thread_kernel, dim3((unsigned)g.nblocks), dim3(256), 0, st, g);
+  else
+    hipLaunchKernelGGL(prep_db_group_kernel, dim3((unsigned)g.nblocks), dim3(256), 0, st, g);
   return hipGetLastError();
 }
 
-hipError_t launch_prep_queries_group(const GroupTab& g, hipStream_t st) {
+hipError_t launch_prep_queries_group(const GroupTab& g, bool thread_rows, hipStream_t st) {
   if (g.np <= 0 || g.nblocks <= 0) return hipSuccess;
-  hipLaunchKernelGGL(prep_query_group_kernel, dim3((unsigned)g.nblocks), dim3(256), 0, st, g);
+  if (thread_rows)
+    hipLaunchKernelGGL(prep_query_group_thread_kernel, dim3((unsigned)g.nblocks), dim3(256), 0, st, g);
+  else
+    hipLaunchKernelGGL(prep_query_group_kernel, dim3((unsigned)g.nblocks), dim3(256), 0, st, g);
   return hipGetLastError();
 }
 
