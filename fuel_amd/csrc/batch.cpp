@@ -45,7 +45,7 @@ namespace {
 
 constexpr int KP_LIMIT = 128;  // widest kept list of the MFMA path's first level
 constexpr int KP_CHAIN = 256;  // ... and of the pass from the certified threshold
-constexpr int kChainLanes = 4;  // certification chains of different problems run concurrently
+constexpr int kChainLanes = 8;  // certification chains of different problems run concurrently (8 against 4: -0.5 % per pass)
 using Clock = std::chrono::steady_clock;
 
 struct Cfg {
