@@ -19,8 +19,10 @@
 // A and B use the same assignment, so the sums run over every coordinate once.
 //
 // Workgroup: 4 waves (1 for small residual batches), each owning A*32 queries;
-// the tile of R*32 DB rows is shared through LDS by the waves and double-buffered: the global->LDS copy
-// of tile t+2 is issued as soon as tile t's buffer is free.
+// the tile of R*32 DB rows is shared through LDS by the waves, with one barrier
+// per tile, and taken in chunks of RC 32-row blocks whose accumulators are the
+// only live ones; two or three tile buffers, the global->LDS copy of tile t+2
+// issued as soon as its buffer is free.
 //
 // Selection: a row block is skipped when no lane's maximum beats its query's
 // threshold (strict: an excluded row has key >= the KP-th kept key).  Otherwise
