@@ -84,6 +84,20 @@ def test_seeded_problems(gpu, oracle, mode):
     b.close()
 
 
+@pytest.mark.parametrize("mode", ["bf16", "fp32", "bf16r"])
+def test_odd_dimensions(gpu, oracle, mode):
+    # dimensions off the 8-column pitch: padded rows, element-wise row tails
+    # in the operand copies and query preparation, one group per dimension
+    probs = _problems(57, 6, 300, 3000, [5, 13, 31, 47, 61])
+    from fuel_amd import BatchKnn
+    b = BatchKnn(probs, 9, mode=mode)
+    b.run()
+    _check(oracle, probs, b.results(), 9)
+    b.run()
+    _check(oracle, probs, b.results(), 9)
+    b.close()
+
+
 def test_autotuned_plan(gpu, oracle):
     from fuel_amd import BatchKnn
     probs = _problems(52, 10, 300, 3000, [8, 64, 256])
