@@ -275,7 +275,12 @@ hipError_t launch_prep_resid(const PrepResidArgs& a, hipStream_t st) {
 // sum of d rounded squares of rounded differences, fused or not): |s^ - s| <=
 // g s + (d + 1) 2^-126 with g = (d + 3) u / (1 - (d + 3) u).  With N = ||q - c||
 // (upper bound), the B operand b~ = bf16(fl32(q - c)) has ||b~ - (q - c)|| <=
-// EA = N (2^-9 + 2^-23) and ||b~|| <= N + EA; per list EX = max ||r~ - r||,
+// EA = N (2^-8 + 2^-23) + sqrt(d) 2^-126 (bf16 keeps 8 significant bits, so
+// round-to-nearest errs by up to 2^-8 of a coordinate, and the f32 difference
+// by 2^-24; the last term covers differences below the smallest normal, where
+// bf16 has fewer bits and the matrix core may flush them) and ||b~|| <= N + EA;
+// per list EX = max ||r~ - r|| (measured), plus sqrt(d) 2^-126 for the same
+// flush of a row coordinate,
 // RN = max ||r||, RT = max ||r~||, EH = max |fl32(||r||^2/2) - ||r||^2/2|,
 // H = max fl32(||r||^2/2).
 //   operand rounding   N EX + EA RN + EA EX
@@ -307,14 +312,14 @@ __device__ __forceinline__ void prep_qlist_body(const QListArgs& a, const int64_
   const double s_lo = fmax(0.0, (sh - g * sh - tiny) * (1 - 0x1p-50));
   const unsigned long long* lc = a.lconsts + size_t(j) * 8;
   const double RN = sqrt(__longlong_as_double((long long)lc[0])) * (1 + 0x1p-50);
-  const double EX = sqrt(__longlong_as_double((long long)lc[1])) * (1 + 0x1p-50);
+  const double sd = sqrt((double)d) * (1 + 0x1p-50);
+  const double EX = sqrt(__longlong_as_double((long long)lc[1])) * (1 + 0x1p-50) + (a.fp8 ? 0.0 : sd * 0x1p-126);
   const double RT = sqrt(__longlong_as_double((long long)lc[2])) * (1 + 0x1p-50);
   const double EH = __longlong_as_double((long long)lc[3]);
   const double H = __longlong_as_double((long long)lc[4]);
   const double N = sqrt(s_hi) * (1 + 0x1p-50);
-  const double sd = sqrt((double)d) * (1 + 0x1p-50);
   const double EA = a.fp8 ? (N * (0x1p-8 + 0x1p-23 + sd * 0x1p-13 / 224.0) + sd * 0x1p-112) * (1 + 0x1p-40)
-                          : N * (0x1p-9 + 0x1p-23) * (1 + 0x1p-40);
+                          : (N * (0x1p-8 + 0x1p-23) + sd * 0x1p-126) * (1 + 0x1p-40);
   const double B = a.fp8 ? (N + EA) * (1 + 0x1p-3) * (1 + 0x1p-40) : (N + EA) * (1 + 0x1p-50);
   const double Q2 = 0.5 * s_hi;
   const double Q2lo = 0.5 * s_lo;

@@ -76,6 +76,34 @@ def test_uniform_and_integer_data(gpu, oracle):
     _check(gpu, oracle, C, C[:120], 20, "euclidean", 1)
 
 
+@pytest.mark.parametrize("mode", ["bf16r", "fp8r"])
+@pytest.mark.parametrize("d", [1, 2, 3])
+def test_low_dimension_uniform(gpu, oracle, mode, d):
+    # One to three coordinates: the rounding of a single query coordinate is
+    # not averaged out over many, so a residual key whose bound undercounted
+    # the bf16 rounding (2^-8 of a coordinate, not 2^-9) lay above ||q - x||^2 / 2
+    # and the pass from the certified threshold missed the nearest row
+    # (2,000 rows in 1-D, 700 queries, k = 5: one query in 700).
+    rng = np.random.default_rng(190153 + d)
+    X = rng.random((2000, d), dtype=np.float32)
+    Q = rng.random((700, d), dtype=np.float32)
+    for k, metric, exclude in [(5, "euclidean", 0), (17, "sqeuclidean", 0), (10, "euclidean", 2)]:
+        _check(gpu, oracle, X, Q if exclude == 0 else X[::3], k, metric, exclude, mode)
+    if d == 1:
+        # the sweep case that showed it, drawn as the sweep drew it
+        r = np.random.default_rng(190153)
+        r.choice([2000, 8500, 12000, 20000, 30000])
+        r.choice([1, 5, 8, 16, 24, 31, 40, 48, 64, 65, 80, 96, 100, 112, 128, 200, 224, 240, 256])
+        r.choice([1, 5, 64, 65, 200, 257, 700])
+        r.choice([1, 2, 5, 10, 17, 40, 100])
+        r.choice(["euclidean", "sqeuclidean"])
+        r.choice([0, 1, 2])
+        X1 = r.random((2000, 1), dtype=np.float32)
+        Q1 = r.random((700, 1), dtype=np.float32)
+        assert X1[1882, 0] == np.float32(0.97617567) and Q1[488, 0] == np.float32(0.97618175)
+        _check(gpu, oracle, X1, Q1, 5, "euclidean", 0, mode)
+
+
 def test_result_independent_of_batch_composition(gpu):
     rng = np.random.default_rng(9)
     X = _mixture(rng, 20000, 96, 50, -10.0, 10.0)
