@@ -98,6 +98,21 @@ def test_odd_dimensions(gpu, oracle, mode):
     b.close()
 
 
+@pytest.mark.parametrize("mode", ["bf16", "fp32", "bf16r"])
+def test_low_dimensions_uniform(gpu, oracle, mode):
+    # one to three uniform coordinates: a single coordinate's bf16 rounding is
+    # not averaged out, so every bound term must hold as stated (the residual
+    # query operand errs by up to 2^-8 of a coordinate)
+    rng = np.random.default_rng(58)
+    probs = [rng.random((int(n), d), dtype=np.float32)
+             for n, d in zip(rng.integers(500, 4000, 9), [1, 2, 3] * 3)]
+    from fuel_amd import BatchKnn
+    b = BatchKnn(probs, 5, mode=mode)
+    b.run()
+    _check(oracle, probs, b.results(), 5)
+    b.close()
+
+
 def test_autotuned_plan(gpu, oracle):
     from fuel_amd import BatchKnn
     probs = _problems(52, 10, 300, 3000, [8, 64, 256])
