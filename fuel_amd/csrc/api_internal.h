@@ -239,6 +239,15 @@ int32_t knn_mfma(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_t m, i
 // probe with this k on the first call): FUEL_AMD_MODE_BF16, _BF16R or _EXACT.
 int32_t resolve_auto_mode(fuel_amd_index* ix, QueryCtx* ctx, int k, hipStream_t st, int* mode);
 bool mfma_supported(const fuel_amd_index* ix, int kind, int k, int exclude);
+// The candidate kind the kNN path takes for a resolved mode (not AUTO) at this
+// k and exclusion: 0 bf16, 1 f32, 2 residual bf16, 3 residual E4M3, -1 the
+// exact scan (modes without an MFMA variant for the shape fall back as listed).
+int knn_kind(const fuel_amd_index* ix, int mode, int k, int exclude);
+// Query-side device bytes the first level of knn_mfma allocates for m queries
+// of `kind` at k (0 for the exact scan or an operand not yet prepared):
+// candidate slots and split counts, the query operand or per-list terms, the
+// f32 query copy, norms, thresholds, the failure list and the seed minima.
+int64_t mfma_level0_bytes(fuel_amd_index* ix, int kind, int64_t m, int k, int exclude);
 // Operand type of the scan for a candidate kind: 0 bf16 (kinds 0 and 2), 1 f32,
 // 2 E4M3 (kind 3).
 inline int mfma_operand_type(int kind) { return kind == 2 ? 0 : (kind == 3 ? 2 : kind); }

@@ -462,6 +462,18 @@ int32_t knn_exact_rows(fuel_amd_index* ix, QueryCtx* ctx, const float* Q, int64_
   return knn_exact<float>(ix, ctx, Q, nrows, ldq, k, exclude, out_idx, out_dist, st, d_rows, qbase);
 }
 
+int knn_kind(const fuel_amd_index* ix, int mode, int k, int exclude) {
+  int kind = mode == FUEL_AMD_MODE_BF16    ? 0
+             : mode == FUEL_AMD_MODE_FP32  ? 1
+             : mode == FUEL_AMD_MODE_BF16R ? 2
+             : mode == FUEL_AMD_MODE_FP8R  ? 3
+                                           : -1;
+  if (kind == 3 && !mfma_supported(ix, 3, k, exclude)) kind = 2;  // no E4M3 variant this wide
+  if (kind == 2 && !mfma_supported(ix, 2, k, exclude)) kind = 0;  // no residual variant this wide
+  if (kind >= 0 && !mfma_supported(ix, kind, k, exclude)) kind = -1;
+  return kind;
+}
+
 int32_t knn_run(fuel_amd_index* ix, QueryCtx* ctx, const void* d_query, int64_t m, int64_t ldq, int k,
                 int exclude, int64_t* d_out_idx, void* d_out_dist, hipStream_t st) {
   if (m == 0 || k == 0) return FUEL_AMD_OK;
@@ -475,14 +487,7 @@ int32_t knn_run(fuel_amd_index* ix, QueryCtx* ctx, const void* d_query, int64_t 
   ctx->stats = fuel_amd_stats{};
   ctx->stats.queries = m;
   ctx->fail_rows.clear();
-  int kind = mode == FUEL_AMD_MODE_BF16    ? 0
-             : mode == FUEL_AMD_MODE_FP32  ? 1
-             : mode == FUEL_AMD_MODE_BF16R ? 2
-             : mode == FUEL_AMD_MODE_FP8R  ? 3
-                                           : -1;
-  if (kind == 3 && !mfma_supported(ix, 3, k, exclude)) kind = 2;  // no E4M3 variant this wide
-  if (kind == 2 && !mfma_supported(ix, 2, k, exclude)) kind = 0;  // no residual variant this wide
-  if (kind >= 0 && !mfma_supported(ix, kind, k, exclude)) kind = -1;
+  const int kind = knn_kind(ix, mode, k, exclude);
   int32_t rc;
   if (kind >= 0) {
     static const int kModeOfKind[4] = {FUEL_AMD_MODE_BF16, FUEL_AMD_MODE_FP32, FUEL_AMD_MODE_BF16R,

@@ -367,6 +367,31 @@ MfmaChunkPlan plan_mfma_chunk(int mk, bool resid, int ks, int64_t m_total, int64
   return p;
 }
 
+int64_t mfma_level0_bytes(fuel_amd_index* ix, int kind, int64_t m, int k, int exclude) {
+  if (kind < 0 || kind > 3 || m <= 0) return 0;
+  auto& op = ix->mop[kind];
+  if (!op.ready.load(std::memory_order_acquire)) return 0;
+  const bool resid = kind >= 2;
+  const int mk = mfma_operand_type(kind);
+  const size_t es = mk == 0 ? 2 : (mk == 1 ? 4 : 1);
+  const int KP = kp_for(ix, kind, k, exclude);
+  const MfmaChunkPlan pl = plan_mfma_chunk(mk, resid, op.ks, m, m, KP, op.n_pad, device_cu_count(ix->device));
+  const size_t nq = size_t(pl.nq_pad), ch = size_t(pl.chunk);
+  // slots, split counts, thresholds, f32 queries, three norms, integer flags,
+  // failure list and thresholds, failure count
+  size_t b = pl.slots_bytes + 4 * size_t(pl.nsplit) * nq + 4 * nq + ch * size_t(ix->ld) * 4 + 24 * ch + 4 * ch +
+             12 * ch + 64;
+  if (resid) {
+    b += 4 * size_t(op.L) * nq + 4 * size_t(op.L) * ch;  // per-list terms and ||q - c||^2
+    if (kind == 3) b += 2 * size_t(op.L) * nq * size_t(op.kp) + 4 * size_t(op.L) * nq;  // E4M3 fragments
+  } else {
+    b += nq * size_t(op.kp) * es;  // query operand
+    const SeedPlan sp = plan_seed(mk, op.ks, false, false, ix->opts.flags, op.n_pad, pl.rows_tile, KP);
+    if (sp.ntiles > 0) b += 4 * nq * size_t(sp.parts) * 2 * SEED_GROUPS;
+  }
+  return (int64_t)b;
+}
+
 bool mfma_supported(const fuel_amd_index* ix, int kind, int k, int exclude) {
   if (ix->dtype != FUEL_AMD_F32) return false;
   const int ks = mfma_ksteps_bucket(mfma_operand_type(kind), ix->d);

@@ -115,8 +115,13 @@ namespace {
 // the certification levels, the outputs and the candidate slots (the scan
 // spreads a small batch over many splits, so slots take at least ~64 MiB).
 int64_t batch_bytes(const fuel_amd_server* s, int64_t m, int k) {
+  // query rows and outputs (device and pinned staging), an allowance per query
+  // for the certification chain's levels, and the first level's workspaces
+  // (candidate slots dominate: ~130 MB for 64 queries against 10M rows)
   const int64_t per_q = s->ix->ld * int64_t(s->es) * 3 + int64_t(k) * (8 + int64_t(s->es)) + (int64_t(16) << 10);
-  return m * per_q + (int64_t(64) << 20);
+  int mode = s->ix->opts.mode;
+  if (mode == FUEL_AMD_MODE_AUTO) mode = s->ix->auto_mode.load(std::memory_order_acquire);
+  return m * per_q + (int64_t(64) << 20) + mfma_level0_bytes(s->ix, knn_kind(s->ix, mode, k, 0), m, k, 0);
 }
 
 int32_t run_batch(fuel_amd_server* s, Worker* w, const std::vector<Request*>& batch, int64_t bm, int k) {
